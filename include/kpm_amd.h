@@ -1,0 +1,173 @@
+/*
+ * kpm_amd.h -- C ABI of the MI355X (gfx950) Chebyshev moment engine.
+ *
+ * Every buffer argument named d_* or documented as "device" is a device
+ * pointer that the caller owns (in the Python host these come from
+ * torch.Tensor.data_ptr()); the library only borrows it for the duration of
+ * the call.  Plans own their workspaces (block partials, counters, the step
+ * counter) and nothing else.  All entry points return KPM_OK (0) on success
+ * and a nonzero status otherwise; kpm_last_error() then describes the failure
+ * (thread-local).  Streams are hipStream_t passed as void*; NULL is the
+ * default stream.  No entry point synchronises the stream unless stated.
+ *
+ * Complex values are interleaved (re, im): KPM_C64 = 2 x float,
+ * KPM_C128 = 2 x double.  Vector blocks are row-major N x W ("W" = block
+ * width = number of random vectors handled together), so one matrix row of a
+ * block is W contiguous complex numbers.
+ *
+ * The reference this replaces is kwant's pure-Python kpm module; its only
+ * "FFI" is numpy/scipy, so each entry point below names the kwant/kpm.py
+ * lines whose numerical work it performs.
+ */
+#ifndef KPM_AMD_H
+#define KPM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KPM_OK 0
+#define KPM_ERR_ARG 1         /* bad argument: maps to ValueError      */
+#define KPM_ERR_DEVICE 2      /* HIP runtime failure: RuntimeError      */
+#define KPM_ERR_UNSUPPORTED 3 /* no kernel variant for this shape       */
+#define KPM_ERR_ALLOC 4       /* device allocation failed               */
+
+#define KPM_C64 1
+#define KPM_C128 2
+
+/* dots_mode of kpm_cheb_steps / kpm_block_dots */
+#define KPM_DOTS_NONE 0     /* recursion only (Correlator expansions)         */
+#define KPM_DOTS_DOUBLING 1 /* <a_n|a_n>, <a_{n+1}|a_n> per column            */
+#define KPM_DOTS_PROJECT 2  /* <w|a_n>, <w|a_{n+1}> per column (operator path) */
+
+typedef void* kpm_stream_t;
+
+/* A CSR matrix in device memory (borrowed).  indptr has n_rows+1 int32
+ * entries, indices nnz int32 entries, values nnz complex entries of
+ * value_type.  This is the csr_matrix that kwant/kpm.py:181 builds from the
+ * int32-indexed COO of kwant/_system.pyx:118-119. */
+typedef struct {
+  int32_t n_rows;
+  int32_t n_cols;
+  int64_t nnz;
+  const int32_t* indptr;
+  const int32_t* indices;
+  const void* values;
+  int32_t value_type;
+} kpm_csr;
+
+typedef struct kpm_plan kpm_plan;
+
+/* Last error message of the calling thread ("" if none). */
+const char* kpm_last_error(void);
+
+/* ABI version (major*100 + minor). */
+int kpm_abi_version(void);
+
+/* Compute units and gfx architecture number (e.g. 950) of a device. */
+int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch);
+
+/* Number of recursion kernel variants for a (width, value_type) pair, and
+ * the launch shape of each: lanes per row, complex values per lane, waves per
+ * workgroup, rows per workgroup.  Used by the host autotuner. */
+int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count);
+int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant,
+                     int32_t* lanes_per_row, int32_t* vec,
+                     int32_t* waves, int32_t* rows_per_block);
+
+/* Create a recursion plan for an N x width vector block whose rows are split
+ * into n_segments independent systems (segment s owns rows
+ * [seg_row_begin[s], seg_row_begin[s+1]) -- host array).  One segment is the
+ * plain SpectralDensity case; several are a batched disorder ensemble whose
+ * block-diagonal matrix is passed as one CSR.  Each segment gets its own
+ * rescaling (a, b) and its own moments.  variant selects the kernel launch
+ * shape (see kpm_variant_info); -1 picks the default.
+ * Replaces the per-system setup of kwant/kpm.py:163-252. */
+int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type,
+                    int32_t n_segments, const int32_t* seg_row_begin,
+                    int32_t variant, kpm_plan** out);
+int kpm_plan_destroy(kpm_plan* plan);
+
+/* Rescaling of each segment: host array of n_segments (a, b) pairs so the
+ * recursion applies (H v - b v) / a  (kwant/kpm.py:1174-1175, 1182-1183).
+ * Synchronous upload. */
+int kpm_plan_set_scale(kpm_plan* plan, const double* h_ab);
+
+/* Device step counter n: the next kpm_cheb_steps launch computes a_{n+1}.
+ * set: asynchronous on stream.  get: synchronises the stream. */
+int kpm_plan_set_step(kpm_plan* plan, int32_t n, kpm_stream_t stream);
+int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream);
+
+/* Run n_steps Chebyshev steps of the three-term recursion over the vector
+ * block (kwant/kpm.py:436-484 with the rescaled matvec of :1182-1183):
+ *   n == 0 : a_1     = (H a_0 - b a_0) / a
+ *   n >= 1 : a_{n+1} = 2 (H a_n - b a_n) / a - a_{n-1}
+ * a_k lives in ring slot (k mod slots): device buffer `ring`, slot k at
+ * ring + k*slot_stride complex elements, each slot an n_rows x width block.
+ * With slots == 2 the update is in place (a_{n+1} overwrites a_{n-1}).
+ * The step index n is read from the plan's device counter and incremented by
+ * each launch, so consecutive launches carry identical arguments.
+ *
+ * dots_mode KPM_DOTS_DOUBLING writes, per step, segment and column c,
+ *   d_dots[((n - out_base) * n_segments + s) * width + c] = 4 doubles
+ *   { <a_n|a_n>, 0, <a_{n+1}|a_n> (re, im) }
+ * (the two vdot()s of kwant/kpm.py:444-445, 470-473; at n == 0 the second
+ * is <a_1|a_0>, the conjugate of mu_1).  KPM_DOTS_PROJECT writes
+ * { <w|a_n>, <w|a_{n+1}> } with w the device block `w` (the matrix-operator
+ * moments vdot(a_0, O a_n) of kwant/kpm.py:193-195, 484 with w = O^H a_0).
+ * Sums are FP64 and deterministic (fixed order), so repeated runs and
+ * resumed runs are bitwise identical.  Steps with (n - out_base) outside
+ * [0, out_cap) write no dots and set an error flag reported by the next
+ * kpm_plan_get_step. */
+int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring,
+                   int64_t slot_stride, int32_t slots, const void* w,
+                   double* d_dots, int32_t out_base, int32_t out_cap,
+                   int32_t n_steps, int32_t dots_mode, kpm_stream_t stream);
+
+/* Dots of one block without a matrix product: per segment and column,
+ * DOUBLING: { <x|x>, 0, 0, 0 }; PROJECT: { <w|x>, 0, 0 }.  Written to
+ * d_out[s * width + c] (4 doubles).  This is the odd-moment tail
+ * vdot(alpha_next, alpha_next) of kwant/kpm.py:474-477. */
+int kpm_block_dots(kpm_plan* plan, const void* x, const void* w,
+                   double* d_out, int32_t dots_mode, kpm_stream_t stream);
+
+/* y = scale * op x for an n_rows x width block (op: n_rows x n_cols CSR of the
+ * same value type; x: n_cols x width; y: n_rows x width), for n_slots
+ * consecutive slots (x + k*x_stride, y + k*y_stride).  This is the operator
+ * application op.dot(v) of kwant/kpm.py:1098-1099, used for
+ * Psi = op2 T_n(H)|r> (kwant/kpm.py:708-718) and Omega_0 = op1|r>
+ * (kwant/kpm.py:697-706). */
+int kpm_spmm(const kpm_csr* op, const void* x, int64_t x_stride, void* y,
+             int64_t y_stride, int32_t width, int32_t n_slots,
+             kpm_stream_t stream);
+
+/* Correlator moment matrix (kwant/kpm.py:685-695):
+ *   mu[m, n] += sum_k conj(A[m, k]) * B[n, k]
+ * A: m_rows x k complex, row stride lda (complex elements); B: n_rows x k,
+ * row stride ldb; mu: FP64 complex m_rows x n_rows, row stride ldc,
+ * accumulated in place.  precision:
+ *   KPM_GEMM_FP64  : complex128 inputs, FP64 MFMA
+ *   KPM_GEMM_FP32  : complex64 inputs, FP32 MFMA, FP32 partial sums over
+ *                    k-chunks of k_chunk, then FP64 accumulation
+ *   KPM_GEMM_FP32X3: complex128 inputs split hi+lo into FP32, three FP32 MFMA
+ *                    products (hi*hi + hi*lo + lo*hi) = split-precision
+ *                    correction, FP64 accumulation across k-chunks
+ * k is split into slices to fill the GPU when a workspace is given: work is a
+ * device FP64 buffer of work_elems doubles (2 * m_rows * n_rows per slice;
+ * NULL = no split).  Slices are added in order (deterministic).
+ * Asynchronous on stream. */
+#define KPM_GEMM_FP64 0
+#define KPM_GEMM_FP32 1
+#define KPM_GEMM_FP32X3 2
+int kpm_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb,
+                  double* mu, int64_t ldc, int32_t m_rows, int32_t n_rows,
+                  int64_t k, int32_t precision, int32_t k_chunk,
+                  double* work, int64_t work_elems, kpm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KPM_AMD_H */

@@ -1,0 +1,8 @@
+"""kwant_amd: kwant's KPM moment engine rebuilt for AMD MI355X (gfx950).
+
+``kwant_amd.kpm`` is a drop-in for ``kwant.kpm``; the numerical work runs in
+the HIP kernels of ``libkpm_amd.so`` (sources in ``kwant_amd/csrc``).
+"""
+from . import kpm  # noqa: F401
+
+__all__ = ['kpm']

@@ -1,0 +1,354 @@
+"""ctypes binding of libkpm_amd.so and the device-side objects built on it.
+
+Device memory and streams come from PyTorch-ROCm (tensors are the buffers,
+``torch.cuda.current_stream()`` the stream); every computation runs in the
+HIP kernels of ``kwant_amd/csrc``.  There is no CPU fallback: without the
+built library or without a visible GPU every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+import scipy.sparse
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkpm_amd.so")
+
+KPM_OK = 0
+KPM_ERR_ARG = 1
+KPM_C64 = 1
+KPM_C128 = 2
+DOTS_NONE = 0
+DOTS_DOUBLING = 1
+DOTS_PROJECT = 2
+GEMM_FP64 = 0
+GEMM_FP32 = 1
+GEMM_FP32X3 = 2
+
+# Exported symbols, in the order include/kpm_amd.h declares them.
+EXPORTS = (
+    "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
+    "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
+    "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
+    "kpm_spmm", "kpm_corr_gemm",
+)
+
+
+class KpmCsr(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", ctypes.c_int32),
+        ("n_cols", ctypes.c_int32),
+        ("nnz", ctypes.c_int64),
+        ("indptr", ctypes.c_void_p),
+        ("indices", ctypes.c_void_p),
+        ("values", ctypes.c_void_p),
+        ("value_type", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libkpm_amd.so (after torch, so that both share one HIP runtime)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "libkpm_amd.so is not built (expected at %s); run "
+            "`python -m kwant_amd.build`" % path)
+    import torch  # noqa: F401  -- loads torch's HIP runtime first
+    lib = ctypes.CDLL(path)
+    i32, i64, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    pi32 = ctypes.POINTER(ctypes.c_int32)
+    sig = {
+        "kpm_last_error": (ctypes.c_char_p, []),
+        "kpm_abi_version": (ctypes.c_int, []),
+        "kpm_device_info": (ctypes.c_int, [i32, pi32, pi32]),
+        "kpm_variant_count": (ctypes.c_int, [i32, i32, pi32]),
+        "kpm_variant_info": (ctypes.c_int, [i32, i32, i32, pi32, pi32, pi32, pi32]),
+        "kpm_plan_create": (ctypes.c_int, [i32, i32, i32, i32, pi32, i32,
+                                           ctypes.POINTER(vp)]),
+        "kpm_plan_destroy": (ctypes.c_int, [vp]),
+        "kpm_plan_set_scale": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double)]),
+        "kpm_plan_set_step": (ctypes.c_int, [vp, i32, vp]),
+        "kpm_plan_get_step": (ctypes.c_int, [vp, pi32, vp]),
+        "kpm_cheb_steps": (ctypes.c_int, [vp, ctypes.POINTER(KpmCsr), vp, i64, i32, vp,
+                                          vp, i32, i32, i32, i32, vp]),
+        "kpm_block_dots": (ctypes.c_int, [vp, vp, vp, vp, i32, vp]),
+        "kpm_spmm": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, i64, vp, i64, i32, i32, vp]),
+        "kpm_corr_gemm": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
+                                         i32, vp, i64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc == KPM_OK:
+        return
+    msg = load_library().kpm_last_error().decode(errors="replace")
+    if rc == KPM_ERR_ARG:
+        raise ValueError("%s: %s" % (what, msg))
+    raise RuntimeError("%s failed (status %d): %s" % (what, rc, msg))
+
+
+def torch_device():
+    """The GPU the engine runs on (torch's current device)."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "kwant_amd runs its KPM kernels on an AMD GPU (ROCm) and none is "
+            "visible to PyTorch; there is no CPU fallback")
+    load_library()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def complex_dtype(precision):
+    import torch
+    if precision in ("c128", "complex128", np.complex128):
+        return torch.complex128, KPM_C128
+    if precision in ("c64", "complex64", np.complex64):
+        return torch.complex64, KPM_C64
+    raise ValueError("precision must be 'complex128' or 'complex64'")
+
+
+def variant_count(width, value_type):
+    """Number of recursion kernel launch shapes for (width, value type)."""
+    n = ctypes.c_int32()
+    check(load_library().kpm_variant_count(width, value_type, ctypes.byref(n)),
+          "kpm_variant_count")
+    return n.value
+
+
+def variant_info(width, value_type, variant):
+    """(lanes per row, complex values per lane, waves, rows per workgroup)."""
+    vals = [ctypes.c_int32() for _ in range(4)]
+    check(load_library().kpm_variant_info(width, value_type, variant,
+                                          *[ctypes.byref(v) for v in vals]),
+          "kpm_variant_info")
+    return tuple(v.value for v in vals)
+
+
+def device_info():
+    lib = load_library()
+    cu, arch = ctypes.c_int32(), ctypes.c_int32()
+    check(lib.kpm_device_info(0, ctypes.byref(cu), ctypes.byref(arch)), "kpm_device_info")
+    return cu.value, arch.value
+
+
+class DeviceCSR:
+    """A CSR matrix resident on the GPU (int32 indices, complex values).
+
+    Keeps kwant's CSR exactly as given (entry order within rows included),
+    so the SpMM sums each row in the order scipy's csr_matvec does."""
+
+    def __init__(self, matrix, precision="c128"):
+        import torch
+        dev = torch_device()
+        csr = scipy.sparse.csr_matrix(matrix)
+        if csr.nnz >= 2 ** 31 or csr.shape[0] >= 2 ** 31:
+            raise ValueError("matrix too large for int32 CSR indices")
+        self.shape = csr.shape
+        self.nnz = int(csr.nnz)
+        self.precision = precision
+        cdt, self.value_type = complex_dtype(precision)
+        np_cdt = np.complex128 if self.value_type == KPM_C128 else np.complex64
+        self.indptr = torch.from_numpy(np.ascontiguousarray(csr.indptr, dtype=np.int32)).to(dev)
+        self.indices = torch.from_numpy(
+            np.ascontiguousarray(csr.indices, dtype=np.int32)).to(dev)
+        vals = np.ascontiguousarray(csr.data, dtype=np.complex128).astype(np_cdt)
+        if vals.size == 0:
+            vals = np.zeros(1, dtype=np_cdt)
+        self.values = torch.from_numpy(vals).to(dev)
+        assert self.values.dtype == cdt
+        self.desc = KpmCsr(int(csr.shape[0]), int(csr.shape[1]), self.nnz,
+                           self.indptr.data_ptr(), self.indices.data_ptr(),
+                           self.values.data_ptr(), self.value_type)
+
+    @classmethod
+    def from_device_arrays(cls, shape, indptr, indices, values, precision):
+        """Wrap CSR arrays that already live on the GPU (generated there)."""
+        self = cls.__new__(cls)
+        cdt, self.value_type = complex_dtype(precision)
+        if values.dtype != cdt:
+            raise ValueError("values dtype does not match precision")
+        self.shape = tuple(shape)
+        self.nnz = int(indices.numel())
+        self.precision = precision
+        self.indptr, self.indices, self.values = indptr, indices, values
+        self.desc = KpmCsr(int(shape[0]), int(shape[1]), self.nnz, indptr.data_ptr(),
+                           indices.data_ptr(), values.data_ptr(), self.value_type)
+        return self
+
+    def spmm(self, x, y, width, n_slots=1, x_stride=0, y_stride=0):
+        """y[slot] = self @ x[slot] for row-major blocks of ``width`` columns."""
+        lib = load_library()
+        check(lib.kpm_spmm(ctypes.byref(self.desc), x.data_ptr(), x_stride, y.data_ptr(),
+                           y_stride, width, n_slots, stream_handle()), "kpm_spmm")
+
+
+class Plan:
+    """A recursion plan (kernel shape, row partition into segments, workspace)."""
+
+    def __init__(self, n_rows, width, value_type, seg_rows=None, variant=-1):
+        lib = load_library()
+        if seg_rows is None:
+            seg_rows = [0, n_rows]
+        seg = np.ascontiguousarray(seg_rows, dtype=np.int32)
+        self.n_rows, self.width, self.value_type = n_rows, width, value_type
+        self.n_segments = len(seg) - 1
+        handle = ctypes.c_void_p()
+        check(lib.kpm_plan_create(n_rows, width, value_type, self.n_segments,
+                                  seg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                  variant, ctypes.byref(handle)), "kpm_plan_create")
+        self.handle = handle
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            try:
+                _lib.kpm_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def set_scale(self, ab):
+        ab = np.ascontiguousarray(np.asarray(ab, dtype=np.float64).reshape(-1))
+        check(load_library().kpm_plan_set_scale(
+            self.handle, ab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+            "kpm_plan_set_scale")
+
+    def set_step(self, n):
+        check(load_library().kpm_plan_set_step(self.handle, n, stream_handle()),
+              "kpm_plan_set_step")
+
+    def get_step(self):
+        n = ctypes.c_int32()
+        check(load_library().kpm_plan_get_step(self.handle, ctypes.byref(n),
+                                               stream_handle()), "kpm_plan_get_step")
+        return n.value
+
+    def steps(self, mat, ring, slot_stride, slots, n_steps, mode, dots=None,
+              out_base=0, w=None):
+        out_cap = 0 if dots is None else int(dots.shape[0])
+        check(load_library().kpm_cheb_steps(
+            self.handle, ctypes.byref(mat.desc), ring.data_ptr(), slot_stride, slots,
+            None if w is None else w.data_ptr(),
+            None if dots is None else dots.data_ptr(), out_base, out_cap, n_steps, mode,
+            stream_handle()), "kpm_cheb_steps")
+
+    def block_dots(self, x, out, mode, w=None):
+        check(load_library().kpm_block_dots(
+            self.handle, x.data_ptr(), None if w is None else w.data_ptr(), out.data_ptr(),
+            mode, stream_handle()), "kpm_block_dots")
+
+
+def pow2_widths(count, max_width):
+    """Split ``count`` vectors into power-of-two block widths <= max_width."""
+    out = []
+    while count > 0:
+        w = max_width
+        while w > count:
+            w //= 2
+        out.append(w)
+        count -= w
+    return out
+
+
+class BlockRecursion:
+    """Chebyshev recursion of one N x W block of vectors on the GPU.
+
+    The ring holds a_{n-1}, a_n in two slots (in-place update), or every
+    a_n when ``store_all`` (Correlator expansions)."""
+
+    def __init__(self, mat, ab, vectors, width, mode, w=None, store_slots=None,
+                 variant=-1, plan=None):
+        import torch
+        dev = torch_device()
+        n = mat.shape[0]
+        self.mat, self.width, self.mode, self.n_rows = mat, width, mode, n
+        self.cdt, self.vt = complex_dtype(mat.precision)
+        self.plan = plan if plan is not None else Plan(n, width, self.vt, variant=variant)
+        self.plan.set_scale([ab[0], ab[1]])
+        self.slots = 2 if store_slots is None else int(store_slots)
+        if self.slots < 2:
+            raise ValueError("a ring needs at least two slots")
+        self.ring = torch.zeros((self.slots, n, width), dtype=self.cdt, device=dev)
+        self.ring[0, :, :vectors.shape[1]] = vectors
+        self.w = w
+        self.n = 0  # steps done: ring holds a_0 .. a_n
+        self.plan.set_step(0)
+
+    @property
+    def slot_stride(self):
+        return self.n_rows * self.width
+
+    def step(self, n_steps, want_dots=True):
+        """Advance ``n_steps``; returns the (n_steps, W, 4) FP64 dots on host."""
+        import torch
+        if n_steps <= 0:
+            return np.zeros((0, self.width, 4))
+        if self.n + n_steps >= self.slots and self.slots > 2:
+            raise ValueError("ring too small for the requested moments")
+        mode = self.mode if want_dots else DOTS_NONE
+        dots = None
+        if mode != DOTS_NONE:
+            dots = torch.empty((n_steps, 1, self.width, 4), dtype=torch.float64,
+                               device=self.ring.device)
+        self.plan.steps(self.mat, self.ring, self.slot_stride, self.slots, n_steps, mode,
+                        dots=dots, out_base=self.n, w=self.w)
+        self.n += n_steps
+        if dots is None:
+            return None
+        return dots[:, 0].cpu().numpy()
+
+    def slot(self, k):
+        return self.ring[k % self.slots]
+
+    def norms(self):
+        """(W, 4) dots of the current a_n alone (odd-moment tail)."""
+        import torch
+        out = torch.empty((1, self.width, 4), dtype=torch.float64, device=self.ring.device)
+        self.plan.block_dots(self.slot(self.n), out, self.mode, w=self.w)
+        return out[0].cpu().numpy()
+
+    def last_two(self, col):
+        """(a_{n-1}, a_n) of column ``col`` as host complex128 arrays."""
+        return (self.slot(self.n - 1)[:, col].cpu().numpy().astype(np.complex128),
+                self.slot(self.n)[:, col].cpu().numpy().astype(np.complex128))
+
+
+def corr_gemm(a, b, mu, precision, k_chunk=256, work=None):
+    """mu += conj(a) @ b.T on the GPU (a: m x k, b: n x k complex tensors with
+    unit stride along k; mu: m x n complex128)."""
+    import torch
+    m, k = a.shape
+    n = b.shape[0]
+    if b.shape[1] != k or tuple(mu.shape) != (m, n):
+        raise ValueError("corr_gemm shape mismatch")
+    if mu.dtype != torch.complex128:
+        raise ValueError("mu must be complex128")
+    want = torch.complex64 if precision == GEMM_FP32 else torch.complex128
+    if a.dtype != want or b.dtype != want:
+        raise ValueError("operand dtype does not match the GEMM precision")
+    if a.stride(1) != 1 or b.stride(1) != 1 or mu.stride(1) != 1:
+        raise ValueError("operands must be contiguous along k")
+    if work is None:
+        tiles = -(-m // 64) * -(-n // 64)
+        slices = max(1, min(-(-2048 // tiles), -(-k // 4096)))
+        work = torch.empty(2 * m * n * slices, dtype=torch.float64, device=mu.device)
+    check(load_library().kpm_corr_gemm(
+        a.data_ptr(), a.stride(0), b.data_ptr(), b.stride(0), mu.data_ptr(), mu.stride(0),
+        m, n, k, precision, k_chunk, work.data_ptr(), work.numel(), stream_handle()),
+        "kpm_corr_gemm")

@@ -1,0 +1,320 @@
+"""Kubo-Bastin Correlator and conductivity with GPU expansions and a GPU GEMM.
+
+Mirrors kwant/kpm.py:493-847.  For every block of random vectors r:
+
+* the "right" expansion T_n(H~)|r> (kwant/kpm.py:574) keeps all its
+  Chebyshev vectors in a GPU ring of num_moments slots;
+* Psi_n = op2 T_n(H~)|r> is one batched SpMM over those slots
+  (kwant/kpm.py:708-718);
+* the "left" expansion T_m(H~) op1|r> (kwant/kpm.py:588-603) starts from
+  op1|r> (SpMM) with the same (a, b) -- recomputed through the shrunken
+  bounds exactly as kwant/kpm.py:578-579 does;
+* mu_mn += sum_i conj(Omega_m[i]) Psi_n[i] is a GPU GEMM over
+  K = N x block width (kwant/kpm.py:685-695), FP64 by default
+  (``gemm_precision='fp32'`` runs the complex64 path on the FP32 matrix cores
+  with FP64 accumulation across k-chunks).
+
+The integral factor (kwant/kpm.py:720-760) is evaluated in its GEMM form,
+factor(x_k) = e+_k^T mu~ T_k + T_k^T mu~ conj(e+_k), and the final
+simpson integral (kwant/kpm.py:608-630) is FP64 on the host.
+"""
+import numpy as np
+from scipy.integrate import simpson
+from scipy.sparse import coo_matrix, csr_matrix
+
+from . import _engine as eng
+from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _is_local_operator,
+                  _is_system, _rescale, _scale_from_bounds, ensure_rng, fermi_distribution,
+                  jackson_kernel, SAMPLING, _chebyshev_nodes)
+from collections.abc import Iterable
+import math
+
+_GEMM = {'fp64': eng.GEMM_FP64, 'fp32': eng.GEMM_FP32, 'fp32x3': eng.GEMM_FP32X3}
+
+
+def _normalize_operator(op, params):
+    """None -> identity, kwant operator -> op.act, callable, or a matrix
+    (kwant/kpm.py:1088-1103).  Returns ('id'|'matrix'|'callable', payload)."""
+    if op is None:
+        return 'id', None
+    if _is_local_operator(op):
+        return 'callable', op.bind(params=params).act
+    if callable(op):
+        return 'callable', op
+    if hasattr(op, 'dot'):
+        return 'matrix', csr_matrix(op)
+    raise TypeError("The operators must have a '.dot' attribute or must be callable.")
+
+
+class _Expansion:
+    """Inputs of one SpectralDensity of kwant/kpm.py:163-241 without its
+    moments: matrix on the GPU, (a, b), num_moments and the vector factory."""
+
+    def __init__(self, hamiltonian, params=None, num_vectors=10, num_moments=None,
+                 energy_resolution=None, vector_factory=None, bounds=None, eps=0.05,
+                 rng=None, kernel=None, mean=True, accumulate_vectors=True,
+                 precision='complex128'):
+        if num_moments and energy_resolution:
+            raise TypeError("either 'num_moments' or 'energy_resolution' must be provided.")
+        self.eps = eps
+        ham = _as_csr(hamiltonian, params)
+        self.csr = ham
+        rng = ensure_rng(rng)
+        self._v0 = np.exp(2j * np.pi * rng.random_sample(ham.shape[0]))
+        if eps <= 0:
+            raise ValueError("'eps' must be positive")
+        self.hamiltonian, (self._a, self._b) = _rescale(ham, eps, self._v0, bounds, precision)
+        self.dmat = self.hamiltonian._dmat
+        if energy_resolution:
+            num_moments = math.ceil((1.6 * self._a) / energy_resolution)
+        elif num_moments is None:
+            num_moments = 100
+        if num_moments <= 0 or num_moments != int(num_moments):
+            raise ValueError("'num_moments' must be a positive integer")
+        self.num_moments = num_moments
+        if vector_factory is None:
+            self._vector_factory = _VectorFactory(
+                RandomVectors(ham, rng=rng), num_vectors=num_vectors,
+                accumulate=accumulate_vectors)
+        else:
+            if not isinstance(vector_factory, Iterable):
+                raise TypeError('vector_factory must be iterable')
+            try:
+                len(vector_factory)
+            except TypeError:
+                if num_vectors is None:
+                    raise ValueError('num_vectors must be provided if'
+                                     'vector_factory has no length.')
+            self._vector_factory = _VectorFactory(
+                vector_factory, num_vectors=num_vectors, accumulate=accumulate_vectors)
+        self.kernel = kernel if kernel is not None else jackson_kernel
+
+    @property
+    def energies(self):
+        return self._a * _chebyshev_nodes(SAMPLING * self.num_moments) + self._b
+
+
+class Correlator:
+    """Response of the correlation between two operators (kwant/kpm.py:493-555).
+    ``operator1`` acts to the right as O_a|r>."""
+
+    def __init__(self, hamiltonian, operator1=None, operator2=None, **kwargs):
+        params = kwargs.get('params')
+        self.mean = kwargs.get('mean', True)
+        self._accumulate = kwargs.get('accumulate_vectors', False)
+        kwargs['accumulate_vectors'] = True
+        kwargs.pop('operator', None)
+        gp = kwargs.pop('gemm_precision', None)
+        precision = kwargs.get('precision', 'complex128')
+        if gp is None:
+            gp = 'fp64' if precision in ('complex128', 'c128') else 'fp32'
+        if gp not in _GEMM:
+            raise ValueError("gemm_precision must be 'fp64', 'fp32' or 'fp32x3'")
+        self._gemm_precision = gp
+        self._max_width = kwargs.pop('max_block_width', 16)
+        kwargs.pop('variant', None)
+        self._op1 = _normalize_operator(operator1, params)
+        self._op2 = _normalize_operator(operator2, params)
+
+        self._spectrum_R = _Expansion(hamiltonian, **kwargs)
+        spec = self._spectrum_R
+        self._a, self._b = spec._a, spec._b
+        shrunk = self._a * (1 - spec.eps / 2)
+        # the second expansion recomputes (a, b) from these bounds, as
+        # kwant/kpm.py:578-579, 597 make it do
+        self._aL, self._bL = _scale_from_bounds(self._b - shrunk, self._b + shrunk, spec.eps)
+        rng = ensure_rng(kwargs.get('rng'))
+        rng.random_sample(spec.csr.shape[0])  # the v0 draw of the second expansion
+        self.num_moments = spec.num_moments
+        self.num_vectors = 0
+        self._dev_ops = {}
+        self._mu_sum = None
+        self._mu_list = []
+        self._add_vectors(spec._vector_factory.num_vectors)
+        self._build_integral_factor()
+
+    # -- public API (kwant/kpm.py:608-683) ---------------------------------
+    def __call__(self, mu=0, temperature=0):
+        e = self.energies
+        x = (e - self._b) / self._a
+        f = fermi_distribution(e, mu, temperature)
+        integrand = np.divide(f, (1 - x ** 2) ** 2)
+        integrand = np.multiply(integrand, self._integral_factor)
+        integral = simpson(integrand, x=x)
+        return 2 * 4 ** 2 / ((2 * self._a) ** 2) * integral
+
+    @property
+    def energies(self):
+        return self._spectrum_R.energies
+
+    def add_moments(self, num_moments=None, *, energy_resolution=None):
+        """Extend both expansions (kwant/kpm.py:636-661); the moment matrix is
+        recomputed from the saved vectors."""
+        spec = self._spectrum_R
+        if not ((num_moments is None) ^ (energy_resolution is None)):
+            raise TypeError("either 'num_moments' or 'energy_resolution' must be provided.")
+        if energy_resolution:
+            if energy_resolution <= 0:
+                raise ValueError("'energy_resolution' must be positive")
+            if spec._a * 1.6 / spec.num_moments < energy_resolution:
+                raise ValueError('Energy resolution is already smaller '
+                                 'than the requested resolution')
+            num_moments = math.ceil((1.6 * spec._a) / energy_resolution)
+        if num_moments is None or num_moments <= 0 or num_moments != int(num_moments):
+            raise ValueError("'num_moments' must be a positive integer")
+        if not self._accumulate:
+            raise ValueError("Cannot increase the number of moments if "
+                             "'accumulate_vectors' is 'False'.")
+        spec.num_moments += num_moments
+        self.num_moments = spec.num_moments
+        count = self.num_vectors
+        self.num_vectors = 0
+        self._mu_sum = None
+        self._mu_list = []
+        self._add_vectors(count)
+        self._build_integral_factor()
+
+    def add_vectors(self, num_vectors=None):
+        """Expand more vectors (kwant/kpm.py:663-683)."""
+        self._spectrum_R._vector_factory.add_vectors(num_vectors)
+        self._add_vectors(num_vectors)
+        self._build_integral_factor()
+
+    # -- GPU work ------------------------------------------------------------
+    def _device_op(self, key, kind, payload):
+        if kind != 'matrix':
+            return None
+        if key not in self._dev_ops:
+            self._dev_ops[key] = eng.DeviceCSR(payload, self._spectrum_R.dmat.precision)
+        return self._dev_ops[key]
+
+    def _apply(self, key, op, src, dst, width, n_slots):
+        """dst[slot] = op src[slot] on the GPU (matrix), on the host
+        (callable), or a copy (identity)."""
+        import torch
+        kind, payload = op
+        if kind == 'id':
+            dst.copy_(src)
+        elif kind == 'matrix':
+            self._device_op(key, kind, payload).spmm(
+                src, dst, width, n_slots=n_slots, x_stride=src[0].numel(),
+                y_stride=dst[0].numel())
+        else:
+            host = src.cpu().numpy().astype(np.complex128)
+            out = np.empty_like(host)
+            for s in range(n_slots):
+                for c in range(width):
+                    out[s, :, c] = payload(host[s, :, c])
+            dst.copy_(torch.from_numpy(out).to(dst.device).to(dst.dtype))
+
+    def _add_vectors(self, count):
+        import torch
+        spec = self._spectrum_R
+        factory = spec._vector_factory
+        dmat = spec.dmat
+        dev = eng.torch_device()
+        cdt, _ = eng.complex_dtype(dmat.precision)
+        n, m_count = dmat.shape[0], self.num_moments
+        r0 = self.num_vectors
+        widths = [1] * count if not self.mean else eng.pow2_widths(count, self._max_width)
+        start = r0
+        for width in widths:
+            vecs = [factory[r] for r in range(start, start + width)]
+            start += width
+            host = np.stack([np.asarray(v) for v in vecs], axis=1).astype(np.complex128)
+            block = torch.from_numpy(np.ascontiguousarray(host)).to(dev).to(cdt)
+            # right expansion: all T_n(H~)|r>
+            rec_r = eng.BlockRecursion(dmat, (self._a, self._b), block, width,
+                                       eng.DOTS_NONE, store_slots=m_count)
+            if m_count > 1:
+                rec_r.step(m_count - 1, want_dots=False)
+            psi = torch.empty_like(rec_r.ring)
+            self._apply('op2', self._op2, rec_r.ring, psi, width, m_count)
+            del rec_r
+            # left expansion from op1|r>
+            start_l = torch.empty_like(block)
+            self._apply('op1', self._op1, block.unsqueeze(0), start_l.unsqueeze(0), width, 1)
+            rec_l = eng.BlockRecursion(dmat, (self._aL, self._bL), start_l, width,
+                                       eng.DOTS_NONE, store_slots=m_count)
+            if m_count > 1:
+                rec_l.step(m_count - 1, want_dots=False)
+            mu = torch.zeros((m_count, m_count), dtype=torch.complex128, device=dev)
+            k = n * width
+            eng.corr_gemm(rec_l.ring.reshape(m_count, k), psi.reshape(m_count, k), mu,
+                          _GEMM[self._gemm_precision])
+            del rec_l, psi
+            if self.mean:
+                self._mu_sum = mu if self._mu_sum is None else self._mu_sum + mu
+            else:
+                self._mu_list.append(mu.cpu().numpy())
+        self.num_vectors = r0 + count
+        if self.mean:
+            self.moments_matrix = self._mu_sum.cpu().numpy() / self.num_vectors
+        else:
+            self.moments_matrix = np.array(self._mu_list)
+
+    def _build_integral_factor(self):
+        """sum_mn mu~_mn Gamma_mn(x_k) at the 2M nodes (kwant/kpm.py:720-760)
+        in GEMM form."""
+        m_count = self.num_moments
+        g = self._spectrum_R.kernel(np.ones(m_count))
+        g[0] /= 2
+        mu_k = np.outer(g, g) * self.moments_matrix
+        x = (self.energies - self._b) / self._a
+        self._integral_factor = integral_factor_gemm(mu_k, x)
+
+
+def integral_factor_gemm(mu_kernel, x):
+    """factor_k = sum_i e+_ki (mu T^T)_ik + sum_j conj(e+_kj) (T mu)_kj with
+    T_kn = cos(n arccos x_k) and e+_kn = (x_k - i n sqrt(1-x_k^2)) e^{i n arccos x_k};
+    equal to the per-energy loop of kwant/kpm.py:745-760."""
+    m_count = mu_kernel.shape[-1]
+    m = np.arange(m_count)
+    theta = np.arccos(x)[:, None] * m[None, :]
+    t = np.cos(theta)
+    e_plus = (x[:, None] - 1j * np.sqrt(1 - x ** 2)[:, None] * m[None, :]) * np.exp(1j * theta)
+    if mu_kernel.ndim == 2:
+        y = mu_kernel @ t.T                      # (M, 2M)
+        z = t @ mu_kernel                        # (2M, M)
+        return np.sum(e_plus * y.T, axis=1) + np.sum(np.conj(e_plus) * z, axis=1)
+    y = mu_kernel @ t.T                          # (R, M, 2M)
+    z = t[None] @ mu_kernel                      # (R, 2M, M)
+    return (np.sum(e_plus[None] * np.swapaxes(y, 1, 2), axis=2)
+            + np.sum(np.conj(e_plus)[None] * z, axis=2))
+
+
+def _velocity(hamiltonian, params, op_type, positions):
+    """Velocity operator i (x_j - x_i) H_ij on the pattern of H, or the given
+    operator (kwant/kpm.py:1042-1085)."""
+    directions = dict(x=0, y=1, z=2)
+    if _is_local_operator(op_type):
+        return op_type
+    if isinstance(op_type, str):
+        direction = directions[op_type]
+        if _is_system(hamiltonian):
+            op, norbs, _ = hamiltonian.hamiltonian_submatrix(
+                params=params, sparse=True, return_norb=True)
+            positions = np.vstack([[hamiltonian.pos(i)] * nb for i, nb in enumerate(norbs)])
+            op = coo_matrix(op)
+        else:
+            op = coo_matrix(hamiltonian, copy=True)
+        disp = positions[op.col] - positions[op.row]
+        if direction > disp.shape[1]:
+            raise ValueError("{} is not an allowed direction.".format(op_type))
+        op.data = op.data * (1j * disp[:, direction])
+        return op.tocsr()
+    try:
+        return csr_matrix(op_type)
+    except Exception:
+        raise ValueError("Velocity operator must be provided as a matrix, "
+                         "a kwant operator, or a direction.")
+
+
+def conductivity(hamiltonian, alpha='x', beta='x', positions=None, **kwargs):
+    """Kubo-Bastin conductivity tensor element (kwant/kpm.py:763-847)."""
+    if positions is None and not _is_system(hamiltonian):
+        raise ValueError("If 'hamiltonian' is a matrix, positions must be provided")
+    params = kwargs.get('params')
+    alpha = _velocity(hamiltonian, params, alpha, positions)
+    beta = _velocity(hamiltonian, params, beta, positions)
+    return Correlator(hamiltonian, operator1=alpha, operator2=beta, **kwargs)

@@ -1,0 +1,375 @@
+// C ABI of libkpm_amd: plans, argument validation, error reporting.
+//
+// The Python host (kwant_amd/_engine.py) binds these through ctypes; every
+// device buffer is a caller-owned torch tensor, passed as a raw pointer.
+
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "kpm_internal.h"
+
+namespace kpm {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return KPM_OK;
+  return fail(KPM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+__global__ void set_ctrl_kernel(int32_t* ctrl, int32_t n) {
+  ctrl[0] = n;
+  ctrl[1] = 0;
+}
+
+}  // namespace kpm
+
+using namespace kpm;
+
+struct kpm_plan {
+  int32_t n_rows = 0, width = 0, value_type = 0, nseg = 0;
+  KernelShape shape{};
+  int32_t nblocks = 0, nblocks_launch = 0, ngroups = 0;
+  int2* blk_rows = nullptr;
+  int32_t* blk_seg = nullptr;
+  int32_t* blk_grp = nullptr;
+  int32_t* grp_blk = nullptr;
+  int32_t* seg_grp = nullptr;
+  double* scale = nullptr;
+  double* part = nullptr;
+  double* gpart = nullptr;
+  unsigned* grp_cnt = nullptr;
+  unsigned* seg_cnt = nullptr;
+  unsigned* done_cnt = nullptr;
+  int32_t* ctrl = nullptr;
+  bool scale_set = false;
+};
+
+namespace {
+
+template <typename T>
+int dev_alloc(T** p, size_t count, const char* what) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(KPM_ERR_ALLOC, std::string("hipMalloc ") + what + ": " + hipGetErrorString(e));
+  }
+  return KPM_OK;
+}
+
+template <typename T>
+int upload(T* d, const std::vector<T>& h, const char* what) {
+  return check_hip(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), what);
+}
+
+void free_plan(kpm_plan* p) {
+  if (!p) return;
+  hipFree(p->blk_rows);
+  hipFree(p->blk_seg);
+  hipFree(p->blk_grp);
+  hipFree(p->grp_blk);
+  hipFree(p->seg_grp);
+  hipFree(p->scale);
+  hipFree(p->part);
+  hipFree(p->gpart);
+  hipFree(p->grp_cnt);
+  hipFree(p->seg_cnt);
+  hipFree(p->done_cnt);
+  hipFree(p->ctrl);
+  delete p;
+}
+
+int fill_params(const kpm_plan* plan, StepParams& p) {
+  p.blk_rows = plan->blk_rows;
+  p.blk_seg = plan->blk_seg;
+  p.blk_grp = plan->blk_grp;
+  p.grp_blk = plan->grp_blk;
+  p.seg_grp = plan->seg_grp;
+  p.scale = plan->scale;
+  p.part = plan->part;
+  p.gpart = plan->gpart;
+  p.grp_cnt = plan->grp_cnt;
+  p.seg_cnt = plan->seg_cnt;
+  p.done_cnt = plan->done_cnt;
+  p.ctrl = plan->ctrl;
+  p.nblocks = plan->nblocks;
+  p.nblocks_launch = plan->nblocks_launch;
+  p.nseg = plan->nseg;
+  p.xcd = kNumXcd;
+  return KPM_OK;
+}
+
+size_t elem_bytes(int value_type) { return value_type == KPM_C128 ? 16 : 8; }
+
+}  // namespace
+
+extern "C" {
+
+const char* kpm_last_error(void) { return g_last_error.c_str(); }
+
+int kpm_abi_version(void) { return 100; }
+
+int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch) {
+  hipDeviceProp_t prop;
+  int rc = check_hip(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (rc) return rc;
+  if (num_cu) *num_cu = prop.multiProcessorCount;
+  if (gfx_arch) {
+    int arch = 0;
+    if (sscanf(prop.gcnArchName, "gfx%x", &arch) == 1) {
+      // "gfx950" parses as hex 0x950; report the decimal-looking digits.
+      char buf[16];
+      snprintf(buf, sizeof(buf), "%x", arch);
+      arch = atoi(buf);
+    }
+    *gfx_arch = arch;
+  }
+  return KPM_OK;
+}
+
+int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count) {
+  if (!count) return fail(KPM_ERR_ARG, "count is NULL");
+  *count = variants_for(width, value_type, nullptr, 0);
+  return KPM_OK;
+}
+
+int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant, int32_t* lanes_per_row,
+                     int32_t* vec, int32_t* waves, int32_t* rows_per_block) {
+  KernelShape ks[16];
+  const int n = variants_for(width, value_type, ks, 16);
+  if (variant < 0 || variant >= n) return fail(KPM_ERR_ARG, "variant out of range");
+  if (lanes_per_row) *lanes_per_row = ks[variant].lpr;
+  if (vec) *vec = ks[variant].vec;
+  if (waves) *waves = ks[variant].waves;
+  if (rows_per_block) *rows_per_block = ks[variant].iters * ks[variant].waves * 64 / ks[variant].lpr;
+  return KPM_OK;
+}
+
+int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n_segments,
+                    const int32_t* seg_row_begin, int32_t variant, kpm_plan** out) {
+  if (!out) return fail(KPM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (value_type != KPM_C64 && value_type != KPM_C128)
+    return fail(KPM_ERR_ARG, "value_type must be KPM_C64 or KPM_C128");
+  if (n_rows <= 0) return fail(KPM_ERR_ARG, "n_rows must be positive");
+  if (n_segments <= 0 || !seg_row_begin) return fail(KPM_ERR_ARG, "need at least one segment");
+  if (seg_row_begin[0] != 0 || seg_row_begin[n_segments] != n_rows)
+    return fail(KPM_ERR_ARG, "segments must cover rows [0, n_rows)");
+  for (int s = 0; s < n_segments; ++s)
+    if (seg_row_begin[s + 1] <= seg_row_begin[s])
+      return fail(KPM_ERR_ARG, "every segment needs at least one row");
+  KernelShape ks[16];
+  const int nv = variants_for(width, value_type, ks, 16);
+  if (nv == 0) return fail(KPM_ERR_UNSUPPORTED, "block width must be a power of two <= 128 "
+                                                "(<= 64 for complex128)");
+  if (variant < 0) variant = 0;
+  if (variant >= nv) return fail(KPM_ERR_ARG, "variant out of range");
+
+  kpm_plan* p = new kpm_plan();
+  p->n_rows = n_rows;
+  p->width = width;
+  p->value_type = value_type;
+  p->nseg = n_segments;
+  p->shape = ks[variant];
+  const int rpb = p->shape.iters * p->shape.waves * 64 / p->shape.lpr;
+
+  std::vector<int2> blk_rows;
+  std::vector<int32_t> blk_seg, blk_grp, grp_blk, seg_grp;
+  for (int s = 0; s < n_segments; ++s) {
+    seg_grp.push_back(int32_t(grp_blk.size()));
+    const int r0 = seg_row_begin[s], r1 = seg_row_begin[s + 1];
+    int nb_in_seg = 0;
+    for (int r = r0; r < r1; r += rpb) {
+      if (nb_in_seg % kGroupBlocks == 0) grp_blk.push_back(int32_t(blk_rows.size()));
+      blk_rows.push_back(make_int2(r, r + rpb < r1 ? r + rpb : r1));
+      blk_seg.push_back(s);
+      blk_grp.push_back(int32_t(grp_blk.size()) - 1);
+      ++nb_in_seg;
+    }
+  }
+  seg_grp.push_back(int32_t(grp_blk.size()));
+  grp_blk.push_back(int32_t(blk_rows.size()));
+  p->nblocks = int32_t(blk_rows.size());
+  p->ngroups = int32_t(grp_blk.size()) - 1;
+  p->nblocks_launch = (p->nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;
+
+  const int maxnc = 4;
+  int rc = KPM_OK;
+  if (!rc) rc = dev_alloc(&p->blk_rows, blk_rows.size(), "blk_rows");
+  if (!rc) rc = dev_alloc(&p->blk_seg, blk_seg.size(), "blk_seg");
+  if (!rc) rc = dev_alloc(&p->blk_grp, blk_grp.size(), "blk_grp");
+  if (!rc) rc = dev_alloc(&p->grp_blk, grp_blk.size(), "grp_blk");
+  if (!rc) rc = dev_alloc(&p->seg_grp, seg_grp.size(), "seg_grp");
+  if (!rc) rc = dev_alloc(&p->scale, size_t(2) * n_segments, "scale");
+  if (!rc) rc = dev_alloc(&p->part, size_t(p->nblocks) * maxnc * width * 2, "partials");
+  if (!rc) rc = dev_alloc(&p->gpart, size_t(p->ngroups) * maxnc * width * 2, "group partials");
+  if (!rc) rc = dev_alloc(&p->grp_cnt, size_t(p->ngroups), "group counters");
+  if (!rc) rc = dev_alloc(&p->seg_cnt, size_t(n_segments), "segment counters");
+  if (!rc) rc = dev_alloc(&p->done_cnt, 1, "done counter");
+  if (!rc) rc = dev_alloc(&p->ctrl, 2, "step counter");
+  if (!rc) rc = upload(p->blk_rows, blk_rows, "upload blk_rows");
+  if (!rc) rc = upload(p->blk_seg, blk_seg, "upload blk_seg");
+  if (!rc) rc = upload(p->blk_grp, blk_grp, "upload blk_grp");
+  if (!rc) rc = upload(p->grp_blk, grp_blk, "upload grp_blk");
+  if (!rc) rc = upload(p->seg_grp, seg_grp, "upload seg_grp");
+  if (!rc) rc = check_hip(hipMemset(p->grp_cnt, 0, sizeof(unsigned) * p->ngroups), "memset");
+  if (!rc) rc = check_hip(hipMemset(p->seg_cnt, 0, sizeof(unsigned) * n_segments), "memset");
+  if (!rc) rc = check_hip(hipMemset(p->done_cnt, 0, sizeof(unsigned)), "memset");
+  if (!rc) rc = check_hip(hipMemset(p->ctrl, 0, 2 * sizeof(int32_t)), "memset");
+  if (!rc) rc = check_hip(hipDeviceSynchronize(), "plan setup");
+  if (rc) {
+    free_plan(p);
+    return rc;
+  }
+  *out = p;
+  return KPM_OK;
+}
+
+int kpm_plan_destroy(kpm_plan* plan) {
+  if (!plan) return KPM_OK;
+  hipError_t e = hipDeviceSynchronize();
+  free_plan(plan);
+  return check_hip(e, "plan destroy");
+}
+
+int kpm_plan_set_scale(kpm_plan* plan, const double* h_ab) {
+  if (!plan || !h_ab) return fail(KPM_ERR_ARG, "NULL argument");
+  for (int s = 0; s < plan->nseg; ++s) {
+    const double a = h_ab[2 * s];
+    if (!(a > 0.0)) return fail(KPM_ERR_ARG, "scale a must be positive and finite");
+  }
+  int rc = check_hip(hipMemcpy(plan->scale, h_ab, sizeof(double) * 2 * plan->nseg,
+                               hipMemcpyHostToDevice),
+                     "upload scale");
+  if (!rc) plan->scale_set = true;
+  return rc;
+}
+
+int kpm_plan_set_step(kpm_plan* plan, int32_t n, kpm_stream_t stream) {
+  if (!plan) return fail(KPM_ERR_ARG, "plan is NULL");
+  if (n < 0) return fail(KPM_ERR_ARG, "step must be non-negative");
+  hipLaunchKernelGGL(set_ctrl_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, plan->ctrl, n);
+  return check_hip(hipGetLastError(), "set_ctrl_kernel launch");
+}
+
+int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream) {
+  if (!plan || !n) return fail(KPM_ERR_ARG, "NULL argument");
+  int32_t h[2] = {0, 0};
+  int rc = check_hip(hipMemcpyAsync(h, plan->ctrl, sizeof(h), hipMemcpyDeviceToHost,
+                                    (hipStream_t)stream),
+                     "read step counter");
+  if (!rc) rc = check_hip(hipStreamSynchronize((hipStream_t)stream), "stream sync");
+  if (rc) return rc;
+  *n = h[0];
+  if (h[1]) return fail(KPM_ERR_ARG, "a step ran with its dots slot outside [0, out_cap)");
+  return KPM_OK;
+}
+
+static int check_csr(const kpm_plan* plan, const kpm_csr* h) {
+  if (!h) return fail(KPM_ERR_ARG, "matrix is NULL");
+  if (h->value_type != plan->value_type)
+    return fail(KPM_ERR_ARG, "matrix value type differs from the plan's");
+  if (h->n_rows != plan->n_rows || h->n_cols != plan->n_rows)
+    return fail(KPM_ERR_ARG, "matrix must be square with the plan's row count");
+  if (!h->indptr || (h->nnz > 0 && (!h->indices || !h->values)))
+    return fail(KPM_ERR_ARG, "matrix arrays are NULL");
+  if (h->nnz < 0 || h->nnz > INT32_MAX) return fail(KPM_ERR_ARG, "nnz out of int32 range");
+  return KPM_OK;
+}
+
+int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_stride,
+                   int32_t slots, const void* w, double* d_dots, int32_t out_base,
+                   int32_t out_cap, int32_t n_steps, int32_t dots_mode,
+                   kpm_stream_t stream) {
+  if (!plan) return fail(KPM_ERR_ARG, "plan is NULL");
+  int rc = check_csr(plan, h);
+  if (rc) return rc;
+  if (!plan->scale_set) return fail(KPM_ERR_ARG, "scale not set (kpm_plan_set_scale)");
+  if (!ring) return fail(KPM_ERR_ARG, "ring is NULL");
+  if (slots < 2) return fail(KPM_ERR_ARG, "ring needs at least two slots");
+  if (slot_stride < int64_t(plan->n_rows) * plan->width)
+    return fail(KPM_ERR_ARG, "slot_stride smaller than n_rows * width");
+  if (dots_mode < KPM_DOTS_NONE || dots_mode > KPM_DOTS_PROJECT)
+    return fail(KPM_ERR_ARG, "unknown dots_mode");
+  if (dots_mode != KPM_DOTS_NONE && (!d_dots || out_cap <= 0))
+    return fail(KPM_ERR_ARG, "dots output buffer missing");
+  if (dots_mode == KPM_DOTS_PROJECT && !w) return fail(KPM_ERR_ARG, "projection vector missing");
+  if (n_steps < 0) return fail(KPM_ERR_ARG, "n_steps must be non-negative");
+  StepParams p{};
+  p.indptr = h->indptr;
+  p.indices = h->indices;
+  p.values = h->values;
+  p.ring = reinterpret_cast<char*>(ring);
+  p.slot_bytes = slot_stride * int64_t(elem_bytes(plan->value_type));
+  p.slots = slots;
+  p.w = w;
+  p.dots = d_dots;
+  p.out_base = out_base;
+  p.out_cap = out_cap;
+  fill_params(plan, p);
+  for (int i = 0; i < n_steps; ++i) {
+    rc = launch_cheb(plan->value_type, plan->shape, dots_mode, false, p, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return KPM_OK;
+}
+
+int kpm_block_dots(kpm_plan* plan, const void* x, const void* w, double* d_out,
+                   int32_t dots_mode, kpm_stream_t stream) {
+  if (!plan) return fail(KPM_ERR_ARG, "plan is NULL");
+  if (!x || !d_out) return fail(KPM_ERR_ARG, "NULL buffer");
+  if (dots_mode != KPM_DOTS_DOUBLING && dots_mode != KPM_DOTS_PROJECT)
+    return fail(KPM_ERR_ARG, "dots_mode must be DOUBLING or PROJECT");
+  if (dots_mode == KPM_DOTS_PROJECT && !w) return fail(KPM_ERR_ARG, "projection vector missing");
+  if (!plan->scale_set) return fail(KPM_ERR_ARG, "scale not set (kpm_plan_set_scale)");
+  StepParams p{};
+  p.ring = reinterpret_cast<char*>(const_cast<void*>(x));
+  p.slot_bytes = 0;
+  p.slots = 1;
+  p.w = w;
+  p.dots = d_out;
+  p.out_base = 0;
+  p.out_cap = 1;
+  fill_params(plan, p);
+  return launch_cheb(plan->value_type, plan->shape, dots_mode, true, p, (hipStream_t)stream);
+}
+
+int kpm_spmm(const kpm_csr* op, const void* x, int64_t x_stride, void* y, int64_t y_stride,
+             int32_t width, int32_t n_slots, kpm_stream_t stream) {
+  if (!op || !x || !y) return fail(KPM_ERR_ARG, "NULL argument");
+  if (op->n_rows <= 0 || op->n_cols <= 0) return fail(KPM_ERR_ARG, "empty operator");
+  if (!op->indptr || (op->nnz > 0 && (!op->indices || !op->values)))
+    return fail(KPM_ERR_ARG, "operator arrays are NULL");
+  if (n_slots <= 0) return fail(KPM_ERR_ARG, "n_slots must be positive");
+  if (n_slots > 1 && (x_stride < int64_t(op->n_cols) * width ||
+                      y_stride < int64_t(op->n_rows) * width))
+    return fail(KPM_ERR_ARG, "slot strides too small");
+  return launch_spmm(op->value_type, width, op, x, x_stride, y, y_stride, n_slots,
+                     (hipStream_t)stream);
+}
+
+int kpm_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb, double* mu,
+                  int64_t ldc, int32_t m_rows, int32_t n_rows, int64_t k, int32_t precision,
+                  int32_t k_chunk, double* work, int64_t work_elems, kpm_stream_t stream) {
+  if (!A || !B || !mu) return fail(KPM_ERR_ARG, "NULL argument");
+  if (m_rows <= 0 || n_rows <= 0 || k <= 0) return fail(KPM_ERR_ARG, "empty GEMM");
+  if (lda < k || ldb < k || ldc < n_rows) return fail(KPM_ERR_ARG, "leading dimension too small");
+  if (precision < KPM_GEMM_FP64 || precision > KPM_GEMM_FP32X3)
+    return fail(KPM_ERR_ARG, "unknown precision");
+  if (work_elems < 0) return fail(KPM_ERR_ARG, "negative workspace size");
+  return launch_corr_gemm(A, lda, B, ldb, mu, ldc, m_rows, n_rows, k, precision, k_chunk,
+                          work, work_elems, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,612 @@
+"""Drop-in ``kwant.kpm`` whose Chebyshev moment engine runs on an MI355X.
+
+Same public surface as kwant/kpm.py:32-34 -- ``SpectralDensity``,
+``Correlator``, ``conductivity``, ``RandomVectors``, ``LocalVectors``,
+``jackson_kernel``, ``lorentz_kernel``, ``fermi_distribution`` -- with the
+same arguments, attributes, resume methods (``add_moments``/``add_vectors``)
+and exceptions.  What differs is where the work runs:
+
+* the Hamiltonian is uploaded once as an int32 CSR (complex128 by default,
+  complex64 with ``precision='complex64'``) and the rescaling (H - b)/a is
+  applied inside the recursion kernel (kwant/kpm.py:1182-1183);
+* random vectors are drawn on the host with the reference's exact
+  ``RandomState`` consumption and processed in blocks of up to 64 vectors by
+  one fused SpMM recursion kernel per Chebyshev step that also accumulates the
+  FP64 dot products of the doubling identity (kwant/kpm.py:436-477);
+* the Correlator expansions keep their Chebyshev vectors on the GPU and the
+  moment matrix is a GEMM on the matrix cores (kwant/kpm.py:685-695);
+* the moment-to-energy transforms (kernel, DCT-III, chebval, simpson) stay in
+  FP64 on the host, exactly as kwant/kpm.py:378-388, 1197-1222 do.
+
+Extra keyword arguments (all optional): ``precision`` ('complex128' |
+'complex64'), ``max_block_width`` (power of two <= 64), ``variant`` (kernel
+launch shape, see kwant_amd.autotune), ``gemm_precision`` for the
+Correlator ('fp64' | 'fp32' | 'fp32x3').
+"""
+import math
+import numbers
+from collections.abc import Iterable
+
+import numpy as np
+import scipy.fftpack
+import scipy.sparse
+from numpy.polynomial.chebyshev import chebval
+from scipy.integrate import simpson
+from scipy.sparse import coo_matrix, csr_matrix
+from scipy.sparse.linalg import LinearOperator
+
+from . import _engine as eng
+from .lanczos import lanczos_bounds
+
+__all__ = ['SpectralDensity', 'Correlator', 'conductivity',
+           'RandomVectors', 'LocalVectors', 'jackson_kernel', 'lorentz_kernel',
+           'fermi_distribution']
+
+SAMPLING = 2  # energies per moment (kwant/kpm.py:36)
+
+
+# ---------------------------------------------------------------------------
+# inputs: systems, operators, random numbers
+
+def _is_system(obj):
+    """Duck-typed kwant.system.System (kwant/kpm.py:177)."""
+    return hasattr(obj, 'hamiltonian_submatrix') and hasattr(obj, 'site_ranges')
+
+
+def _is_local_operator(obj):
+    """Duck-typed kwant.operator._LocalOperator (bind + act)."""
+    return hasattr(obj, 'bind') and hasattr(obj, 'act')
+
+
+def ensure_rng(rng=None):
+    """Seed contract of kwant/_common.py:115-131: None -> numpy's global
+    RandomState, an integer -> a new RandomState(seed), else an object with
+    the RandomState interface."""
+    if rng is None:
+        return np.random.mtrand._rand
+    if isinstance(rng, numbers.Integral):
+        return np.random.RandomState(rng)
+    if all(hasattr(rng, a) for a in ('random_sample', 'randn', 'randint', 'choice')):
+        return rng
+    raise ValueError("Expecting a seed or an object that offers the "
+                     "numpy.random.RandomState interface.")
+
+
+def _as_csr(hamiltonian, params=None):
+    if _is_system(hamiltonian):
+        hamiltonian = hamiltonian.hamiltonian_submatrix(params=params, sparse=True)
+    try:
+        return csr_matrix(hamiltonian)
+    except Exception:
+        raise ValueError("'hamiltonian' is neither a matrix nor a Kwant system.")
+
+
+def _orbitals_of(syst, where):
+    """(total orbitals, orbital indices) of ``where`` (kwant/kpm.py:1025-1039)."""
+    if _is_system(syst):
+        return _system_orbitals(syst, where)
+    try:
+        tot = csr_matrix(syst).shape[0]
+    except TypeError:
+        raise TypeError("'syst' is neither a matrix nor a Kwant system.")
+    return tot, (range(tot) if where is None else np.asarray(where, int))
+
+
+def _system_orbitals(syst, where):
+    """Orbitals of the sites in ``where`` for a kwant system, from its
+    ``site_ranges`` (kwant/system.py:27-34, kwant/operator.pyx:114-199)."""
+    ranges = np.asarray(syst.site_ranges, dtype=np.int64)
+    tot = int(ranges[-1][2])
+    sites = list(range(syst.graph.num_nodes)) if where is None else None
+    if sites is None:
+        if callable(where):
+            sites = [i for i, s in enumerate(syst.sites) if where(s)]
+        else:
+            ids = getattr(syst, 'id_by_site', None)
+            sites = [ids[s] if ids is not None and not isinstance(s, numbers.Integral)
+                     else int(s) for s in where]
+    orbs = []
+    for site in sites:
+        k = np.searchsorted(ranges[:, 0], site, side='right') - 1
+        first, norbs, offset = ranges[k]
+        start = int(offset + (site - first) * norbs)
+        orbs.extend(range(start, start + int(norbs)))
+    return tot, orbs
+
+
+def RandomVectors(syst, where=None, rng=None):
+    """Random-phase vectors exp(2 pi i u), u ~ U[0, 1), on the orbitals of
+    ``where`` and zero elsewhere; one ``rng.random_sample(len(where))`` draw
+    per vector (kwant/kpm.py:932-951)."""
+    rng = ensure_rng(rng)
+    tot, orbs = _orbitals_of(syst, where)
+    while True:
+        vec = np.zeros(tot, dtype=complex)
+        vec[orbs] = np.exp(2j * np.pi * rng.random_sample(len(orbs)))
+        yield vec
+
+
+class LocalVectors:
+    """Unit (float64) vectors on each orbital of ``where``, in order
+    (kwant/kpm.py:954-985)."""
+
+    def __init__(self, syst, where=None, *args):
+        self.tot_norbs, self.orbs = _orbitals_of(syst, where)
+        self._idx = 0
+
+    def __len__(self):
+        return len(self.orbs)
+
+    def __iter__(self):
+        return self
+
+    def __next__(self):
+        if self._idx >= len(self):
+            raise StopIteration('Too many vectors requested from this generator')
+        vec = np.zeros(self.tot_norbs)
+        vec[self.orbs[self._idx]] = 1
+        self._idx += 1
+        return vec
+
+
+class _VectorFactory:
+    """Indexed, lazily filled view of a vector iterable (kwant/kpm.py:850-929)."""
+
+    def __init__(self, vectors=None, num_vectors=None, accumulate=True):
+        assert isinstance(vectors, Iterable)
+        try:
+            size = len(vectors)
+            if num_vectors is None:
+                num_vectors = size
+        except TypeError:
+            size = np.inf
+            if num_vectors is None:
+                raise ValueError("'num_vectors' must be specified when "
+                                 "'vectors' has no len() method.")
+        self._max_vectors = size
+        self._iterator = iter(vectors)
+        self.accumulate = accumulate
+        self.saved_vectors = []
+        self.num_vectors = 0
+        self.add_vectors(num_vectors=num_vectors)
+        self._last_idx = -np.inf
+        self._last_vector = None
+
+    def __getitem__(self, index):
+        if index < self._last_idx and not self.accumulate:
+            raise ValueError("Cannot get previous values if 'accumulate' is False")
+        if index >= self.num_vectors:
+            raise IndexError('Requested more vectors than available')
+        self._last_idx = index
+        if self.accumulate:
+            if self.saved_vectors[index] is None:
+                self.saved_vectors[index] = next(self._iterator)
+            return self.saved_vectors[index]
+        self._last_vector = next(self._iterator)
+        return self._last_vector
+
+    def add_vectors(self, num_vectors=None):
+        if num_vectors is None:
+            raise ValueError("'num_vectors' must be specified.")
+        if num_vectors <= 0 or num_vectors != int(num_vectors):
+            raise ValueError("'num_vectors' must be a positive integer")
+        if self.num_vectors + num_vectors > self._max_vectors:
+            raise ValueError("'num_vectors' is larger than available vectors")
+        self.num_vectors += num_vectors
+        if self.accumulate:
+            self.saved_vectors.extend([None] * num_vectors)
+
+
+# ---------------------------------------------------------------------------
+# kernels and transforms (host FP64; kwant/kpm.py:990-1009, 1106-1142, 1190-1222)
+
+def jackson_kernel(moments):
+    """Jackson damping g_m (Weisse et al., RMP 78, 275, Eq. 71) applied along
+    axis 0 (kwant/kpm.py:1106-1123)."""
+    count = len(moments)
+    q = np.pi / (count + 1)
+    m = np.arange(count)
+    g = ((count - m + 1) * np.cos(q * m) + np.sin(q * m) / np.tan(q)) / (count + 1)
+    return np.transpose(moments.transpose() * g)
+
+
+def lorentz_kernel(moments, l=4):
+    """Lorentz damping sinh(l (1 - m/M)) / sinh(l) (kwant/kpm.py:1126-1142)."""
+    count = len(moments)
+    g = np.sinh(l * (1 - np.arange(count) / count)) / np.sinh(l)
+    return np.transpose(moments.transpose() * g)
+
+
+def fermi_distribution(energy, mu, temperature):
+    """Fermi function; a step at T = 0 (kwant/kpm.py:990-1009)."""
+    if temperature < 0:
+        raise ValueError("temperature must be non-negative")
+    if temperature == 0:
+        return np.array(np.less(energy - mu, 0), dtype=float)
+    return 1 / (1 + np.exp((energy - mu) / temperature))
+
+
+def _chebyshev_nodes(n_sampling):
+    """x_j = cos(pi (1 - (2 j + 1) / (2 n))), ascending (kwant/kpm.py:1190-1194)."""
+    raw, step = np.linspace(np.pi, 0, n_sampling, endpoint=False, retstep=True)
+    return np.cos(raw + step / 2)
+
+
+def _calc_fft_moments(moments):
+    """Densities (and Gauss-Chebyshev weights) at the 2M Chebyshev nodes by an
+    unnormalised DCT-III of the zero-padded moments (kwant/kpm.py:1197-1222)."""
+    moments = np.asarray(moments)
+    count, *extra = moments.shape
+    n_sampling = SAMPLING * count
+    padded = np.zeros([n_sampling] + extra, dtype=moments.dtype)
+    padded[:count] = moments
+    nodes = _chebyshev_nodes(n_sampling)
+    gammas = np.transpose(scipy.fftpack.dct(padded.transpose(), type=3))[::-1]
+    rho = np.transpose(np.divide(gammas.transpose(), np.pi * np.sqrt(1 - nodes ** 2)))
+    return rho, gammas
+
+
+def _scale_from_bounds(lmin, lmax, eps):
+    """(a, b) and the single-eigenvalue check of kwant/kpm.py:1164, 1174-1180."""
+    tol = eps / 2
+    a = np.abs(lmax - lmin) / (2. - eps)
+    b = (lmax + lmin) / 2.
+    if lmax - lmin <= abs(lmax + lmin) * tol / 2:
+        raise ValueError('The Hamiltonian has a single eigenvalue, it is not possible '
+                         'to obtain a spectral density.')
+    return a, b
+
+
+class _RescaledOperator(LinearOperator):
+    """(H v - b v) / a evaluated on the GPU (the ``hamiltonian`` attribute of
+    kwant/kpm.py:1185)."""
+
+    def __init__(self, dmat, a, b):
+        self._dmat, self._a, self._b = dmat, a, b
+        super().__init__(dtype=np.complex128, shape=dmat.shape)
+
+    def _matvec(self, v):
+        import torch
+        dev = eng.torch_device()
+        cdt, _ = eng.complex_dtype(self._dmat.precision)
+        x = torch.from_numpy(np.ascontiguousarray(np.asarray(v).reshape(-1),
+                                                  dtype=np.complex128)).to(dev).to(cdt)
+        y = torch.empty_like(x)
+        self._dmat.spmm(x, y, 1)
+        y = (y - self._b * x) / self._a
+        return y.cpu().numpy().astype(np.complex128)
+
+
+def _rescale(hamiltonian, eps, v0, bounds, precision='complex128'):
+    """Upload H and return (GPU rescaled operator, (a, b)) as
+    kwant/kpm.py:1145-1187; bounds=None runs FP64 Lanczos on the GPU."""
+    dmat = eng.DeviceCSR(hamiltonian, precision)
+    if bounds:
+        lmin, lmax = bounds
+    else:
+        dmat64 = dmat if dmat.precision in ('complex128', 'c128') else \
+            eng.DeviceCSR(hamiltonian, 'complex128')
+        lmin, lmax = lanczos_bounds(dmat64, v0, tol=eps / 2)
+    a, b = _scale_from_bounds(lmin, lmax, eps)
+    return _RescaledOperator(dmat, a, b), (a, b)
+
+
+# ---------------------------------------------------------------------------
+# GPU moment blocks
+
+class _DosBlock:
+    """W vectors expanded together: one BlockRecursion plus their moments."""
+
+    def __init__(self, dmat, ab, vectors, width, mode, w=None, variant=-1):
+        import torch
+        dev = eng.torch_device()
+        cdt, _ = eng.complex_dtype(dmat.precision)
+        host = np.stack([np.asarray(v) for v in vectors], axis=1).astype(np.complex128)
+        block = torch.from_numpy(np.ascontiguousarray(host)).to(dev).to(cdt)
+        self.count = len(vectors)
+        self.rec = eng.BlockRecursion(dmat, ab, block, width, mode, w=w, variant=variant)
+        self.moments = None  # (count, M) complex128
+
+
+def _doubling_fill(mom, dots, first_step, mu0, mu1):
+    """Moments 2s, 2s+1 of steps s = first_step.. from the per-step dots
+    (kwant/kpm.py:470-473)."""
+    d0 = dots[:, :, 0]
+    d1 = dots[:, :, 2] + 1j * dots[:, :, 3]
+    for i in range(dots.shape[0]):
+        s = first_step + i
+        if s == 0:
+            continue
+        mom[:, 2 * s] = 2 * d0[i, :mom.shape[0]] - mu0
+        mom[:, 2 * s + 1] = 2 * d1[i, :mom.shape[0]] - mu1
+
+
+class SpectralDensity:
+    """Spectral density of an operator by the kernel polynomial method,
+    computed on the GPU.  Arguments and attributes as kwant/kpm.py:39-161."""
+
+    def __init__(self, hamiltonian, params=None, operator=None,
+                 num_vectors=10, num_moments=None, energy_resolution=None,
+                 vector_factory=None, bounds=None, eps=0.05, rng=None,
+                 kernel=None, mean=True, accumulate_vectors=True, *,
+                 precision='complex128', max_block_width=64, variant=-1):
+        if num_moments and energy_resolution:
+            raise TypeError("either 'num_moments' or 'energy_resolution' "
+                            "must be provided.")
+        self.eps = eps
+        hamiltonian = _as_csr(hamiltonian, params)
+
+        self._op_matrix = None
+        if operator is None:
+            self.operator = None
+        elif _is_local_operator(operator):
+            self.operator = operator.bind(params=params)
+        elif callable(operator):
+            self.operator = operator
+        elif hasattr(operator, 'dot'):
+            op = csr_matrix(operator)
+            self._op_matrix = op
+            self.operator = lambda bra, ket: np.vdot(bra, op.dot(ket))
+        else:
+            raise ValueError("Parameter 'operator' has no '.dot' attribute "
+                             "and is not callable.")
+
+        self.mean = mean
+        rng = ensure_rng(rng)
+        self._v0 = np.exp(2j * np.pi * rng.random_sample(hamiltonian.shape[0]))
+        if eps <= 0:
+            raise ValueError("'eps' must be positive")
+        if max_block_width < 1 or max_block_width > 64 or \
+                max_block_width & (max_block_width - 1):
+            raise ValueError("max_block_width must be a power of two <= 64")
+        self._precision = precision
+        self._max_width = max_block_width
+        self._variant = variant
+        self.hamiltonian, (self._a, self._b) = _rescale(
+            hamiltonian, self.eps, self._v0, bounds, precision)
+        self._dmat = self.hamiltonian._dmat
+        self.bounds = (self._b - self._a, self._b + self._a)
+
+        if energy_resolution:
+            num_moments = math.ceil((1.6 * self._a) / energy_resolution)
+        elif num_moments is None:
+            num_moments = 100
+        if num_moments <= 0 or num_moments != int(num_moments):
+            raise ValueError("'num_moments' must be a positive integer")
+
+        if vector_factory is None:
+            self._vector_factory = _VectorFactory(
+                RandomVectors(hamiltonian, rng=rng), num_vectors=num_vectors,
+                accumulate=accumulate_vectors)
+        else:
+            if not isinstance(vector_factory, Iterable):
+                raise TypeError('vector_factory must be iterable')
+            try:
+                len(vector_factory)
+            except TypeError:
+                if num_vectors is None:
+                    raise ValueError('num_vectors must be provided if'
+                                     'vector_factory has no length.')
+            self._vector_factory = _VectorFactory(
+                vector_factory, num_vectors=num_vectors, accumulate=accumulate_vectors)
+        num_vectors = self._vector_factory.num_vectors
+
+        self._blocks = []
+        self._moments_list = []
+        self.num_moments = num_moments
+        self._update_moments_list(self.num_moments, num_vectors)
+
+        self.kernel = kernel if kernel is not None else jackson_kernel
+        self.densities, self._gammas = _calc_fft_moments(self._moments())
+
+    @property
+    def energies(self):
+        return self._a * _chebyshev_nodes(SAMPLING * self.num_moments) + self._b
+
+    @property
+    def num_vectors(self):
+        return len(self._moments_list)
+
+    @property
+    def _last_two_alphas(self):
+        """(alpha_{n-1}, alpha_n) of every vector, copied from the GPU."""
+        out = []
+        for blk in self._blocks:
+            for c in range(blk.count):
+                out.append(blk.rec.last_two(c) if blk.rec is not None else 0.)
+        return out
+
+    def __call__(self, energy=None):
+        """Densities at the nodes, or chebval at ``energy`` (kwant/kpm.py:263-295)."""
+        if energy is None:
+            return self.energies, self.densities
+        energy = np.asarray(energy)
+        e = (energy - self._b) / self._a
+        g_e = np.pi * np.sqrt(1 - e) * np.sqrt(1 + e)
+        moments = self._moments()
+        moments[1:] = 2 * moments[1:]
+        return np.transpose(chebval(e, moments) / g_e)
+
+    def integrate(self, distribution_function=None):
+        """Gauss-Chebyshev integral a/(2M) sum_k gamma_k f(E_k)
+        (kwant/kpm.py:297-316)."""
+        factor = self._a / (2 * self.num_moments)
+        if distribution_function is None:
+            rho = self._gammas
+        else:
+            dist = distribution_function(self.energies)
+            rho = np.transpose(self._gammas.transpose() * dist)
+        return factor * np.sum(rho, axis=0)
+
+    def add_moments(self, num_moments=None, *, energy_resolution=None):
+        """Extend every expansion by ``num_moments`` (kwant/kpm.py:318-358)."""
+        if not ((num_moments is None) ^ (energy_resolution is None)):
+            raise TypeError("either 'num_moments' or 'energy_resolution' "
+                            "must be provided.")
+        if energy_resolution:
+            if energy_resolution <= 0:
+                raise ValueError("'energy_resolution' must be positive")
+            if self._a * 1.6 / self.num_moments < energy_resolution:
+                raise ValueError('Energy resolution is already smaller '
+                                 'than the requested resolution')
+            num_moments = math.ceil((1.6 * self._a) / energy_resolution)
+        if (num_moments is None or num_moments <= 0
+                or num_moments != int(num_moments)):
+            raise ValueError("'num_moments' must be a positive integer")
+        self._update_moments_list(self.num_moments + num_moments, self.num_vectors)
+        self.num_moments += num_moments
+        self.densities, self._gammas = _calc_fft_moments(self._moments())
+
+    def add_vectors(self, num_vectors=None):
+        """Expand ``num_vectors`` more vectors (kwant/kpm.py:360-376)."""
+        self._vector_factory.add_vectors(num_vectors)
+        num_vectors = self._vector_factory.num_vectors - self.num_vectors
+        self._update_moments_list(self.num_moments, self.num_vectors + num_vectors)
+        self.densities, self._gammas = _calc_fft_moments(self._moments())
+
+    def _moments(self):
+        """real_if_close, vectors to axis 1, mean, / a, kernel
+        (kwant/kpm.py:378-388)."""
+        moments = np.real_if_close(self._moments_list)
+        moments = np.swapaxes(moments, 0, 1)
+        if self.mean:
+            moments = np.mean(moments, axis=1)
+        moments /= self._a
+        return self.kernel(moments)
+
+    # -- the moment engine ---------------------------------------------------
+    def _update_moments_list(self, n_moments, num_vectors):
+        """Validation and bookkeeping of kwant/kpm.py:390-435; the moments of
+        new vectors or new moments come from the GPU blocks."""
+        if self.num_vectors == num_vectors:
+            r_start, new_vectors = 0, 0
+        elif self.num_vectors < num_vectors:
+            r_start = self.num_vectors
+            new_vectors = num_vectors - self.num_vectors
+        else:
+            raise ValueError('Cannot decrease number of vectors')
+        if n_moments == self.num_moments:
+            new_moments = 0
+            if new_vectors == 0:
+                return
+        else:
+            if not self._vector_factory.accumulate:
+                raise ValueError("Cannot increase the number of moments if "
+                                 "'accumulate_vectors' is 'False'.")
+            new_moments = n_moments - self.num_moments
+            if new_moments < 0:
+                raise ValueError('Cannot decrease number of moments')
+            if new_vectors != 0:
+                raise ValueError("Only 'num_moments' *or* 'num_vectors' "
+                                 "may be updated at a time.")
+
+        if new_vectors:
+            vectors = [self._vector_factory[r] for r in range(r_start, num_vectors)]
+            if n_moments < 2:
+                # kwant/kpm.py:439-448 writes moments 0 and 1 of every new vector
+                raise IndexError('list assignment index out of range')
+            start = 0
+            for width in eng.pow2_widths(len(vectors), self._max_width):
+                chunk = vectors[start:start + width]
+                start += width
+                blk = self._new_block(chunk, width)
+                self._expand_new(blk, n_moments)
+                self._moments_list.extend(list(blk.moments))
+                if not self._vector_factory.accumulate:
+                    blk.rec = None  # no resume without accumulated vectors
+                self._blocks.append(blk)
+        else:
+            for blk in self._blocks:
+                self._expand_more(blk, self.num_moments, n_moments)
+            self._moments_list = [m for blk in self._blocks for m in blk.moments]
+
+    def _new_block(self, vectors, width):
+        ab = (self._a, self._b)
+        if self.operator is None:
+            return _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_DOUBLING,
+                             variant=self._variant)
+        if self._op_matrix is not None:
+            import torch
+            blk = _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_PROJECT,
+                            variant=self._variant)
+            # w = O^H alpha_0, so <w|alpha_n> = vdot(alpha_0, O alpha_n)
+            op_h = eng.DeviceCSR(self._op_matrix.conj().T.tocsr(), self._dmat.precision)
+            w = torch.empty_like(blk.rec.ring[0])
+            op_h.spmm(blk.rec.ring[0], w, width)
+            blk.rec.w = w
+            blk._op_h = op_h
+            return blk
+        return _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_NONE,
+                         variant=self._variant)
+
+    def _expand_new(self, blk, n_moments):
+        rec = blk.rec
+        if rec.mode == eng.DOTS_DOUBLING:
+            mom = np.zeros((blk.count, n_moments), dtype=np.complex128)
+            dots = rec.step(n_moments // 2)
+            mu0 = dots[0, :blk.count, 0] + 0j
+            mu1 = np.conj(dots[0, :blk.count, 2] + 1j * dots[0, :blk.count, 3])
+            mom[:, 0], mom[:, 1] = mu0, mu1
+            _doubling_fill(mom, dots, 0, mu0, mu1)
+            if n_moments % 2:
+                tail = rec.norms()
+                mom[:, n_moments - 1] = 2 * tail[:blk.count, 0] - mu0
+            blk.moments = mom
+        elif rec.mode == eng.DOTS_PROJECT:
+            mom = np.zeros((blk.count, n_moments), dtype=np.complex128)
+            dots = rec.step(n_moments - 1)
+            mom[:, 0] = dots[0, :blk.count, 0] + 1j * dots[0, :blk.count, 1]
+            mom[:, 1:] = (dots[:, :blk.count, 2] + 1j * dots[:, :blk.count, 3]).T
+            blk.moments = mom
+        else:
+            blk.moments = self._callable_moments(blk, 0, n_moments, None)
+
+    def _expand_more(self, blk, m_old, m_new):
+        rec = blk.rec
+        if rec.mode == eng.DOTS_DOUBLING:
+            mom = np.zeros((blk.count, m_new), dtype=np.complex128)
+            mom[:, :m_old] = blk.moments
+            mu0, mu1 = mom[:, 0].copy(), mom[:, 1].copy()
+            first = m_old // 2
+            dots = rec.step(m_new // 2 - first)
+            _doubling_fill(mom, dots, first, mu0, mu1)
+            if m_new % 2:
+                tail = rec.norms()
+                mom[:, m_new - 1] = 2 * tail[:blk.count, 0] - mu0
+            blk.moments = mom
+        elif rec.mode == eng.DOTS_PROJECT:
+            mom = np.zeros((blk.count, m_new), dtype=np.complex128)
+            mom[:, :m_old] = blk.moments
+            dots = rec.step(m_new - m_old)
+            mom[:, m_old:] = (dots[:, :blk.count, 2] + 1j * dots[:, :blk.count, 3]).T
+            blk.moments = mom
+        else:
+            blk.moments = self._callable_moments(blk, m_old, m_new, blk.moments)
+
+    def _callable_moments(self, blk, m_old, m_new, old):
+        """Operator path with a Python callable (kwant/kpm.py:478-484): the
+        recursion runs on the GPU; each new alpha_n is handed to the callable
+        on the host, one moment per step."""
+        rec = blk.rec
+        alpha0 = rec.slot(0).cpu().numpy().astype(np.complex128)
+        moms = [list(old[c]) if old is not None else [] for c in range(blk.count)]
+        if m_old == 0:
+            alpha1 = None
+            rec.step(1, want_dots=False)
+            alpha1 = rec.slot(1).cpu().numpy().astype(np.complex128)
+            for c in range(blk.count):
+                moms[c].append(self.operator(alpha0[:, c], alpha0[:, c]))
+                moms[c].append(self.operator(alpha0[:, c], alpha1[:, c]))
+            m_old = 2
+        for _ in range(m_old, m_new):
+            rec.step(1, want_dots=False)
+            cur = rec.slot(rec.n).cpu().numpy().astype(np.complex128)
+            for c in range(blk.count):
+                moms[c].append(self.operator(alpha0[:, c], cur[:, c]))
+        return moms
+
+
+# ---------------------------------------------------------------------------
+# Correlator / conductivity
+
+from .correlator import Correlator, conductivity  # noqa: E402

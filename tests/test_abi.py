@@ -1,0 +1,66 @@
+"""The C-ABI library loads and exports every symbol include/kpm_amd.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kpm_amd.h")
+LIB = os.path.join(ROOT, "kwant_amd", "libkpm_amd.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kpm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for required in ("kpm_plan_create", "kpm_cheb_steps", "kpm_block_dots", "kpm_spmm",
+                     "kpm_corr_gemm", "kpm_last_error"):
+        assert required in names
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_library_exports_all_declared_symbols():
+    import torch  # noqa: F401  (the library shares torch's HIP runtime)
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    lib.kpm_abi_version.restype = ctypes.c_int
+    assert lib.kpm_abi_version() >= 100
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_engine_binding_covers_header():
+    from kwant_amd import _engine
+    assert sorted(_engine.EXPORTS) == declared_functions()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_argument_errors_without_gpu():
+    """Argument validation happens before any device work."""
+    import torch  # noqa: F401
+    lib = ctypes.CDLL(LIB)
+    lib.kpm_last_error.restype = ctypes.c_char_p
+    out = ctypes.c_void_p()
+    seg = (ctypes.c_int32 * 2)(0, 10)
+    rc = lib.kpm_plan_create(10, 3, 1, 1, seg, -1, ctypes.byref(out))  # width 3
+    assert rc != 0 and b"power of two" in lib.kpm_last_error()
+    rc = lib.kpm_plan_create(10, 4, 7, 1, seg, -1, ctypes.byref(out))  # bad type
+    assert rc == 1
+    seg_bad = (ctypes.c_int32 * 2)(0, 9)
+    rc = lib.kpm_plan_create(10, 4, 1, 1, seg_bad, -1, ctypes.byref(out))
+    assert rc == 1 and b"cover" in lib.kpm_last_error()
+
+
+def test_pow2_widths():
+    from kwant_amd._engine import pow2_widths
+    assert pow2_widths(10, 64) == [8, 2]
+    assert pow2_widths(64, 64) == [64]
+    assert pow2_widths(100, 64) == [64, 32, 4]
+    assert pow2_widths(1, 64) == [1]
+    assert sum(pow2_widths(1000, 16)) == 1000

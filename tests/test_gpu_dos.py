@@ -1,0 +1,336 @@
+"""GPU parity of the SpectralDensity path against the CPU oracle and the dense
+brute force.  Tolerances: complex128 moments within 1e-11 of the oracle
+relative to mu_0 (the BASELINE target), complex64 within 1e-5; the
+reference's own TOL=1e-12 / TOL_SP=1e-6 for the mirrored API tests
+(kwant/tests/test_kpm.py:38-40)."""
+import numpy as np
+import pytest
+
+from oracle import brute
+from oracle import kpm_oracle as ko
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL_SP = 1e-6
+
+
+def kpm():
+    from kwant_amd import kpm as k
+    return k
+
+
+def rel(x, y, scale):
+    return float(np.max(np.abs(np.asarray(x) - np.asarray(y))) / scale)
+
+
+def gue(n, seed):
+    return brute.gaussian_unitary(n, seed)
+
+
+def gershgorin(h):
+    import scipy.sparse as sp
+    r = float(np.max(np.asarray(abs(sp.csr_matrix(h)).sum(axis=1)).ravel()))
+    return (-r, r)
+
+
+@pytest.mark.parametrize("n_moments,num_vectors", [(200, 5), (201, 3), (2, 4), (3, 1), (64, 17)])
+def test_moments_c128_match_oracle(n_moments, num_vectors):
+    h = gue(20, 1)
+    bounds = gershgorin(h)
+    kw = dict(num_moments=n_moments, num_vectors=num_vectors, bounds=bounds, rng=3)
+    gpu = kpm().SpectralDensity(h, **kw)
+    ora = ko.SpectralDensity(h, **kw)
+    g = np.array(gpu._moments_list)
+    o = np.array(ora._moments_list)
+    assert g.shape == o.shape
+    assert rel(g, o, 20) < 1e-11
+    assert rel(gpu.densities, ora.densities, np.abs(ora.densities).max()) < 1e-11
+    assert np.allclose(gpu.energies, ora.energies, atol=1e-14, rtol=0)
+
+
+def test_moments_match_brute_force():
+    h = gue(32, 2)
+    bounds = gershgorin(h)
+    sd = kpm().SpectralDensity(h, num_moments=120, num_vectors=3, bounds=bounds, rng=5)
+    for r, mom in enumerate(sd._moments_list):
+        vec = sd._vector_factory.saved_vectors[r]
+        ref = brute.moments(h, sd._a, sd._b, vec, 120)
+        assert rel(mom, ref, 32) < 1e-11
+
+
+def test_square_lattice_100x100_c128():
+    """The CPU workload: square 100x100, U[-0.5, 0.5] disorder, R=10, M=100."""
+    from kwant_amd.lattices import square
+    h, _ = square(100, disorder=1.0, seed=1)
+    kw = dict(num_moments=100, num_vectors=10, bounds=(-4.5, 4.5), rng=0)
+    gpu = kpm().SpectralDensity(h, **kw)
+    ora = ko.SpectralDensity(h, **kw)
+    assert rel(np.array(gpu._moments_list), np.array(ora._moments_list), 1e4) < 1e-11
+    assert rel(gpu.densities, ora.densities, np.abs(ora.densities).max()) < 1e-11
+
+
+@pytest.mark.parametrize("width", [64, 16, 4, 1])
+def test_graphene_c64_within_1e5(width):
+    from kwant_amd.lattices import honeycomb, honeycomb_bounds
+    h, _ = honeycomb(30, disorder=0.5, seed=2)
+    kw = dict(num_moments=256, num_vectors=width, bounds=honeycomb_bounds(1, 0.5), rng=4)
+    g64 = kpm().SpectralDensity(h, precision='complex64', **kw)
+    g128 = kpm().SpectralDensity(h, **kw)
+    ora = ko.SpectralDensity(h, **kw)
+    n = h.shape[0]
+    o = np.array(ora._moments_list)
+    assert rel(np.array(g128._moments_list), o, n) < 1e-11
+    assert rel(np.array(g64._moments_list), o, n) < 1e-5
+
+
+def test_all_variants_agree_bitwise():
+    from kwant_amd import _engine as eng
+    h = gue(40, 3)
+    bounds = gershgorin(h)
+    for prec in ('complex128', 'complex64'):
+        _, vt = eng.complex_dtype(prec)
+        for width in (1, 8, 32):
+            base = None
+            for v in range(eng.variant_count(width, vt)):
+                sd = kpm().SpectralDensity(h, num_moments=50, num_vectors=width, bounds=bounds,
+                                           rng=1, precision=prec, variant=v,
+                                           max_block_width=width)
+                m = np.array(sd._moments_list)
+                if base is None:
+                    base = m
+                else:
+                    assert np.array_equal(m, base), (prec, width, v)
+
+
+# --- mirrors of kwant/tests/test_kpm.py on the GPU path ---------------------
+
+def test_increase_num_moments_bitwise():
+    """kwant/tests/test_kpm.py:481-507."""
+    h = gue(20, 4)
+    k = kpm()
+    sp1 = k.SpectralDensity(h, num_moments=200, num_vectors=5, rng=1)
+    sp2 = k.SpectralDensity(h, num_moments=400, num_vectors=5, rng=1)
+    sp1.add_moments(200)
+    assert np.array_equal(np.array(sp1._moments_list), np.array(sp2._moments_list))
+    odd = k.SpectralDensity(h, num_moments=201, num_vectors=5, rng=1)
+    assert not np.all(np.asarray(odd._moments_list)[:, -1] == 0)
+    even = k.SpectralDensity(h, num_moments=200, num_vectors=5, rng=1)
+    even.add_moments(num_moments=1)
+    assert np.array_equal(np.array(even._moments_list), np.array(odd._moments_list))
+    odd.add_moments(num_moments=3)
+    fresh = k.SpectralDensity(h, num_moments=204, num_vectors=5, rng=1)
+    assert np.array_equal(np.array(odd._moments_list), np.array(fresh._moments_list))
+
+
+def test_increase_num_vectors_bitwise():
+    """kwant/tests/test_kpm.py:526-535: 10 vectors at once (blocks 8 + 2)
+    versus 5 + 5 (blocks 4 + 1 twice)."""
+    h = gue(20, 5)
+    k = kpm()
+    full = k.SpectralDensity(h, num_moments=200, num_vectors=10, rng=1)
+    part = k.SpectralDensity(h, num_moments=200, num_vectors=5, rng=1)
+    part.add_vectors(num_vectors=5)
+    assert np.array_equal(full.densities, part.densities)
+
+
+def test_energy_resolution_bitwise():
+    """kwant/tests/test_kpm.py:324-339."""
+    h = gue(20, 6)
+    k = kpm()
+    sp1 = k.SpectralDensity(h, rng=1, energy_resolution=0.05)
+    sp2 = k.SpectralDensity(h, rng=1, num_moments=sp1.num_moments)
+    assert np.all(sp1.densities == sp2.densities)
+
+
+def test_operator_none_vs_identity():
+    """kwant/tests/test_kpm.py:342-353 (doubling path vs operator path)."""
+    h = gue(20, 7)
+    k = kpm()
+    sp1 = k.SpectralDensity(h, operator=None, rng=1)
+    sp2 = k.SpectralDensity(h, operator=np.identity(20), rng=1)
+    np.testing.assert_allclose(sp1.densities, sp2.densities, rtol=0, atol=TOL_SP)
+
+
+def test_operator_matrix_matches_oracle():
+    h = gue(24, 8)
+    op = gue(24, 9)
+    kw = dict(num_moments=80, num_vectors=6, bounds=gershgorin(h), rng=2, operator=op)
+    gpu = kpm().SpectralDensity(h, **kw)
+    ora = ko.SpectralDensity(h, **kw)
+    scale = 24 * np.abs(op).max()
+    assert rel(np.array(gpu._moments_list), np.array(ora._moments_list), scale) < 1e-11
+    gpu.add_moments(20)
+    ora.add_moments(20)
+    assert rel(np.array(gpu._moments_list), np.array(ora._moments_list), scale) < 1e-11
+
+
+def test_operator_user_callable():
+    """kwant/tests/test_kpm.py:378-390."""
+    h = gue(20, 10)
+    ident = np.identity(20)
+    k = kpm()
+    user = lambda bra, ket: np.vdot(bra, np.dot(ident, ket))
+    sp1 = k.SpectralDensity(h, operator=user, rng=1)
+    sp2 = k.SpectralDensity(h, operator=ident, rng=1)
+    np.testing.assert_allclose(sp1.densities, sp2.densities, rtol=0, atol=TOL_SP)
+
+
+def test_vector_valued_callable_operator():
+    """A callable returning a vector (local density) gives (2M, N) densities
+    whose sum over sites is the identity-operator density."""
+    h = gue(16, 11)
+    k = kpm()
+    local = lambda bra, ket: bra.conj() * ket
+    sp1 = k.SpectralDensity(h, operator=local, num_moments=40, num_vectors=3, rng=2)
+    sp2 = k.SpectralDensity(h, operator=np.identity(16), num_moments=40, num_vectors=3, rng=2)
+    assert sp1.densities.shape == (80, 16)
+    np.testing.assert_allclose(sp1.densities.sum(axis=1), sp2.densities, rtol=0, atol=1e-10)
+
+
+def test_mean_false_local_vectors():
+    """kwant/tests/test_kpm.py:50-60."""
+    h = gue(20, 12)
+    k = kpm()
+    sites = [3, 11]
+    kw = dict(num_vectors=None, num_moments=10, rng=0)
+    loc = k.SpectralDensity(h, vector_factory=k.LocalVectors(h, where=sites), mean=False, **kw)
+    avg = k.SpectralDensity(h, vector_factory=k.LocalVectors(h, where=sites), mean=True, **kw)
+    np.testing.assert_allclose(loc.energies, avg.energies, rtol=0, atol=TOL)
+    np.testing.assert_allclose(loc.densities.sum(axis=1) / 2, avg.densities, rtol=0, atol=TOL)
+
+
+def test_call_and_integrate():
+    """kwant/tests/test_kpm.py:645-674."""
+    from scipy.integrate import simpson
+    h = gue(20, 13)
+    sd = kpm().SpectralDensity(h, num_moments=200, num_vectors=5)
+    e, rho = sd()
+    np.testing.assert_allclose(e, sd.energies, rtol=0, atol=TOL_SP)
+    dens = np.array([sd(x) for x in sd.energies])
+    np.testing.assert_allclose(dens, sd.densities, rtol=0, atol=TOL_SP)
+    ones = lambda x: np.ones_like(x)
+    assert abs((sd.integrate() - simpson(sd.densities, x=sd.energies)) / sd.integrate()) < TOL_SP
+    assert abs(sd.integrate() - sd.integrate(distribution_function=ones)) < TOL
+
+
+def test_bounds_none_lanczos():
+    """bounds=None: the GPU Lanczos bounds put the rescaled spectrum inside
+    (-1, 1) and agree with ARPACK's to the eps/2 tolerance
+    (kwant/tests/test_kpm.py:356-375, 695-711)."""
+    import scipy.sparse.linalg as sla
+    h = gue(20, 14)
+    eps = 0.05
+    sd = kpm().SpectralDensity(h, eps=eps, rng=1)
+    lam = np.linalg.eigvalsh((h - sd._b * np.eye(20)) / sd._a)
+    assert np.all(np.abs(lam) < 1)
+    rs = np.random.RandomState(1)
+    v0 = np.exp(2j * np.pi * rs.random_sample(20))
+    lmax = sla.eigsh(h, k=1, which='LA', return_eigenvectors=False, tol=eps / 2, v0=v0)[0]
+    lmin = sla.eigsh(h, k=1, which='SA', return_eigenvectors=False, tol=eps / 2, v0=v0)[0]
+    assert abs(sd.bounds[1] - sd.bounds[0] - (lmax - lmin) * 2 / (2 - eps)) < eps * (lmax - lmin)
+    sd2 = kpm().SpectralDensity(h, bounds=(lmin, lmax), eps=eps, rng=1)
+    np.testing.assert_allclose(sd.densities, sd2.densities, rtol=0, atol=5e-2 * sd2.densities.max())
+
+
+def test_invalid_input():
+    """kwant/tests/test_kpm.py:538-568 and the API errors of :306-321."""
+    h = gue(20, 15)
+    k = kpm()
+    with pytest.raises(ValueError):
+        k.SpectralDensity('not a hamiltonian')
+    with pytest.raises(ValueError):
+        k.SpectralDensity(h, operator='not an operator')
+    with pytest.raises(ValueError):
+        k.SpectralDensity(np.identity(20, dtype=complex))
+    with pytest.raises(TypeError):
+        k.SpectralDensity(h, num_moments=10, energy_resolution=0.1)
+    s = k.SpectralDensity(h)
+    with pytest.raises(TypeError):
+        s.add_moments(num_moments=20, energy_resolution=0.1)
+    for v in (-20, 0, 10.5):
+        for p in ('num_vectors', 'num_moments'):
+            with pytest.raises(ValueError):
+                k.SpectralDensity(h, **{p: v})
+        with pytest.raises(ValueError):
+            s.add_moments(num_moments=v)
+        with pytest.raises(ValueError):
+            s.add_vectors(num_vectors=v)
+    for v in (-1, -0.5, 0):
+        with pytest.raises(ValueError):
+            k.SpectralDensity(h, eps=v)
+        with pytest.raises(ValueError):
+            s.add_moments(energy_resolution=v)
+    with pytest.raises(ValueError):
+        s.add_moments(energy_resolution=10)
+
+
+def test_vector_factory_counts():
+    """kwant/tests/test_kpm.py:161-189."""
+    h = gue(20, 16)
+    k = kpm()
+    vectors = np.identity(20)
+    for n, want in ((None, 20), (3, 3)):
+        sd = k.SpectralDensity(h, vector_factory=vectors, num_vectors=n)
+        assert sd.num_vectors == want
+    sd.add_vectors(num_vectors=2)
+    assert sd.num_vectors == 5
+
+
+def test_convergence_to_eigenvalues():
+    """kwant/tests/test_kpm.py:571-592: DOS peaks approach the eigenvalues."""
+    rng = np.random.RandomState(1)
+    devs = []
+    for moments in (200, 2200):
+        d = []
+        for _ in range(3):
+            h = brute.gaussian_unitary(20, rng)
+            sd = kpm().SpectralDensity(h, num_moments=moments, num_vectors=10, rng=rng)
+            rho = sd.densities
+            grad = np.gradient(rho)
+            idx = np.where(np.diff(np.sign(grad)))[0]
+            idx = idx[rho[idx] > 0.005 * rho.max()]
+            idx = idx[np.gradient(grad)[idx] < 0]
+            lam = np.linalg.eigvalsh(h)
+            d.append(max(np.min(np.abs(sd.energies[idx] - l)) for l in lam) / (2 * sd._a))
+        devs.append(np.mean(d))
+    assert devs[1] < devs[0]
+    assert devs[1] < 1e-3
+
+
+def test_ensemble_segments_equal_separate_runs():
+    """Three systems in one block-diagonal launch (separate (a, b) and
+    moments per segment) give each system's own moments."""
+    import scipy.sparse as sp
+    import torch
+    from kwant_amd import _engine as eng
+    hs = [gue(n, 20 + n) for n in (30, 45, 17)]
+    scales = [(1.05 * np.max(np.abs(np.linalg.eigvalsh(x))), 0.1 * i) for i, x in enumerate(hs)]
+    big = sp.block_diag([sp.csr_matrix(x) for x in hs]).tocsr()
+    seg = np.cumsum([0] + [x.shape[0] for x in hs])
+    width = 4
+    rng = np.random.RandomState(3)
+    vecs = np.exp(2j * np.pi * rng.random_sample((seg[-1], width)))
+    dmat = eng.DeviceCSR(big, 'complex128')
+    plan = eng.Plan(seg[-1], width, eng.KPM_C128, seg_rows=seg)
+    plan.set_scale(np.array(scales).ravel())
+    dev = eng.torch_device()
+    ring = torch.zeros((2, seg[-1], width), dtype=torch.complex128, device=dev)
+    ring[0] = torch.from_numpy(vecs).to(dev)
+    plan.set_step(0)
+    steps = 20
+    dots = torch.zeros((steps, 3, width, 4), dtype=torch.float64, device=dev)
+    plan.steps(dmat, ring, seg[-1] * width, 2, steps, eng.DOTS_DOUBLING, dots=dots)
+    assert plan.get_step() == steps
+    d = dots.cpu().numpy()
+    for s, x in enumerate(hs):
+        a, b = scales[s]
+        for c in range(width):
+            r = vecs[seg[s]:seg[s + 1], c]
+            mom = brute.moments(x, a, b, r, 2 * steps)
+            mu0 = d[0, s, c, 0]
+            mu1 = d[0, s, c, 2] - 1j * d[0, s, c, 3]
+            got = [mu0, mu1]
+            for n in range(1, steps):
+                got += [2 * d[n, s, c, 0] - mu0, 2 * (d[n, s, c, 2] + 1j * d[n, s, c, 3]) - mu1]
+            assert rel(got, mom, x.shape[0]) < 1e-11, (s, c)
