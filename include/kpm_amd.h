@@ -70,12 +70,14 @@ int kpm_abi_version(void);
 int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch);
 
 /* Number of recursion kernel variants for a (width, value_type) pair, and
- * the launch shape of each: lanes per row, complex values per lane, waves per
- * workgroup, rows per workgroup.  Used by the host autotuner. */
+ * the launch shape of each: lanes per matrix row, complex values per lane,
+ * waves per workgroup, and the residency rounds the grid is sized for (rows
+ * per workgroup = rows / (CUs x resident workgroups per CU x rounds)).  Used
+ * by the host autotuner. */
 int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count);
 int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant,
                      int32_t* lanes_per_row, int32_t* vec,
-                     int32_t* waves, int32_t* rows_per_block);
+                     int32_t* waves, int32_t* rounds);
 
 /* Create a recursion plan for an N x width vector block whose rows are split
  * into n_segments independent systems (segment s owns rows
@@ -95,8 +97,8 @@ int kpm_plan_destroy(kpm_plan* plan);
  * Synchronous upload. */
 int kpm_plan_set_scale(kpm_plan* plan, const double* h_ab);
 
-/* Device step counter n: the next kpm_cheb_steps launch computes a_{n+1}.
- * set: asynchronous on stream.  get: synchronises the stream. */
+/* Step counter n of the plan (host side): the next kpm_cheb_steps launch
+ * computes a_{n+1}.  The stream argument is unused (kept for symmetry). */
 int kpm_plan_set_step(kpm_plan* plan, int32_t n, kpm_stream_t stream);
 int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream);
 
@@ -107,8 +109,10 @@ int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream);
  * a_k lives in ring slot (k mod slots): device buffer `ring`, slot k at
  * ring + k*slot_stride complex elements, each slot an n_rows x width block.
  * With slots == 2 the update is in place (a_{n+1} overwrites a_{n-1}).
- * The step index n is read from the plan's device counter and incremented by
- * each launch, so consecutive launches carry identical arguments.
+ * The step index n comes from the plan's counter, which each step advances.
+ * A step with a dots mode is two launches: the streaming kernel, which
+ * writes one (sum, error) partial per workgroup, and a reduction kernel that
+ * sums them in workgroup order into the step's record.
  *
  * dots_mode KPM_DOTS_DOUBLING writes, per step, segment and column c,
  *   d_dots[((n - out_base) * n_segments + s) * width + c] = 4 doubles
@@ -118,9 +122,9 @@ int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream);
  * { <w|a_n>, <w|a_{n+1}> } with w the device block `w` (the matrix-operator
  * moments vdot(a_0, O a_n) of kwant/kpm.py:193-195, 484 with w = O^H a_0).
  * Sums are FP64 and deterministic (fixed order), so repeated runs and
- * resumed runs are bitwise identical.  Steps with (n - out_base) outside
- * [0, out_cap) write no dots and set an error flag reported by the next
- * kpm_plan_get_step. */
+ * resumed runs are bitwise identical.  The call fails with KPM_ERR_ARG
+ * (nothing launched) when a step's slot (n - out_base) would fall outside
+ * [0, out_cap). */
 int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring,
                    int64_t slot_stride, int32_t slots, const void* w,
                    double* d_dots, int32_t out_base, int32_t out_cap,
@@ -161,6 +165,9 @@ int kpm_spmm(const kpm_csr* op, const void* x, int64_t x_stride, void* y,
 #define KPM_GEMM_FP64 0
 #define KPM_GEMM_FP32 1
 #define KPM_GEMM_FP32X3 2
+/* OR-ed into precision: use the vector-ALU (FMA) kernel instead of the
+ * matrix cores -- a cross-check of the MFMA path. */
+#define KPM_GEMM_VECTOR_ALU 16
 int kpm_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb,
                   double* mu, int64_t ldc, int32_t m_rows, int32_t n_rows,
                   int64_t k, int32_t precision, int32_t k_chunk,

@@ -24,6 +24,7 @@ DOTS_PROJECT = 2
 GEMM_FP64 = 0
 GEMM_FP32 = 1
 GEMM_FP32X3 = 2
+GEMM_VECTOR_ALU = 16
 
 # Exported symbols, in the order include/kpm_amd.h declares them.
 EXPORTS = (
@@ -132,7 +133,7 @@ def variant_count(width, value_type):
 
 
 def variant_info(width, value_type, variant):
-    """(lanes per row, complex values per lane, waves, rows per workgroup)."""
+    """(lanes per row, complex values per lane, waves, residency rounds)."""
     vals = [ctypes.c_int32() for _ in range(4)]
     check(load_library().kpm_variant_info(width, value_type, variant,
                                           *[ctypes.byref(v) for v in vals]),
@@ -339,7 +340,7 @@ def corr_gemm(a, b, mu, precision, k_chunk=256, work=None):
         raise ValueError("corr_gemm shape mismatch")
     if mu.dtype != torch.complex128:
         raise ValueError("mu must be complex128")
-    want = torch.complex64 if precision == GEMM_FP32 else torch.complex128
+    want = torch.complex64 if (precision & ~GEMM_VECTOR_ALU) == GEMM_FP32 else torch.complex128
     if a.dtype != want or b.dtype != want:
         raise ValueError("operand dtype does not match the GEMM precision")
     if a.stride(1) != 1 or b.stride(1) != 1 or mu.stride(1) != 1:

@@ -27,11 +27,6 @@ int check_hip(hipError_t e, const char* what) {
   return fail(KPM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-__global__ void set_ctrl_kernel(int32_t* ctrl, int32_t n) {
-  ctrl[0] = n;
-  ctrl[1] = 0;
-}
-
 }  // namespace kpm
 
 using namespace kpm;
@@ -39,19 +34,13 @@ using namespace kpm;
 struct kpm_plan {
   int32_t n_rows = 0, width = 0, value_type = 0, nseg = 0;
   KernelShape shape{};
-  int32_t nblocks = 0, nblocks_launch = 0, ngroups = 0;
+  int32_t nblocks = 0, nblocks_launch = 0;
+  int32_t step = 0;  // the next kpm_cheb_steps launch computes a_{step+1}
   int2* blk_rows = nullptr;
   int32_t* blk_seg = nullptr;
-  int32_t* blk_grp = nullptr;
-  int32_t* grp_blk = nullptr;
-  int32_t* seg_grp = nullptr;
+  int32_t* seg_blk = nullptr;
   double* scale = nullptr;
   double* part = nullptr;
-  double* gpart = nullptr;
-  unsigned* grp_cnt = nullptr;
-  unsigned* seg_cnt = nullptr;
-  unsigned* done_cnt = nullptr;
-  int32_t* ctrl = nullptr;
   bool scale_set = false;
 };
 
@@ -75,34 +64,20 @@ int upload(T* d, const std::vector<T>& h, const char* what) {
 
 void free_plan(kpm_plan* p) {
   if (!p) return;
-  hipFree(p->blk_rows);
-  hipFree(p->blk_seg);
-  hipFree(p->blk_grp);
-  hipFree(p->grp_blk);
-  hipFree(p->seg_grp);
-  hipFree(p->scale);
-  hipFree(p->part);
-  hipFree(p->gpart);
-  hipFree(p->grp_cnt);
-  hipFree(p->seg_cnt);
-  hipFree(p->done_cnt);
-  hipFree(p->ctrl);
+  (void)hipFree(p->blk_rows);
+  (void)hipFree(p->blk_seg);
+  (void)hipFree(p->seg_blk);
+  (void)hipFree(p->scale);
+  (void)hipFree(p->part);
   delete p;
 }
 
 int fill_params(const kpm_plan* plan, StepParams& p) {
   p.blk_rows = plan->blk_rows;
   p.blk_seg = plan->blk_seg;
-  p.blk_grp = plan->blk_grp;
-  p.grp_blk = plan->grp_blk;
-  p.seg_grp = plan->seg_grp;
+  p.seg_blk = plan->seg_blk;
   p.scale = plan->scale;
   p.part = plan->part;
-  p.gpart = plan->gpart;
-  p.grp_cnt = plan->grp_cnt;
-  p.seg_cnt = plan->seg_cnt;
-  p.done_cnt = plan->done_cnt;
-  p.ctrl = plan->ctrl;
   p.nblocks = plan->nblocks;
   p.nblocks_launch = plan->nblocks_launch;
   p.nseg = plan->nseg;
@@ -152,7 +127,7 @@ int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant, int32_t
   if (lanes_per_row) *lanes_per_row = ks[variant].lpr;
   if (vec) *vec = ks[variant].vec;
   if (waves) *waves = ks[variant].waves;
-  if (rows_per_block) *rows_per_block = ks[variant].iters * ks[variant].waves * 64 / ks[variant].lpr;
+  if (rows_per_block) *rows_per_block = ks[variant].rounds;
   return KPM_OK;
 }
 
@@ -182,51 +157,48 @@ int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n
   p->value_type = value_type;
   p->nseg = n_segments;
   p->shape = ks[variant];
-  const int rpb = p->shape.iters * p->shape.waves * 64 / p->shape.lpr;
+  int occ = 0, cus = 0, dev = 0;
+  int rc0 = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (!rc0)
+    rc0 = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                    "hipDeviceGetAttribute");
+  if (!rc0) rc0 = blocks_per_cu(value_type, p->shape, &occ);
+  if (rc0) {
+    delete p;
+    return rc0;
+  }
+  if (occ < 1) occ = 1;
+  const int64_t rows_per_iter = int64_t(p->shape.waves) * 64 / p->shape.lpr;
+  const int64_t target = int64_t(cus) * occ * p->shape.rounds;
+  int64_t rpb = (int64_t(n_rows) + target - 1) / target;
+  rpb = (rpb + rows_per_iter - 1) / rows_per_iter * rows_per_iter;
+  if (rpb < rows_per_iter) rpb = rows_per_iter;
+  p->shape.iters = int(rpb / rows_per_iter);
 
   std::vector<int2> blk_rows;
-  std::vector<int32_t> blk_seg, blk_grp, grp_blk, seg_grp;
+  std::vector<int32_t> blk_seg, seg_blk;
   for (int s = 0; s < n_segments; ++s) {
-    seg_grp.push_back(int32_t(grp_blk.size()));
+    seg_blk.push_back(int32_t(blk_rows.size()));
     const int r0 = seg_row_begin[s], r1 = seg_row_begin[s + 1];
-    int nb_in_seg = 0;
-    for (int r = r0; r < r1; r += rpb) {
-      if (nb_in_seg % kGroupBlocks == 0) grp_blk.push_back(int32_t(blk_rows.size()));
-      blk_rows.push_back(make_int2(r, r + rpb < r1 ? r + rpb : r1));
+    for (int64_t r = r0; r < r1; r += rpb) {
+      blk_rows.push_back(make_int2(int(r), int(r + rpb < r1 ? r + rpb : r1)));
       blk_seg.push_back(s);
-      blk_grp.push_back(int32_t(grp_blk.size()) - 1);
-      ++nb_in_seg;
     }
   }
-  seg_grp.push_back(int32_t(grp_blk.size()));
-  grp_blk.push_back(int32_t(blk_rows.size()));
+  seg_blk.push_back(int32_t(blk_rows.size()));
   p->nblocks = int32_t(blk_rows.size());
-  p->ngroups = int32_t(grp_blk.size()) - 1;
   p->nblocks_launch = (p->nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;
 
   const int maxnc = 4;
   int rc = KPM_OK;
   if (!rc) rc = dev_alloc(&p->blk_rows, blk_rows.size(), "blk_rows");
   if (!rc) rc = dev_alloc(&p->blk_seg, blk_seg.size(), "blk_seg");
-  if (!rc) rc = dev_alloc(&p->blk_grp, blk_grp.size(), "blk_grp");
-  if (!rc) rc = dev_alloc(&p->grp_blk, grp_blk.size(), "grp_blk");
-  if (!rc) rc = dev_alloc(&p->seg_grp, seg_grp.size(), "seg_grp");
+  if (!rc) rc = dev_alloc(&p->seg_blk, seg_blk.size(), "seg_blk");
   if (!rc) rc = dev_alloc(&p->scale, size_t(2) * n_segments, "scale");
   if (!rc) rc = dev_alloc(&p->part, size_t(p->nblocks) * maxnc * width * 2, "partials");
-  if (!rc) rc = dev_alloc(&p->gpart, size_t(p->ngroups) * maxnc * width * 2, "group partials");
-  if (!rc) rc = dev_alloc(&p->grp_cnt, size_t(p->ngroups), "group counters");
-  if (!rc) rc = dev_alloc(&p->seg_cnt, size_t(n_segments), "segment counters");
-  if (!rc) rc = dev_alloc(&p->done_cnt, 1, "done counter");
-  if (!rc) rc = dev_alloc(&p->ctrl, 2, "step counter");
   if (!rc) rc = upload(p->blk_rows, blk_rows, "upload blk_rows");
   if (!rc) rc = upload(p->blk_seg, blk_seg, "upload blk_seg");
-  if (!rc) rc = upload(p->blk_grp, blk_grp, "upload blk_grp");
-  if (!rc) rc = upload(p->grp_blk, grp_blk, "upload grp_blk");
-  if (!rc) rc = upload(p->seg_grp, seg_grp, "upload seg_grp");
-  if (!rc) rc = check_hip(hipMemset(p->grp_cnt, 0, sizeof(unsigned) * p->ngroups), "memset");
-  if (!rc) rc = check_hip(hipMemset(p->seg_cnt, 0, sizeof(unsigned) * n_segments), "memset");
-  if (!rc) rc = check_hip(hipMemset(p->done_cnt, 0, sizeof(unsigned)), "memset");
-  if (!rc) rc = check_hip(hipMemset(p->ctrl, 0, 2 * sizeof(int32_t)), "memset");
+  if (!rc) rc = upload(p->seg_blk, seg_blk, "upload seg_blk");
   if (!rc) rc = check_hip(hipDeviceSynchronize(), "plan setup");
   if (rc) {
     free_plan(p);
@@ -257,22 +229,17 @@ int kpm_plan_set_scale(kpm_plan* plan, const double* h_ab) {
 }
 
 int kpm_plan_set_step(kpm_plan* plan, int32_t n, kpm_stream_t stream) {
+  (void)stream;
   if (!plan) return fail(KPM_ERR_ARG, "plan is NULL");
   if (n < 0) return fail(KPM_ERR_ARG, "step must be non-negative");
-  hipLaunchKernelGGL(set_ctrl_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, plan->ctrl, n);
-  return check_hip(hipGetLastError(), "set_ctrl_kernel launch");
+  plan->step = n;
+  return KPM_OK;
 }
 
 int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream) {
+  (void)stream;
   if (!plan || !n) return fail(KPM_ERR_ARG, "NULL argument");
-  int32_t h[2] = {0, 0};
-  int rc = check_hip(hipMemcpyAsync(h, plan->ctrl, sizeof(h), hipMemcpyDeviceToHost,
-                                    (hipStream_t)stream),
-                     "read step counter");
-  if (!rc) rc = check_hip(hipStreamSynchronize((hipStream_t)stream), "stream sync");
-  if (rc) return rc;
-  *n = h[0];
-  if (h[1]) return fail(KPM_ERR_ARG, "a step ran with its dots slot outside [0, out_cap)");
+  *n = plan->step;
   return KPM_OK;
 }
 
@@ -306,6 +273,9 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
     return fail(KPM_ERR_ARG, "dots output buffer missing");
   if (dots_mode == KPM_DOTS_PROJECT && !w) return fail(KPM_ERR_ARG, "projection vector missing");
   if (n_steps < 0) return fail(KPM_ERR_ARG, "n_steps must be non-negative");
+  if (dots_mode != KPM_DOTS_NONE &&
+      (plan->step - out_base < 0 || plan->step + n_steps - out_base > out_cap))
+    return fail(KPM_ERR_ARG, "dots slots [step - out_base, + n_steps) outside [0, out_cap)");
   StepParams p{};
   p.indptr = h->indptr;
   p.indices = h->indices;
@@ -314,13 +284,15 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
   p.slot_bytes = slot_stride * int64_t(elem_bytes(plan->value_type));
   p.slots = slots;
   p.w = w;
-  p.dots = d_dots;
-  p.out_base = out_base;
-  p.out_cap = out_cap;
   fill_params(plan, p);
+  const int64_t rec = int64_t(plan->nseg) * plan->width * 4;
   for (int i = 0; i < n_steps; ++i) {
-    rc = launch_cheb(plan->value_type, plan->shape, dots_mode, false, p, (hipStream_t)stream);
+    const int n = plan->step;
+    p.dots = d_dots ? d_dots + int64_t(n - out_base) * rec : nullptr;
+    rc = launch_cheb(plan->value_type, plan->shape, dots_mode, false, n, p,
+                     (hipStream_t)stream);
     if (rc) return rc;
+    plan->step = n + 1;
   }
   return KPM_OK;
 }
@@ -339,10 +311,9 @@ int kpm_block_dots(kpm_plan* plan, const void* x, const void* w, double* d_out,
   p.slots = 1;
   p.w = w;
   p.dots = d_out;
-  p.out_base = 0;
-  p.out_cap = 1;
   fill_params(plan, p);
-  return launch_cheb(plan->value_type, plan->shape, dots_mode, true, p, (hipStream_t)stream);
+  return launch_cheb(plan->value_type, plan->shape, dots_mode, true, 0, p,
+                     (hipStream_t)stream);
 }
 
 int kpm_spmm(const kpm_csr* op, const void* x, int64_t x_stride, void* y, int64_t y_stride,
@@ -365,7 +336,9 @@ int kpm_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb, double
   if (!A || !B || !mu) return fail(KPM_ERR_ARG, "NULL argument");
   if (m_rows <= 0 || n_rows <= 0 || k <= 0) return fail(KPM_ERR_ARG, "empty GEMM");
   if (lda < k || ldb < k || ldc < n_rows) return fail(KPM_ERR_ARG, "leading dimension too small");
-  if (precision < KPM_GEMM_FP64 || precision > KPM_GEMM_FP32X3)
+  const int base = precision & ~KPM_GEMM_VECTOR_ALU;
+  if (base < KPM_GEMM_FP64 || base > KPM_GEMM_FP32X3 ||
+      (precision & ~(KPM_GEMM_VECTOR_ALU | 3)) != 0)
     return fail(KPM_ERR_ARG, "unknown precision");
   if (work_elems < 0) return fail(KPM_ERR_ARG, "negative workspace size");
   return launch_corr_gemm(A, lda, B, ldb, mu, ldc, m_rows, n_rows, k, precision, k_chunk,

@@ -13,13 +13,15 @@
 // write of a_{n+1} into the same slot (in place).  The two dot products of
 // the doubling identity (kwant/kpm.py:470-473) are accumulated in FP64 in
 // the same pass: one lane owns VEC columns of the block, so per-column sums
-// need no shuffles until the end of the workgroup; workgroup partials are
-// then reduced by the last workgroup of a 32-block group and the group
-// partials by the last group of the segment (atomic tickets, fixed summation
-// order => bitwise reproducible).  Every addition is a TwoSum and the
+// need no shuffles until the end of the workgroup.  Workgroup partials are
+// stored with plain stores and summed in block order by cheb_reduce, a small
+// kernel launched after each step (no cross-workgroup synchronisation inside
+// the streaming kernel; on a multi-XCD part an agent-scope fence per
+// workgroup costs an L2 write-back).  Every addition is a TwoSum and the
 // (sum, error) pair is carried through every level and rounded once at the
 // end, so a column's result is the correctly rounded sum of its terms (up to
-// the error of the error term) whatever the launch shape or block width.
+// the error of the error term) whatever the launch shape or block width, and
+// repeated or resumed runs are bitwise identical.
 //
 // Wave layout: LPR lanes per matrix row, VEC complex values per lane
 // (W = LPR * VEC), 64 / LPR rows per wavefront.  Loads are 8 or 16 bytes per
@@ -140,20 +142,6 @@ __device__ inline void acc2(double& s, double& c, double t) {
   c += e;
 }
 
-__device__ inline unsigned ticket(unsigned* ctr) {
-  return __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void store_rel(unsigned* p, unsigned v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void store_coh(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline double load_coh(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Number of FP64 accumulators per column for a dots mode.
 template <int MODE>
 struct NComp {
@@ -167,9 +155,161 @@ __device__ inline int out_pos(int c) {
   return c;
 }
 
+// sum_k H[row, j_k] x[j_k, cols] for the RPW (1 or 2) consecutive rows of a
+// wavefront, row0 wave-uniform.  indptr and every entry are read with scalar
+// loads (uniform addresses); each lane group then selects its own row's
+// entry, so a chunk of 4 entries per row costs one scalar wait and its 4
+// gathers are in flight together.  Surplus slots of a chunk reuse a valid
+// entry with a zero weight.
+template <typename T, int VEC, int RPW>
+__device__ inline CLane<T, VEC> spmm_rows_scalar(const int32_t* __restrict__ indptr,
+                                                 const int32_t* __restrict__ indices,
+                                                 const void* __restrict__ values, const T* cur,
+                                                 int row0, int rows_left, int grp, int col,
+                                                 int W) {
+  using IO = LaneIO<T, VEC>;
+  const int nrows = rows_left < RPW ? rows_left : RPW;
+  int kb[RPW + 1];
+  kb[0] = indptr[row0];
+#pragma unroll
+  for (int r = 1; r <= RPW; ++r) kb[r] = r <= nrows ? indptr[row0 + r] : kb[r - 1];
+  int len[RPW];
+  int nmax = 0;
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    len[r] = kb[r + 1] - kb[r];
+    nmax = len[r] > nmax ? len[r] : nmax;
+  }
+  const int safe = kb[RPW] - 1;  // a valid entry whenever nmax > 0
+  int my_len = len[0];
+#pragma unroll
+  for (int r = 1; r < RPW; ++r)
+    if (grp == r) my_len = len[r];
+  CLane<T, VEC> acc;
+  zero(acc);
+  for (int c = 0; c < nmax; c += 4) {
+    int j[4];
+    T hr[4], hi[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int jr[RPW];
+      T hrr[RPW], hir[RPW];
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        const int e = c + u < len[r] ? kb[r] + c + u : safe;
+        jr[r] = indices[e];
+        load_value<T>(values, e, hrr[r], hir[r]);
+        if (c + u >= len[r]) hrr[r] = hir[r] = T(0);
+      }
+      j[u] = jr[0];
+      hr[u] = hrr[0];
+      hi[u] = hir[0];
+#pragma unroll
+      for (int r = 1; r < RPW; ++r)
+        if (grp == r) {
+          j[u] = jr[r];
+          hr[u] = hrr[r];
+          hi[u] = hir[r];
+        }
+    }
+    CLane<T, VEC> x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = IO::load(cur + 2 * (int64_t(j[u]) * W + col));
+      if (c + u >= my_len) zero(x[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        acc.re[v] = fma(hr[u], x[u].re[v], acc.re[v]);
+        acc.re[v] = fma(-hi[u], x[u].im[v], acc.re[v]);
+        acc.im[v] = fma(hr[u], x[u].im[v], acc.im[v]);
+        acc.im[v] = fma(hi[u], x[u].re[v], acc.im[v]);
+      }
+  }
+  return acc;
+}
+
+// The same product for one row per lane group with per-lane (vector) loads of
+// the matrix entries (several rows per wavefront).  Also loads the row's own
+// a_n (xs) and a_{n-1} (xp), after the first chunk of matrix entries.
+template <typename T, int VEC>
+__device__ inline CLane<T, VEC> spmm_row_vector(const int32_t* __restrict__ indptr,
+                                                const int32_t* __restrict__ indices,
+                                                const void* __restrict__ values, const T* cur,
+                                                int row, bool live, int col, int W,
+                                                CLane<T, VEC>& xs, CLane<T, VEC>& xp,
+                                                const T* prv, int64_t roff, bool has_prev) {
+  using IO = LaneIO<T, VEC>;
+  const int k0 = live ? indptr[row] : 0;
+  const int k1 = live ? indptr[row + 1] : 0;
+  int j0[4];
+  T hr0[4], hi0[4];
+  if (k0 < k1) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kk = k0 + u < k1 ? k0 + u : k1 - 1;
+      j0[u] = indices[kk];
+      load_value<T>(values, kk, hr0[u], hi0[u]);
+      if (k0 + u >= k1) hr0[u] = hi0[u] = T(0);
+    }
+  }
+  xs = IO::load(cur + roff);
+  if (has_prev)
+    xp = IO::load_nt(prv + roff);
+  else
+    zero(xp);
+  CLane<T, VEC> acc;
+  zero(acc);
+  for (int k = k0; k < k1; k += 4) {
+    T hr[4], hi[4];
+    int j[4];
+    CLane<T, VEC> x[4];
+    if (k == k0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        j[u] = j0[u];
+        hr[u] = hr0[u];
+        hi[u] = hi0[u];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kk = k + u < k1 ? k + u : k1 - 1;
+        j[u] = indices[kk];
+        load_value<T>(values, kk, hr[u], hi[u]);
+        if (k + u >= k1) hr[u] = hi[u] = T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = IO::load(cur + 2 * (int64_t(j[u]) * W + col));
+      if (k + u >= k1) zero(x[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        acc.re[v] = fma(hr[u], x[u].re[v], acc.re[v]);
+        acc.re[v] = fma(-hi[u], x[u].im[v], acc.re[v]);
+        acc.im[v] = fma(hr[u], x[u].im[v], acc.im[v]);
+        acc.im[v] = fma(hi[u], x[u].re[v], acc.im[v]);
+      }
+  }
+  return acc;
+}
+
+// The CSR arrays are separate __restrict__ arguments: nothing the kernel
+// stores can alias them, so for one or two rows per wavefront (LPR >= 32) the
+// compiler reads them with scalar loads, which complete on their own counter
+// instead of queueing behind the vector gathers.
 template <typename T, int VEC, int LPR, int WAVES, int MODE>
-__global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p, int iters,
-                                                          int only_dots) {
+__global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
+                                                          const int32_t* __restrict__ indptr,
+                                                          const int32_t* __restrict__ indices,
+                                                          const void* __restrict__ values,
+                                                          int n, int iters, int only_dots) {
   constexpr int W = LPR * VEC;
   constexpr int RPW = 64 / LPR;
   constexpr int NC = NComp<MODE>::value;
@@ -186,8 +326,6 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p, int iter
   const int grp = lane / LPR;
   const int col = (lane % LPR) * VEC;
 
-  const int n = __builtin_amdgcn_readfirstlane(
-      __hip_atomic_load(p.ctrl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   const int seg = p.blk_seg[lb];
   const int2 rr = p.blk_rows[lb];
   const double sa = p.scale[2 * seg];
@@ -211,53 +349,44 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p, int iter
     }
 
   for (int it = 0; it < iters; ++it) {
-    const int row = rr.x + (it * WAVES + wave) * RPW + grp;
-    if (row >= rr.y) break;
-    const int64_t roff = 2 * (int64_t(row) * W + col);
-    CLane<T, VEC> xs = IO::load(cur + roff);
+    // rows row0 .. row0 + RPW - 1 belong to this wave; lane group grp owns one
+    const int row0 = __builtin_amdgcn_readfirstlane(rr.x + (it * WAVES + wave) * RPW);
+    if (row0 >= rr.y) break;
+    const int row = row0 + grp;
+    const bool live = row < rr.y;
+    const int64_t roff = 2 * (int64_t(live ? row : row0) * W + col);
     CLane<T, VEC> y;
     zero(y);
+    CLane<T, VEC> xs;
     if (!only_dots) {
-      CLane<T, VEC> xp;
-      if (n > 0)
-        xp = IO::load_nt(prv + roff);
-      else
-        zero(xp);
-      const int k0 = p.indptr[row];
-      const int k1 = p.indptr[row + 1];
-      CLane<T, VEC> acc;
-      zero(acc);
-      for (int k = k0; k < k1; k += 4) {
-        T hr[4], hi[4];
-        CLane<T, VEC> x[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (k + u < k1) {
-            const int j = p.indices[k + u];
-            load_value<T>(p.values, k + u, hr[u], hi[u]);
-            x[u] = IO::load(cur + 2 * (int64_t(j) * W + col));
-          } else {
-            hr[u] = T(0);
-            hi[u] = T(0);
-            zero(x[u]);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) {
-            acc.re[v] = fma(hr[u], x[u].re[v], acc.re[v]);
-            acc.re[v] = fma(-hi[u], x[u].im[v], acc.re[v]);
-            acc.im[v] = fma(hr[u], x[u].im[v], acc.im[v]);
-            acc.im[v] = fma(hi[u], x[u].re[v], acc.im[v]);
-          }
+      CLane<T, VEC> acc, xp;
+      if constexpr (RPW <= 2) {
+        // matrix entries come through the scalar unit: stream the rows first
+        xs = IO::load(cur + roff);
+        if (n > 0)
+          xp = IO::load_nt(prv + roff);
+        else
+          zero(xp);
+        acc = spmm_rows_scalar<T, VEC, RPW>(indptr, indices, values, cur, row0,
+                                             rr.y - row0, grp, col, W);
+      } else {
+        // per-lane matrix entries are vector loads, which complete in order:
+        // request them before the streaming row loads
+        acc = spmm_row_vector<T, VEC>(indptr, indices, values, cur, live ? row : row0, live,
+                                      col, W, xs, xp, prv, roff, n > 0);
       }
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {
         y.re[v] = fma(acc.re[v] - bsh * xs.re[v], mul, -xp.re[v]);
         y.im[v] = fma(acc.im[v] - bsh * xs.im[v], mul, -xp.im[v]);
       }
-      IO::store_nt(nxt + roff, y);
+      if (live) IO::store_nt(nxt + roff, y);
+    } else {
+      xs = IO::load(cur + roff);
+    }
+    if (!live) {
+      zero(xs);
+      zero(y);
     }
     if (MODE == KPM_DOTS_DOUBLING) {
 #pragma unroll
@@ -283,18 +412,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p, int iter
     }
   }
 
-  if (MODE == KPM_DOTS_NONE) {
-    // Only the step counter: the last workgroup to finish advances it.
-    __syncthreads();
-    if (tid == 0) {
-      const unsigned t = ticket(p.done_cnt);
-      if (t == unsigned(p.nblocks - 1)) {
-        store_rel(p.done_cnt, 0u);
-        __hip_atomic_store(p.ctrl, n + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    return;
-  }
+  if (MODE == KPM_DOTS_NONE) return;
 
   // ---- workgroup reduction: row groups of a wave, then waves via LDS ----
 #pragma unroll
@@ -311,7 +429,6 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p, int iter
   }
   __shared__ double red_s[WAVES][NC][W];
   __shared__ double red_c[WAVES][NC][W];
-  __shared__ int s_flag;
   if (grp == 0) {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -322,7 +439,8 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p, int iter
       }
   }
   __syncthreads();
-  // Partials are (sum, error) pairs: part[((blk * NC + c) * W + col) * 2 + {0,1}].
+  // Workgroup partials are (sum, error) pairs:
+  // part[((blk * NC + c) * W + col) * 2 + {0, 1}]; cheb_reduce sums them.
   for (int t = tid; t < NC * W; t += WAVES * 64) {
     const int c = t / W, cc = t % W;
     double s = 0.0, e = 0.0;
@@ -332,80 +450,49 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p, int iter
       e += red_c[wv2][c][cc];
     }
     double* dst = p.part + ((int64_t(lb) * NC + c) * W + cc) * 2;
-    store_coh(dst, s);
-    store_coh(dst + 1, e);
+    dst[0] = s;
+    dst[1] = e;
   }
-  __threadfence();
-  __syncthreads();
+}
 
-  // ---- group level: the last workgroup of the group sums its partials ----
-  const int gi = p.blk_grp[lb];
-  if (tid == 0) {
-    const unsigned t = ticket(p.grp_cnt + gi);
-    const int gsize = p.grp_blk[gi + 1] - p.grp_blk[gi];
-    s_flag = (t == unsigned(gsize - 1));
-    if (s_flag) store_rel(p.grp_cnt + gi, 0u);
-  }
-  __syncthreads();
-  if (!s_flag) return;
-  __threadfence();
-  {
-    const int b0 = p.grp_blk[gi], b1 = p.grp_blk[gi + 1];
-    for (int t = tid; t < NC * W; t += WAVES * 64) {
-      const int c = t / W, cc = t % W;
-      double s = 0.0, e = 0.0;
-      for (int b = b0; b < b1; ++b) {
-        const double* src = p.part + ((int64_t(b) * NC + c) * W + cc) * 2;
-        acc2(s, e, load_coh(src));
-        e += load_coh(src + 1);
-      }
-      double* dst = p.gpart + ((int64_t(gi) * NC + c) * W + cc) * 2;
-      store_coh(dst, s);
-      store_coh(dst + 1, e);
+// Sum the workgroup partials of every segment in block order (one launch after
+// the step; the kernel boundary makes all partials visible) and write the
+// per-step dots record.  Grid: (segments, ceil(NC * W / 64)); the 4 waves of a
+// workgroup take every 4th block of the segment, then combine in LDS in wave
+// order, so the summation order is fixed.
+template <int MODE>
+__global__ __launch_bounds__(256) void cheb_reduce(const double* __restrict__ part,
+                                                   const int32_t* __restrict__ seg_blk,
+                                                   int W, double* __restrict__ out) {
+  constexpr int NC = NComp<MODE>::value;
+  const int seg = blockIdx.x;
+  const int t = blockIdx.y * 64 + (threadIdx.x & 63);
+  const int wave = threadIdx.x >> 6;
+  const bool live = t < NC * W;
+  const int c = live ? t / W : 0, cc = live ? t % W : 0;
+  const int b0 = seg_blk[seg], b1 = seg_blk[seg + 1];
+  double s = 0.0, e = 0.0;
+  if (live) {
+    for (int b = b0 + wave; b < b1; b += 4) {
+      const double* src = part + ((int64_t(b) * NC + c) * W + cc) * 2;
+      acc2(s, e, src[0]);
+      e += src[1];
     }
   }
-  __threadfence();
+  __shared__ double rs[4][64], rc[4][64];
+  rs[wave][threadIdx.x & 63] = s;
+  rc[wave][threadIdx.x & 63] = e;
   __syncthreads();
-
-  // ---- segment level: the last group of the segment writes the moments ----
-  if (tid == 0) {
-    const unsigned t = ticket(p.seg_cnt + seg);
-    const int ngr = p.seg_grp[seg + 1] - p.seg_grp[seg];
-    s_flag = (t == unsigned(ngr - 1));
-    if (s_flag) store_rel(p.seg_cnt + seg, 0u);
-  }
-  __syncthreads();
-  if (!s_flag) return;
-  __threadfence();
-  const int slot = only_dots ? 0 : n - p.out_base;
-  const bool ok = slot >= 0 && slot < p.out_cap;
-  {
-    const int g0 = p.seg_grp[seg], g1 = p.seg_grp[seg + 1];
-    for (int t = tid; t < NC * W; t += WAVES * 64) {
-      const int c = t / W, cc = t % W;
-      double s = 0.0, e = 0.0;
-      for (int g = g0; g < g1; ++g) {
-        const double* src = p.gpart + ((int64_t(g) * NC + c) * W + cc) * 2;
-        acc2(s, e, load_coh(src));
-        e += load_coh(src + 1);
-      }
-      if (ok) {
-        double* rec = p.dots + ((int64_t(slot) * p.nseg + seg) * W + cc) * 4;
-        rec[out_pos<MODE>(c)] = s + e;
-        if (MODE == KPM_DOTS_DOUBLING && c == 0) rec[1] = 0.0;
-      }
+  if (wave == 0 && live) {
+    double S = 0.0, E = 0.0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      acc2(S, E, rs[w][threadIdx.x]);
+      E += rc[w][threadIdx.x];
     }
-  }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) {
-    if (!ok) __hip_atomic_fetch_or(p.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned t = ticket(p.done_cnt);
-    if (t == unsigned(p.nseg - 1)) {
-      store_rel(p.done_cnt, 0u);
-      if (!only_dots)
-        __hip_atomic_store(p.ctrl, n + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    double* rec = out + (int64_t(seg) * W + cc) * 4;
+    rec[out_pos<MODE>(c)] = S + E;
+    if (MODE == KPM_DOTS_DOUBLING && c == 0) rec[1] = 0.0;
   }
 }
 
@@ -465,27 +552,41 @@ namespace {
 
 constexpr int kWaves = 4;
 
+template <int MODE>
+int launch_reduce(const StepParams& p, int width, hipStream_t stream) {
+  constexpr int NC = NComp<MODE>::value;
+  dim3 grid(p.nseg, (NC * width + 63) / 64);
+  hipLaunchKernelGGL((cheb_reduce<MODE>), grid, dim3(256), 0, stream, p.part, p.seg_blk, width,
+                     p.dots);
+  return check_hip(hipGetLastError(), "cheb_reduce launch");
+}
+
 template <typename T, int VEC, int LPR>
-int launch_one(int mode, bool only_dots, int iters, const StepParams& p,
+int launch_one(int mode, bool only_dots, int iters, int n, const StepParams& p,
                hipStream_t stream) {
   dim3 grid(p.nblocks_launch), block(kWaves * 64);
+  constexpr int W = VEC * LPR;
   switch (mode) {
     case KPM_DOTS_NONE:
       hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_NONE>), grid, block, 0,
-                         stream, p, iters, int(only_dots));
-      break;
-    case KPM_DOTS_DOUBLING:
+                         stream, p, p.indptr, p.indices, p.values, n, iters, int(only_dots));
+      return check_hip(hipGetLastError(), "cheb_kernel launch");
+    case KPM_DOTS_DOUBLING: {
       hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_DOUBLING>), grid, block,
-                         0, stream, p, iters, int(only_dots));
-      break;
-    case KPM_DOTS_PROJECT:
+                         0, stream, p, p.indptr, p.indices, p.values, n, iters,
+                         int(only_dots));
+      const int rc = check_hip(hipGetLastError(), "cheb_kernel launch");
+      return rc ? rc : launch_reduce<KPM_DOTS_DOUBLING>(p, W, stream);
+    }
+    case KPM_DOTS_PROJECT: {
       hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_PROJECT>), grid, block,
-                         0, stream, p, iters, int(only_dots));
-      break;
-    default:
-      return fail(KPM_ERR_ARG, "unknown dots mode");
+                         0, stream, p, p.indptr, p.indices, p.values, n, iters,
+                         int(only_dots));
+      const int rc = check_hip(hipGetLastError(), "cheb_kernel launch");
+      return rc ? rc : launch_reduce<KPM_DOTS_PROJECT>(p, W, stream);
+    }
   }
-  return check_hip(hipGetLastError(), "cheb_kernel launch");
+  return fail(KPM_ERR_ARG, "unknown dots mode");
 }
 
 template <typename T, int VEC, int LPR>
@@ -533,33 +634,44 @@ int dispatch_shape(int vec, int lpr, F&& f) {
 int variants_for(int width, int value_type, KernelShape* out, int max_out) {
   if (width < 1 || width > 128 || (width & (width - 1)) != 0) return 0;
   int n = 0;
-  auto add = [&](int lpr, int vec, int iters) {
-    if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, iters};
-    ++n;
+  // A variant is a lane layout plus how many residency rounds the grid is
+  // sized for (1: every workgroup resident at once, one contiguous row range
+  // each; 2: twice as many, half as long).  Rows per workgroup are set by the
+  // plan from the kernel's occupancy and the device's CU count.
+  auto add = [&](int lpr, int vec) {
+    for (int rounds = 1; rounds <= 2; ++rounds) {
+      if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, rounds, 0};
+      ++n;
+    }
   };
-  // Rows per workgroup = iters * waves * 64 / lpr; two sizes per shape.
   if (value_type == KPM_C64) {
-    if (width >= 2) {
-      const int lpr = width / 2;
-      const int rpw = 64 / lpr;
-      add(lpr, 2, 1024 / (kWaves * rpw) > 0 ? 1024 / (kWaves * rpw) : 1);
-      add(lpr, 2, 256 / (kWaves * rpw) > 0 ? 256 / (kWaves * rpw) : 1);
-    }
-    if (width <= 64) {
-      const int lpr = width;
-      const int rpw = 64 / lpr;
-      add(lpr, 1, 1024 / (kWaves * rpw) > 0 ? 1024 / (kWaves * rpw) : 1);
-      add(lpr, 1, 256 / (kWaves * rpw) > 0 ? 256 / (kWaves * rpw) : 1);
-    }
+    if (width >= 2) add(width / 2, 2);
+    if (width <= 64) add(width, 1);
   } else if (value_type == KPM_C128) {
-    if (width <= 64) {
-      const int lpr = width;
-      const int rpw = 64 / lpr;
-      add(lpr, 1, 1024 / (kWaves * rpw) > 0 ? 1024 / (kWaves * rpw) : 1);
-      add(lpr, 1, 256 / (kWaves * rpw) > 0 ? 256 / (kWaves * rpw) : 1);
-    }
+    if (width <= 64) add(width, 1);
   }
   return n;
+}
+
+namespace {
+
+struct OccQuery {
+  int* out;
+  template <typename T, int V, int LP>
+  int operator()() {
+    return check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         out, cheb_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING>, kWaves * 64, 0),
+                     "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  }
+};
+
+}  // namespace
+
+int blocks_per_cu(int value_type, const KernelShape& ks, int* out) {
+  OccQuery q{out};
+  if (value_type == KPM_C64) return dispatch_shape<float>(ks.vec, ks.lpr, q);
+  if (value_type == KPM_C128) return dispatch_shape<double>(ks.vec, ks.lpr, q);
+  return fail(KPM_ERR_ARG, "unknown value type");
 }
 
 namespace {
@@ -568,11 +680,12 @@ struct ChebLauncher {
   int mode;
   bool only_dots;
   int iters;
+  int n;
   const StepParams& p;
   hipStream_t s;
   template <typename T, int V, int LP>
   int operator()() {
-    return launch_one<T, V, LP>(mode, only_dots, iters, p, s);
+    return launch_one<T, V, LP>(mode, only_dots, iters, n, p, s);
   }
 };
 
@@ -592,9 +705,9 @@ struct SpmmLauncher {
 
 }  // namespace
 
-int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots,
+int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots, int n,
                 const StepParams& p, hipStream_t stream) {
-  ChebLauncher l{mode, only_dots, ks.iters, p, stream};
+  ChebLauncher l{mode, only_dots, ks.iters, n, p, stream};
   if (value_type == KPM_C64) return dispatch_shape<float>(ks.vec, ks.lpr, l);
   if (value_type == KPM_C128) return dispatch_shape<double>(ks.vec, ks.lpr, l);
   return fail(KPM_ERR_ARG, "unknown value type");

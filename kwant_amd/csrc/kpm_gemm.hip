@@ -204,10 +204,201 @@ __global__ void add_slices(const double* work, int slices, double* mu, int64_t l
   mu[2 * (int64_t(m) * ldc + n) + 1] += im;
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core kernel.  The complex product is two real GEMMs over the
+// interleaved k axis k' = 2k + {0: re, 1: im}:
+//   Re mu = sum_k' A'[m, k'] B'[n, k']     with B' = (Br, Bi)
+//   Im mu = sum_k' A'[m, k'] B''[n, k']    with B'' = (Bi, -Br)
+// (conj(A) B = (Ar Br + Ai Bi) + i (Ar Bi - Ai Br)).  B'' is formed while
+// reading B' from LDS.  One v_mfma_*_16x16x4 computes a 16 x 16 tile over 4
+// k'; a workgroup of 4 waves owns a 64 x 64 complex tile, each wave 2 x 2
+// MFMA tiles, for both the real and the imaginary accumulators.
+// Operand layout of the 16x16x4 MFMAs: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result register r of lane l is D[4 (l / 16) + r][l % 16]
+// for FP32 and D[l / 16 + 4 r][l % 16] for FP64.
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4m __attribute__((ext_vector_type(4)));
+
+template <typename F>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = d4v;
+  __device__ static acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = f4m;
+  __device__ static acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int MB = 64;            // complex rows / cols per workgroup tile
+constexpr int MKC = 16;           // complex k per LDS stage
+constexpr int MKR = 2 * MKC;      // real k' per stage
+constexpr int MLD = MKR + 1;      // padded LDS row stride
+
+template <int PREC>
+__global__ __launch_bounds__(256) void corr_gemm_mfma(const void* Av, int64_t lda,
+                                                      const void* Bv, int64_t ldb,
+                                                      double* out, int64_t ldo, int m_rows,
+                                                      int n_rows, int64_t k, int64_t kps,
+                                                      int k_chunk, int accumulate) {
+  using T = typename GemmIn<PREC>::T;
+  constexpr bool SPLIT = PREC == KPM_GEMM_FP32X3;
+  constexpr bool F32 = PREC != KPM_GEMM_FP64;
+  using F = typename std::conditional<F32, float, double>::type;
+  using MF = Mfma<F>;
+  using acc_t = typename MF::acc_t;
+
+  const T* A = reinterpret_cast<const T*>(Av);
+  const T* B = reinterpret_cast<const T*>(Bv);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * MB, n0 = blockIdx.x * MB;
+  const int64_t kb = int64_t(blockIdx.z) * kps;
+  const int64_t ke = kb + kps < k ? kb + kps : k;
+
+  __shared__ F as_[MB * MLD], bs_[MB * MLD];
+  __shared__ F al_[SPLIT ? MB * MLD : 1], bl_[SPLIT ? MB * MLD : 1];
+
+  acc_t cr[2][2], ci[2][2];
+  double dr[2][2][4], di[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cr[i][j][r] = F(0);
+        ci[i][j][r] = F(0);
+        dr[i][j][r] = 0.0;
+        di[i][j][r] = 0.0;
+      }
+    }
+
+  const int lr = tid >> 2;        // staged row
+  const int lq = (tid & 3) * 4;   // first staged complex k of this thread
+  int64_t since_fold = 0;
+  for (int64_t k0 = kb; k0 < ke; k0 += MKC) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t kc = k0 + lq + e;
+      T ar = T(0), ai = T(0), br = T(0), bi = T(0);
+      if (kc < ke) {
+        if (m0 + lr < m_rows) {
+          const T* p = A + 2 * (int64_t(m0 + lr) * lda + kc);
+          ar = p[0];
+          ai = p[1];
+        }
+        if (n0 + lr < n_rows) {
+          const T* p = B + 2 * (int64_t(n0 + lr) * ldb + kc);
+          br = p[0];
+          bi = p[1];
+        }
+      }
+      const int o = lr * MLD + 2 * (lq + e);
+      as_[o] = F(ar);
+      as_[o + 1] = F(ai);
+      bs_[o] = F(br);
+      bs_[o + 1] = F(bi);
+      if constexpr (SPLIT) {
+        al_[o] = F(ar - double(F(ar)));
+        al_[o + 1] = F(ai - double(F(ai)));
+        bl_[o] = F(br - double(F(br)));
+        bl_[o + 1] = F(bi - double(F(bi)));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < MKR / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      const F sgn = (kr & 1) ? F(-1) : F(1);
+      F a[2], b[2], bsw[2];
+      F la[2], lb[2], lbsw[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int ra = (wm * 32 + i * 16 + (lane & 15)) * MLD;
+        const int rb = (wn * 32 + i * 16 + (lane & 15)) * MLD;
+        a[i] = as_[ra + kr];
+        b[i] = bs_[rb + kr];
+        bsw[i] = sgn * bs_[rb + (kr ^ 1)];
+        if constexpr (SPLIT) {
+          la[i] = al_[ra + kr];
+          lb[i] = bl_[rb + kr];
+          lbsw[i] = sgn * bl_[rb + (kr ^ 1)];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cr[i][j] = MF::run(a[i], b[j], cr[i][j]);
+          ci[i][j] = MF::run(a[i], bsw[j], ci[i][j]);
+          if constexpr (SPLIT) {
+            cr[i][j] = MF::run(a[i], lb[j], cr[i][j]);
+            cr[i][j] = MF::run(la[i], b[j], cr[i][j]);
+            ci[i][j] = MF::run(a[i], lbsw[j], ci[i][j]);
+            ci[i][j] = MF::run(la[i], bsw[j], ci[i][j]);
+          }
+        }
+    }
+    __syncthreads();
+    if constexpr (F32) {
+      since_fold += MKC;
+      if (since_fold >= k_chunk || k0 + MKC >= ke) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              dr[i][j][r] += double(cr[i][j][r]);
+              di[i][j][r] += double(ci[i][j][r]);
+              cr[i][j][r] = F(0);
+              ci[i][j][r] = F(0);
+            }
+        since_fold = 0;
+      }
+    }
+  }
+
+  double* o = out + int64_t(blockIdx.z) * (accumulate ? 0 : int64_t(m_rows) * ldo * 2);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // result row of register r: FP32 16x16x4 holds rows 4 (l/16) + r,
+        // FP64 16x16x4 holds rows (l/16) + 4 r (measured on gfx950)
+        const int m = m0 + wm * 32 + i * 16 + (F32 ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r);
+        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (m < m_rows && n < n_rows) {
+          const double vr = F32 ? dr[i][j][r] : double(cr[i][j][r]);
+          const double vi = F32 ? di[i][j][r] : double(ci[i][j][r]);
+          double* p = o + 2 * (int64_t(m) * ldo + n);
+          if (accumulate) {
+            p[0] += vr;
+            p[1] += vi;
+          } else {
+            p[0] = vr;
+            p[1] = vi;
+          }
+        }
+      }
+}
+
 template <int PREC>
 int launch_fma(const void* A, int64_t lda, const void* B, int64_t ldb, double* mu, int64_t ldc,
                int m_rows, int n_rows, int64_t k, int k_chunk, double* work,
-               int64_t work_elems, hipStream_t stream) {
+               int64_t work_elems, hipStream_t stream, bool use_mfma) {
   const int tiles = ((m_rows + TM - 1) / TM) * ((n_rows + TN - 1) / TN);
   int64_t slices = 1;
   const int64_t per_slice = 2 * int64_t(m_rows) * n_rows;
@@ -224,13 +415,15 @@ int launch_fma(const void* A, int64_t lda, const void* B, int64_t ldb, double* m
   slices = (k + kps - 1) / kps;
   if (k_chunk <= 0) k_chunk = 256;
   dim3 grid((n_rows + TN - 1) / TN, (m_rows + TM - 1) / TM, unsigned(slices));
+  static_assert(TM == MB && TN == MB && KT == MKC, "tile shapes of the two kernels differ");
+  auto kern = use_mfma ? corr_gemm_mfma<PREC> : corr_gemm_fma<PREC>;
   if (slices == 1) {
-    hipLaunchKernelGGL(corr_gemm_fma<PREC>, grid, dim3(256), 0, stream, A, lda, B, ldb, mu, ldc,
-                       m_rows, n_rows, k, kps, k_chunk, 1);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, A, lda, B, ldb, mu, ldc, m_rows, n_rows,
+                       k, kps, k_chunk, 1);
     return check_hip(hipGetLastError(), "corr_gemm launch");
   }
-  hipLaunchKernelGGL(corr_gemm_fma<PREC>, grid, dim3(256), 0, stream, A, lda, B, ldb, work,
-                     int64_t(n_rows), m_rows, n_rows, k, kps, k_chunk, 0);
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, A, lda, B, ldb, work, int64_t(n_rows),
+                     m_rows, n_rows, k, kps, k_chunk, 0);
   int rc = check_hip(hipGetLastError(), "corr_gemm launch");
   if (rc) return rc;
   const int64_t total = int64_t(m_rows) * n_rows;
@@ -244,16 +437,17 @@ int launch_fma(const void* A, int64_t lda, const void* B, int64_t ldb, double* m
 int launch_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb, double* mu,
                      int64_t ldc, int m_rows, int n_rows, int64_t k, int precision, int k_chunk,
                      double* work, int64_t work_elems, hipStream_t stream) {
-  switch (precision) {
+  const bool use_mfma = !(precision & KPM_GEMM_VECTOR_ALU);
+  switch (precision & ~KPM_GEMM_VECTOR_ALU) {
     case KPM_GEMM_FP64:
       return launch_fma<KPM_GEMM_FP64>(A, lda, B, ldb, mu, ldc, m_rows, n_rows, k, k_chunk,
-                                       work, work_elems, stream);
+                                       work, work_elems, stream, use_mfma);
     case KPM_GEMM_FP32:
       return launch_fma<KPM_GEMM_FP32>(A, lda, B, ldb, mu, ldc, m_rows, n_rows, k, k_chunk,
-                                       work, work_elems, stream);
+                                       work, work_elems, stream, use_mfma);
     case KPM_GEMM_FP32X3:
       return launch_fma<KPM_GEMM_FP32X3>(A, lda, B, ldb, mu, ldc, m_rows, n_rows, k, k_chunk,
-                                         work, work_elems, stream);
+                                         work, work_elems, stream, use_mfma);
   }
   return fail(KPM_ERR_ARG, "unknown precision");
 }

@@ -14,10 +14,6 @@ namespace kpm {
 // of rows (its own L2 then sees the neighbour rows of its gathers).
 constexpr int kNumXcd = 8;
 
-// Blocks per reduction group: the last block of a group sums the group's
-// partials, the last group of a segment sums the group partials.
-constexpr int kGroupBlocks = 32;
-
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
@@ -26,7 +22,8 @@ struct KernelShape {
   int lpr;    // lanes per matrix row
   int vec;    // complex values per lane
   int waves;  // wavefronts per workgroup
-  int iters;  // row iterations per workgroup (rows per block = iters*waves*64/lpr)
+  int rounds; // residency rounds the grid is sized for
+  int iters;  // row iterations per workgroup (set by the plan)
 };
 
 // Launch-side parameters of the recursion kernel (passed by value).
@@ -38,35 +35,28 @@ struct StepParams {
   int64_t slot_bytes;
   int32_t slots;
   const void* w;
-  double* dots;
-  int32_t out_base;
-  int32_t out_cap;
-  const int2* blk_rows;
-  const int32_t* blk_seg;
-  const int32_t* blk_grp;
-  const int32_t* grp_blk;
-  const int32_t* seg_grp;
-  const double* scale;
-  double* part;
-  double* gpart;
-  unsigned* grp_cnt;
-  unsigned* seg_cnt;
-  unsigned* done_cnt;
-  int32_t* ctrl;
+  double* dots;            // this step's record: nseg x W x 4 doubles
+  const int2* blk_rows;    // logical block -> [row_begin, row_end)
+  const int32_t* blk_seg;  // logical block -> segment
+  const int32_t* seg_blk;  // segment -> first logical block (nseg + 1)
+  const double* scale;     // segment -> (a, b)
+  double* part;            // block partials, (sum, error) pairs
   int32_t nblocks;
   int32_t nblocks_launch;
   int32_t nseg;
   int32_t xcd;
 };
 
-// Launchers implemented in kpm_cheb.hip.  mode: dots mode; only_dots: no
-// matrix product, dots of the ring slot n (kpm_block_dots).
-int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots,
+// Launchers implemented in kpm_cheb.hip.  mode: dots mode; n: the step (the
+// launch computes a_{n+1}); only_dots: no matrix product, dots of ring slot n
+// (kpm_block_dots).  Dots modes add a cheb_reduce launch after the step.
+int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots, int n,
                 const StepParams& p, hipStream_t stream);
 int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x,
                 int64_t x_stride, void* y, int64_t y_stride, int n_slots,
                 hipStream_t stream);
 int variants_for(int width, int value_type, KernelShape* out, int max_out);
+int blocks_per_cu(int value_type, const KernelShape& ks, int* out);
 
 int launch_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb,
                      double* mu, int64_t ldc, int m_rows, int n_rows, int64_t k,

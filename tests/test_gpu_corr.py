@@ -31,13 +31,15 @@ def test_corr_gemm_against_numpy():
         for prec, tol, dt in ((eng.GEMM_FP64, 1e-12, torch.complex128),
                               (eng.GEMM_FP32, 2e-5, torch.complex64),
                               (eng.GEMM_FP32X3, 2e-5, torch.complex128)):
-            mu = torch.zeros((m, n), dtype=torch.complex128, device=dev)
-            ta = torch.from_numpy(a).to(dev).to(dt)
-            tb = torch.from_numpy(b).to(dev).to(dt)
-            eng.corr_gemm(ta, tb, mu, prec)
-            got = mu.cpu().numpy()
-            err = np.max(np.abs(got - ref)) / np.sqrt(k) / 2
-            assert err < tol, (m, n, k, prec, err)
+            for alu in (0, eng.GEMM_VECTOR_ALU):
+                mu = torch.zeros((m, n), dtype=torch.complex128, device=dev)
+                ta = torch.from_numpy(a).to(dev).to(dt)
+                tb = torch.from_numpy(b).to(dev).to(dt)
+                eng.corr_gemm(ta, tb, mu, prec | alu)
+                got = mu.cpu().numpy()
+                # error relative to the natural scale sqrt(k) |a| |b| of an entry
+                err = np.max(np.abs(got - ref)) / np.sqrt(k) / 2
+                assert err < tol, (m, n, k, prec, alu, err)
 
 
 def test_correlator_matches_oracle():
