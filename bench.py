@@ -171,7 +171,8 @@ def main():
     steps_s = 1000.0 / ms_step
     bstep = bytes_per_step(n, nnz, width, 8, 8)
     achieved = bstep / (ms_step * 1e-3) / 1e9
-    lpr, vec, waves, rounds = eng.variant_info(width, eng.KPM_C64, max(args.variant, 0))
+    lpr, vec, waves, rounds, unroll = eng.variant_info(width, eng.KPM_C64,
+                                                       max(args.variant, 0))
     log("%.4f ms/step, %.1f steps/s, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
         % (ms_step, steps_s, achieved, 100 * achieved / HBM_PEAK_GBS))
 
@@ -209,7 +210,7 @@ def main():
                                % (args.cells, args.cells, n, nnz, width),
                    "N": n, "nnz": nnz, "block_width": width,
                    "kernel": {"lanes_per_row": lpr, "complex_per_lane": vec, "waves": waves,
-                              "grid_rounds": rounds}},
+                              "grid_rounds": rounds, "row_slots": unroll}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_step": bstep},

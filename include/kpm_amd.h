@@ -71,13 +71,13 @@ int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch);
 
 /* Number of recursion kernel variants for a (width, value_type) pair, and
  * the launch shape of each: lanes per matrix row, complex values per lane,
- * waves per workgroup, and the residency rounds the grid is sized for (rows
- * per workgroup = rows / (CUs x resident workgroups per CU x rounds)).  Used
- * by the host autotuner. */
+ * waves per workgroup, the residency rounds the grid is sized for (rows per
+ * workgroup = rows / (CUs x resident workgroups per CU x rounds)) and the row
+ * slots each wavefront processes per iteration.  Used by the host autotuner. */
 int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count);
 int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant,
                      int32_t* lanes_per_row, int32_t* vec,
-                     int32_t* waves, int32_t* rounds);
+                     int32_t* waves, int32_t* rounds, int32_t* unroll);
 
 /* Create a recursion plan for an N x width vector block whose rows are split
  * into n_segments independent systems (segment s owns rows

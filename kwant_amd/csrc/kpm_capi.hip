@@ -40,7 +40,8 @@ struct kpm_plan {
   int32_t* blk_seg = nullptr;
   int32_t* seg_blk = nullptr;
   double* scale = nullptr;
-  double* part = nullptr;
+  double* part = nullptr;   // two buffers of part_half doubles (step parity)
+  int64_t part_half = 0;
   bool scale_set = false;
 };
 
@@ -78,6 +79,8 @@ int fill_params(const kpm_plan* plan, StepParams& p) {
   p.seg_blk = plan->seg_blk;
   p.scale = plan->scale;
   p.part = plan->part;
+  p.prev_part = nullptr;
+  p.prev_dots = nullptr;
   p.nblocks = plan->nblocks;
   p.nblocks_launch = plan->nblocks_launch;
   p.nseg = plan->nseg;
@@ -120,14 +123,15 @@ int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count) {
 }
 
 int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant, int32_t* lanes_per_row,
-                     int32_t* vec, int32_t* waves, int32_t* rows_per_block) {
+                     int32_t* vec, int32_t* waves, int32_t* rounds, int32_t* unroll) {
   KernelShape ks[16];
   const int n = variants_for(width, value_type, ks, 16);
   if (variant < 0 || variant >= n) return fail(KPM_ERR_ARG, "variant out of range");
   if (lanes_per_row) *lanes_per_row = ks[variant].lpr;
   if (vec) *vec = ks[variant].vec;
   if (waves) *waves = ks[variant].waves;
-  if (rows_per_block) *rows_per_block = ks[variant].rounds;
+  if (rounds) *rounds = ks[variant].rounds;
+  if (unroll) *unroll = ks[variant].unroll;
   return KPM_OK;
 }
 
@@ -195,7 +199,8 @@ int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n
   if (!rc) rc = dev_alloc(&p->blk_seg, blk_seg.size(), "blk_seg");
   if (!rc) rc = dev_alloc(&p->seg_blk, seg_blk.size(), "seg_blk");
   if (!rc) rc = dev_alloc(&p->scale, size_t(2) * n_segments, "scale");
-  if (!rc) rc = dev_alloc(&p->part, size_t(p->nblocks) * maxnc * width * 2, "partials");
+  p->part_half = int64_t(p->nblocks) * maxnc * width * 2;
+  if (!rc) rc = dev_alloc(&p->part, size_t(2 * p->part_half), "partials");
   if (!rc) rc = upload(p->blk_rows, blk_rows, "upload blk_rows");
   if (!rc) rc = upload(p->blk_seg, blk_seg, "upload blk_seg");
   if (!rc) rc = upload(p->seg_blk, seg_blk, "upload seg_blk");
@@ -285,16 +290,33 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
   p.slots = slots;
   p.w = w;
   fill_params(plan, p);
+  // Partials alternate between two buffers by step parity: step n writes
+  // buffer n % 2 and reduces step n-1's buffer into its record first, so a
+  // step costs one launch; the last step of the call is reduced by a
+  // standalone launch.
   const int64_t rec = int64_t(plan->nseg) * plan->width * 4;
+  const int64_t half = plan->part_half;
   for (int i = 0; i < n_steps; ++i) {
     const int n = plan->step;
-    p.dots = d_dots ? d_dots + int64_t(n - out_base) * rec : nullptr;
+    p.part = plan->part + (n & 1) * half;
+    p.prev_part = nullptr;
+    p.prev_dots = nullptr;
+    if (dots_mode != KPM_DOTS_NONE && i > 0) {
+      p.prev_part = plan->part + ((n - 1) & 1) * half;
+      p.prev_dots = d_dots + int64_t(n - 1 - out_base) * rec;
+    }
     rc = launch_cheb(plan->value_type, plan->shape, dots_mode, false, n, p,
                      (hipStream_t)stream);
     if (rc) return rc;
     plan->step = n + 1;
   }
-  return KPM_OK;
+  if (dots_mode != KPM_DOTS_NONE && n_steps > 0) {
+    const int last = plan->step - 1;
+    p.part = plan->part + (last & 1) * half;
+    p.dots = d_dots + int64_t(last - out_base) * rec;
+    rc = launch_reduce(dots_mode, plan->width, p, (hipStream_t)stream);
+  }
+  return rc;
 }
 
 int kpm_block_dots(kpm_plan* plan, const void* x, const void* w, double* d_out,
@@ -312,8 +334,10 @@ int kpm_block_dots(kpm_plan* plan, const void* x, const void* w, double* d_out,
   p.w = w;
   p.dots = d_out;
   fill_params(plan, p);
-  return launch_cheb(plan->value_type, plan->shape, dots_mode, true, 0, p,
-                     (hipStream_t)stream);
+  int rc = launch_cheb(plan->value_type, plan->shape, dots_mode, true, 0, p,
+                       (hipStream_t)stream);
+  if (!rc) rc = launch_reduce(dots_mode, plan->width, p, (hipStream_t)stream);
+  return rc;
 }
 
 int kpm_spmm(const kpm_csr* op, const void* x, int64_t x_stride, void* y, int64_t y_stride,

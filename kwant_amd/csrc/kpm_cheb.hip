@@ -14,10 +14,12 @@
 // the doubling identity (kwant/kpm.py:470-473) are accumulated in FP64 in
 // the same pass: one lane owns VEC columns of the block, so per-column sums
 // need no shuffles until the end of the workgroup.  Workgroup partials are
-// stored with plain stores and summed in block order by cheb_reduce, a small
-// kernel launched after each step (no cross-workgroup synchronisation inside
-// the streaming kernel; on a multi-XCD part an agent-scope fence per
-// workgroup costs an L2 write-back).  Every addition is a TwoSum and the
+// stored with plain stores (no cross-workgroup synchronisation inside the
+// streaming kernel: on a multi-XCD part an agent-scope fence per workgroup
+// costs an L2 write-back); the next step's launch reduces them, one
+// wavefront per sum in fixed order, before its own rows (the kernel boundary
+// makes them visible), and cheb_reduce does so for the last step of a call.
+// Every addition is a TwoSum and the
 // (sum, error) pair is carried through every level and rounded once at the
 // end, so a column's result is the correctly rounded sum of its terms (up to
 // the error of the error term) whatever the launch shape or block width, and
@@ -155,80 +157,119 @@ __device__ inline int out_pos(int c) {
   return c;
 }
 
-// sum_k H[row, j_k] x[j_k, cols] for the RPW (1 or 2) consecutive rows of a
-// wavefront, row0 wave-uniform.  indptr and every entry are read with scalar
-// loads (uniform addresses); each lane group then selects its own row's
-// entry, so a chunk of 4 entries per row costs one scalar wait and its 4
-// gathers are in flight together.  Surplus slots of a chunk reuse a valid
-// entry with a zero weight.
-template <typename T, int VEC, int RPW>
-__device__ inline CLane<T, VEC> spmm_rows_scalar(const int32_t* __restrict__ indptr,
-                                                 const int32_t* __restrict__ indices,
-                                                 const void* __restrict__ values, const T* cur,
-                                                 int row0, int rows_left, int grp, int col,
-                                                 int W) {
+// sum_k H[row, j_k] x[j_k, cols] for U slots of RPW (1 or 2) consecutive rows
+// of a wavefront (rows row0 .. row0 + U*RPW - 1, row0 wave-uniform; slot u,
+// lane group g owns row row0 + u*RPW + g).  indptr and every entry are read
+// with scalar loads (uniform addresses); each lane group selects its own row's
+// entry, so a chunk of 4 entries per row costs one scalar wait and the
+// gathers of all U slots are in flight together.  Surplus slots of a chunk
+// reuse a valid entry with a zero weight.
+template <typename T, int VEC, int RPW, int U>
+__device__ inline void spmm_rows_scalar(const int32_t* __restrict__ indptr,
+                                        const int32_t* __restrict__ indices,
+                                        const void* __restrict__ values, const T* cur, int row0,
+                                        int rows_left, int grp, int col, int W,
+                                        CLane<T, VEC> (&acc)[U]) {
   using IO = LaneIO<T, VEC>;
-  const int nrows = rows_left < RPW ? rows_left : RPW;
-  int kb[RPW + 1];
+  constexpr int Q = U * RPW;
+  const int nrows = rows_left < Q ? rows_left : Q;
+  int kb[Q + 1];
   kb[0] = indptr[row0];
 #pragma unroll
-  for (int r = 1; r <= RPW; ++r) kb[r] = r <= nrows ? indptr[row0 + r] : kb[r - 1];
-  int len[RPW];
+  for (int q = 1; q <= Q; ++q) kb[q] = q <= nrows ? indptr[row0 + q] : kb[q - 1];
+  int len[Q];
   int nmax = 0;
 #pragma unroll
-  for (int r = 0; r < RPW; ++r) {
-    len[r] = kb[r + 1] - kb[r];
-    nmax = len[r] > nmax ? len[r] : nmax;
+  for (int q = 0; q < Q; ++q) {
+    len[q] = kb[q + 1] - kb[q];
+    nmax = len[q] > nmax ? len[q] : nmax;
   }
-  const int safe = kb[RPW] - 1;  // a valid entry whenever nmax > 0
-  int my_len = len[0];
+  const int safe = kb[Q] - 1;  // a valid entry whenever nmax > 0
+  int my_len[U];
 #pragma unroll
-  for (int r = 1; r < RPW; ++r)
-    if (grp == r) my_len = len[r];
-  CLane<T, VEC> acc;
-  zero(acc);
+  for (int u = 0; u < U; ++u) {
+    my_len[u] = len[u * RPW];
+#pragma unroll
+    for (int r = 1; r < RPW; ++r)
+      if (grp == r) my_len[u] = len[u * RPW + r];
+    zero(acc[u]);
+  }
   for (int c = 0; c < nmax; c += 4) {
-    int j[4];
-    T hr[4], hi[4];
+    int j[U][4];
+    T hr[U][4], hi[U][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      int jr[RPW];
-      T hrr[RPW], hir[RPW];
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int r = 0; r < RPW; ++r) {
-        const int e = c + u < len[r] ? kb[r] + c + u : safe;
-        jr[r] = indices[e];
-        load_value<T>(values, e, hrr[r], hir[r]);
-        if (c + u >= len[r]) hrr[r] = hir[r] = T(0);
-      }
-      j[u] = jr[0];
-      hr[u] = hrr[0];
-      hi[u] = hir[0];
+      for (int v4 = 0; v4 < 4; ++v4) {
+        int jr[RPW];
+        T hrr[RPW], hir[RPW];
 #pragma unroll
-      for (int r = 1; r < RPW; ++r)
-        if (grp == r) {
-          j[u] = jr[r];
-          hr[u] = hrr[r];
-          hi[u] = hir[r];
+        for (int r = 0; r < RPW; ++r) {
+          const int q = u * RPW + r;
+          const int e = c + v4 < len[q] ? kb[q] + c + v4 : safe;
+          jr[r] = indices[e];
+          load_value<T>(values, e, hrr[r], hir[r]);
+          if (c + v4 >= len[q]) hrr[r] = hir[r] = T(0);
         }
-    }
-    CLane<T, VEC> x[4];
+        j[u][v4] = jr[0];
+        hr[u][v4] = hrr[0];
+        hi[u][v4] = hir[0];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      x[u] = IO::load(cur + 2 * (int64_t(j[u]) * W + col));
-      if (c + u >= my_len) zero(x[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        acc.re[v] = fma(hr[u], x[u].re[v], acc.re[v]);
-        acc.re[v] = fma(-hi[u], x[u].im[v], acc.re[v]);
-        acc.im[v] = fma(hr[u], x[u].im[v], acc.im[v]);
-        acc.im[v] = fma(hi[u], x[u].re[v], acc.im[v]);
+        for (int r = 1; r < RPW; ++r)
+          if (grp == r) {
+            j[u][v4] = jr[r];
+            hr[u][v4] = hrr[r];
+            hi[u][v4] = hir[r];
+          }
       }
+    CLane<T, VEC> x[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int v4 = 0; v4 < 4; ++v4) {
+        x[u][v4] = IO::load(cur + 2 * (int64_t(j[u][v4]) * W + col));
+        if (c + v4 >= my_len[u]) zero(x[u][v4]);
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int v4 = 0; v4 < 4; ++v4)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          acc[u].re[v] = fma(hr[u][v4], x[u][v4].re[v], acc[u].re[v]);
+          acc[u].re[v] = fma(-hi[u][v4], x[u][v4].im[v], acc[u].re[v]);
+          acc[u].im[v] = fma(hr[u][v4], x[u][v4].im[v], acc[u].im[v]);
+          acc[u].im[v] = fma(hi[u][v4], x[u][v4].re[v], acc[u].im[v]);
+        }
   }
-  return acc;
+}
+
+// FP64 dot contributions of one row (lane columns) to the accumulators.
+template <typename T, int VEC, int MODE, int NC>
+__device__ inline void add_dots(double (&as)[NC][VEC], double (&ac)[NC][VEC],
+                                const CLane<T, VEC>& xs, const CLane<T, VEC>& y,
+                                const CLane<T, VEC>& ww) {
+  if (MODE == KPM_DOTS_DOUBLING) {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      const double xr = xs.re[v], xi = xs.im[v];
+      const double yr = y.re[v], yi = y.im[v];
+      acc2(as[0][v], ac[0][v], fma(xr, xr, xi * xi));
+      acc2(as[1][v], ac[1][v], fma(yr, xr, yi * xi));
+      acc2(as[2][v], ac[2][v], fma(yr, xi, -(yi * xr)));
+    }
+  } else if (MODE == KPM_DOTS_PROJECT) {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      const double wr = ww.re[v], wi = ww.im[v];
+      const double xr = xs.re[v], xi = xs.im[v];
+      const double yr = y.re[v], yi = y.im[v];
+      acc2(as[0][v], ac[0][v], fma(wr, xr, wi * xi));
+      acc2(as[1][v], ac[1][v], fma(wr, xi, -(wi * xr)));
+      acc2(as[2][v], ac[2][v], fma(wr, yr, wi * yi));
+      acc2(as[3][v], ac[3][v], fma(wr, yi, -(wi * yr)));
+    }
+  }
 }
 
 // The same product for one row per lane group with per-lane (vector) loads of
@@ -300,11 +341,16 @@ __device__ inline CLane<T, VEC> spmm_row_vector(const int32_t* __restrict__ indp
   return acc;
 }
 
+template <int NC, int MODE>
+__device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
+                                   const int32_t* __restrict__ seg_blk, int nseg, int W,
+                                   double* __restrict__ rec_out, int first, int stride);
+
 // The CSR arrays are separate __restrict__ arguments: nothing the kernel
 // stores can alias them, so for one or two rows per wavefront (LPR >= 32) the
 // compiler reads them with scalar loads, which complete on their own counter
 // instead of queueing behind the vector gathers.
-template <typename T, int VEC, int LPR, int WAVES, int MODE>
+template <typename T, int VEC, int LPR, int WAVES, int MODE, int U>
 __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
                                                           const int32_t* __restrict__ indptr,
                                                           const int32_t* __restrict__ indices,
@@ -326,6 +372,11 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
   const int grp = lane / LPR;
   const int col = (lane % LPR) * VEC;
 
+  if (MODE != KPM_DOTS_NONE && p.prev_part != nullptr) {
+    // the previous step's dots: its partials are complete (kernel boundary)
+    reduce_sums<NC, MODE>(p.prev_part, p.nblocks, p.seg_blk, p.nseg, W, p.prev_dots,
+                          lb * WAVES + wave, p.nblocks * WAVES);
+  }
   const int seg = p.blk_seg[lb];
   const int2 rr = p.blk_rows[lb];
   const double sa = p.scale[2 * seg];
@@ -348,67 +399,92 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
       ac[c][v] = 0.0;
     }
 
-  for (int it = 0; it < iters; ++it) {
-    // rows row0 .. row0 + RPW - 1 belong to this wave; lane group grp owns one
-    const int row0 = __builtin_amdgcn_readfirstlane(rr.x + (it * WAVES + wave) * RPW);
-    if (row0 >= rr.y) break;
-    const int row = row0 + grp;
-    const bool live = row < rr.y;
-    const int64_t roff = 2 * (int64_t(live ? row : row0) * W + col);
-    CLane<T, VEC> y;
-    zero(y);
-    CLane<T, VEC> xs;
-    if (!only_dots) {
-      CLane<T, VEC> acc, xp;
-      if constexpr (RPW <= 2) {
-        // matrix entries come through the scalar unit: stream the rows first
-        xs = IO::load(cur + roff);
-        if (n > 0)
-          xp = IO::load_nt(prv + roff);
+  if constexpr (RPW <= 2) {
+    // Each wave owns a contiguous range of iters * RPW rows and walks it U
+    // row slots at a time: all streaming loads and gathers of the U slots are
+    // issued before the first is consumed.
+    const int wbeg = rr.x + wave * iters * RPW;
+    const int wend = wbeg + iters * RPW < rr.y ? wbeg + iters * RPW : rr.y;
+    for (int r0 = __builtin_amdgcn_readfirstlane(wbeg); r0 < wend; r0 += U * RPW) {
+      CLane<T, VEC> xs[U], xp[U], y[U], ww[U];
+      int64_t roff[U];
+      bool live[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int row = r0 + u * RPW + grp;
+        live[u] = row < wend;
+        roff[u] = 2 * (int64_t(live[u] ? row : r0) * W + col);
+        // a separate, early load of the row's own a_n (it also brings the row
+        // into L2 ahead of the gathers of its neighbours)
+        xs[u] = IO::load(cur + roff[u]);
+        if (!only_dots && n > 0)
+          xp[u] = IO::load_nt(prv + roff[u]);
         else
-          zero(xp);
-        acc = spmm_rows_scalar<T, VEC, RPW>(indptr, indices, values, cur, row0,
-                                             rr.y - row0, grp, col, W);
-      } else {
+          zero(xp[u]);
+        if (MODE == KPM_DOTS_PROJECT)
+          ww[u] = IO::load(wv + roff[u]);
+        else
+          zero(ww[u]);
+        zero(y[u]);
+      }
+      if (!only_dots) {
+        CLane<T, VEC> acc[U];
+        spmm_rows_scalar<T, VEC, RPW, U>(indptr, indices, values, cur, r0, wend - r0, grp, col,
+                                         W, acc);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            y[u].re[v] = fma(acc[u].re[v] - bsh * xs[u].re[v], mul, -xp[u].re[v]);
+            y[u].im[v] = fma(acc[u].im[v] - bsh * xs[u].im[v], mul, -xp[u].im[v]);
+          }
+          if (live[u]) IO::store_nt(nxt + roff[u], y[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!live[u]) {
+          zero(xs[u]);
+          zero(y[u]);
+        }
+        add_dots<T, VEC, MODE, NC>(as, ac, xs[u], y[u], ww[u]);
+      }
+    }
+  } else {
+    for (int it = 0; it < iters; ++it) {
+      // rows row0 .. row0 + RPW - 1 belong to this wave; lane group grp owns one
+      const int row0 = __builtin_amdgcn_readfirstlane(rr.x + (it * WAVES + wave) * RPW);
+      if (row0 >= rr.y) break;
+      const int row = row0 + grp;
+      const bool live = row < rr.y;
+      const int64_t roff = 2 * (int64_t(live ? row : row0) * W + col);
+      CLane<T, VEC> y, xs, ww;
+      zero(y);
+      if (!only_dots) {
         // per-lane matrix entries are vector loads, which complete in order:
         // request them before the streaming row loads
-        acc = spmm_row_vector<T, VEC>(indptr, indices, values, cur, live ? row : row0, live,
-                                      col, W, xs, xp, prv, roff, n > 0);
-      }
+        CLane<T, VEC> xp;
+        CLane<T, VEC> acc = spmm_row_vector<T, VEC>(indptr, indices, values, cur,
+                                                    live ? row : row0, live, col, W, xs, xp,
+                                                    prv, roff, n > 0);
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        y.re[v] = fma(acc.re[v] - bsh * xs.re[v], mul, -xp.re[v]);
-        y.im[v] = fma(acc.im[v] - bsh * xs.im[v], mul, -xp.im[v]);
+        for (int v = 0; v < VEC; ++v) {
+          y.re[v] = fma(acc.re[v] - bsh * xs.re[v], mul, -xp.re[v]);
+          y.im[v] = fma(acc.im[v] - bsh * xs.im[v], mul, -xp.im[v]);
+        }
+        if (live) IO::store_nt(nxt + roff, y);
+      } else {
+        xs = IO::load(cur + roff);
       }
-      if (live) IO::store_nt(nxt + roff, y);
-    } else {
-      xs = IO::load(cur + roff);
-    }
-    if (!live) {
-      zero(xs);
-      zero(y);
-    }
-    if (MODE == KPM_DOTS_DOUBLING) {
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        const double xr = xs.re[v], xi = xs.im[v];
-        const double yr = y.re[v], yi = y.im[v];
-        acc2(as[0][v], ac[0][v], fma(xr, xr, xi * xi));
-        acc2(as[1][v], ac[1][v], fma(yr, xr, yi * xi));
-        acc2(as[2][v], ac[2][v], fma(yr, xi, -(yi * xr)));
+      if (MODE == KPM_DOTS_PROJECT)
+        ww = IO::load(wv + roff);
+      else
+        zero(ww);
+      if (!live) {
+        zero(xs);
+        zero(y);
       }
-    } else if (MODE == KPM_DOTS_PROJECT) {
-      CLane<T, VEC> ww = IO::load(wv + roff);
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        const double wr = ww.re[v], wi = ww.im[v];
-        const double xr = xs.re[v], xi = xs.im[v];
-        const double yr = y.re[v], yi = y.im[v];
-        acc2(as[0][v], ac[0][v], fma(wr, xr, wi * xi));
-        acc2(as[1][v], ac[1][v], fma(wr, xi, -(wi * xr)));
-        acc2(as[2][v], ac[2][v], fma(wr, yr, wi * yi));
-        acc2(as[3][v], ac[3][v], fma(wr, yi, -(wi * yr)));
-      }
+      add_dots<T, VEC, MODE, NC>(as, ac, xs, y, ww);
     }
   }
 
@@ -439,8 +515,9 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
       }
   }
   __syncthreads();
-  // Workgroup partials are (sum, error) pairs:
-  // part[((blk * NC + c) * W + col) * 2 + {0, 1}]; cheb_reduce sums them.
+  // Workgroup partials are (sum, error) pairs, stored column-major over
+  // workgroups -- part[((c * W + col) * nblocks + blk) * 2 + {0, 1}] -- so that
+  // the reduction of one sum reads contiguous memory.
   for (int t = tid; t < NC * W; t += WAVES * 64) {
     const int c = t / W, cc = t % W;
     double s = 0.0, e = 0.0;
@@ -449,51 +526,58 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
       acc2(s, e, red_s[wv2][c][cc]);
       e += red_c[wv2][c][cc];
     }
-    double* dst = p.part + ((int64_t(lb) * NC + c) * W + cc) * 2;
+    double* dst = p.part + ((int64_t(c) * W + cc) * p.nblocks + lb) * 2;
     dst[0] = s;
     dst[1] = e;
   }
 }
 
-// Sum the workgroup partials of every segment in block order (one launch after
-// the step; the kernel boundary makes all partials visible) and write the
-// per-step dots record.  Grid: (segments, ceil(NC * W / 64)); the 4 waves of a
-// workgroup take every 4th block of the segment, then combine in LDS in wave
-// order, so the summation order is fixed.
-template <int MODE>
-__global__ __launch_bounds__(256) void cheb_reduce(const double* __restrict__ part,
-                                                   const int32_t* __restrict__ seg_blk,
-                                                   int W, double* __restrict__ out) {
-  constexpr int NC = NComp<MODE>::value;
-  const int seg = blockIdx.x;
-  const int t = blockIdx.y * 64 + (threadIdx.x & 63);
-  const int wave = threadIdx.x >> 6;
-  const bool live = t < NC * W;
-  const int c = live ? t / W : 0, cc = live ? t % W : 0;
-  const int b0 = seg_blk[seg], b1 = seg_blk[seg + 1];
-  double s = 0.0, e = 0.0;
-  if (live) {
-    for (int b = b0 + wave; b < b1; b += 4) {
-      const double* src = part + ((int64_t(b) * NC + c) * W + cc) * 2;
-      acc2(s, e, src[0]);
-      e += src[1];
+// One wavefront sums the workgroup partials of one (segment, accumulator,
+// column) and writes it to the dots record: lane l takes the segment's
+// blocks l, l + 64, ... in order, then a fixed butterfly combines the lanes.
+// Sums q = first, first + stride, ... of the nseg * NC * W are done.
+template <int NC, int MODE>
+__device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
+                                   const int32_t* __restrict__ seg_blk, int nseg, int W,
+                                   double* __restrict__ rec_out, int first, int stride) {
+  const int lane = threadIdx.x & 63;
+  const int total = nseg * NC * W;
+  for (int q = first; q < total; q += stride) {
+    const int seg = q / (NC * W);
+    const int rem = q - seg * NC * W;
+    const int c = rem / W, cc = rem - (rem / W) * W;
+    const int b0 = seg_blk[seg], b1 = seg_blk[seg + 1];
+    const double* src = part + (int64_t(c) * W + cc) * nblocks * 2;
+    double s = 0.0, e = 0.0;
+#pragma unroll 4
+    for (int b = b0 + lane; b < b1; b += 64) {
+      acc2(s, e, src[2 * b]);
+      e += src[2 * b + 1];
     }
-  }
-  __shared__ double rs[4][64], rc[4][64];
-  rs[wave][threadIdx.x & 63] = s;
-  rc[wave][threadIdx.x & 63] = e;
-  __syncthreads();
-  if (wave == 0 && live) {
-    double S = 0.0, E = 0.0;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      acc2(S, E, rs[w][threadIdx.x]);
-      E += rc[w][threadIdx.x];
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double os = __shfl_xor(s, off);
+      const double oe = __shfl_xor(e, off);
+      acc2(s, e, os);
+      e += oe;
     }
-    double* rec = out + (int64_t(seg) * W + cc) * 4;
-    rec[out_pos<MODE>(c)] = S + E;
-    if (MODE == KPM_DOTS_DOUBLING && c == 0) rec[1] = 0.0;
+    if (lane == 0) {
+      double* rec = rec_out + (int64_t(seg) * W + cc) * 4;
+      rec[out_pos<MODE>(c)] = s + e;
+      if (MODE == KPM_DOTS_DOUBLING && c == 0) rec[1] = 0.0;
+    }
   }
+}
+
+// Standalone reduction (after the last step of a call and for kpm_block_dots):
+// one wavefront per sum.
+template <int MODE>
+__global__ __launch_bounds__(256) void cheb_reduce(const double* __restrict__ part, int nblocks,
+                                                   const int32_t* __restrict__ seg_blk,
+                                                   int nseg, int W, double* __restrict__ out) {
+  constexpr int NC = NComp<MODE>::value;
+  reduce_sums<NC, MODE>(part, nblocks, seg_blk, nseg, W, out,
+                        blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 
 // y[slot] = op x[slot] for an n_rows x W block (no recursion, no dots).
@@ -550,43 +634,53 @@ __global__ __launch_bounds__(WAVES * 64) void spmm_kernel(const int32_t* indptr,
 
 namespace {
 
-constexpr int kWaves = 4;
+#ifndef KPM_WAVES
+#define KPM_WAVES 4
+#endif
+constexpr int kWaves = KPM_WAVES;
 
 template <int MODE>
-int launch_reduce(const StepParams& p, int width, hipStream_t stream) {
+int launch_reduce_mode(const StepParams& p, int width, hipStream_t stream) {
   constexpr int NC = NComp<MODE>::value;
-  dim3 grid(p.nseg, (NC * width + 63) / 64);
-  hipLaunchKernelGGL((cheb_reduce<MODE>), grid, dim3(256), 0, stream, p.part, p.seg_blk, width,
-                     p.dots);
+  const int sums = p.nseg * NC * width;
+  hipLaunchKernelGGL((cheb_reduce<MODE>), dim3((sums + 3) / 4), dim3(256), 0, stream, p.part,
+                     p.nblocks, p.seg_blk, p.nseg, width, p.dots);
   return check_hip(hipGetLastError(), "cheb_reduce launch");
 }
 
-template <typename T, int VEC, int LPR>
-int launch_one(int mode, bool only_dots, int iters, int n, const StepParams& p,
-               hipStream_t stream) {
+template <typename T, int VEC, int LPR, int U>
+int launch_u(int mode, bool only_dots, int iters, int n, const StepParams& p,
+             hipStream_t stream) {
   dim3 grid(p.nblocks_launch), block(kWaves * 64);
-  constexpr int W = VEC * LPR;
   switch (mode) {
     case KPM_DOTS_NONE:
-      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_NONE>), grid, block, 0,
+      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_NONE, U>), grid, block, 0,
                          stream, p, p.indptr, p.indices, p.values, n, iters, int(only_dots));
-      return check_hip(hipGetLastError(), "cheb_kernel launch");
-    case KPM_DOTS_DOUBLING: {
-      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_DOUBLING>), grid, block,
+      break;
+    case KPM_DOTS_DOUBLING:
+      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_DOUBLING, U>), grid, block,
                          0, stream, p, p.indptr, p.indices, p.values, n, iters,
                          int(only_dots));
-      const int rc = check_hip(hipGetLastError(), "cheb_kernel launch");
-      return rc ? rc : launch_reduce<KPM_DOTS_DOUBLING>(p, W, stream);
-    }
-    case KPM_DOTS_PROJECT: {
-      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_PROJECT>), grid, block,
+      break;
+    case KPM_DOTS_PROJECT:
+      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_PROJECT, U>), grid, block,
                          0, stream, p, p.indptr, p.indices, p.values, n, iters,
                          int(only_dots));
-      const int rc = check_hip(hipGetLastError(), "cheb_kernel launch");
-      return rc ? rc : launch_reduce<KPM_DOTS_PROJECT>(p, W, stream);
-    }
+      break;
+    default:
+      return fail(KPM_ERR_ARG, "unknown dots mode");
   }
-  return fail(KPM_ERR_ARG, "unknown dots mode");
+  return check_hip(hipGetLastError(), "cheb_kernel launch");
+}
+
+// Row-slot unrolling applies to the scalar-metadata shapes (LPR >= 32).
+template <typename T, int VEC, int LPR>
+int launch_one(int mode, bool only_dots, int iters, int n, int unroll, const StepParams& p,
+               hipStream_t stream) {
+  if constexpr (LPR >= 32) {
+    if (unroll == 2) return launch_u<T, VEC, LPR, 2>(mode, only_dots, iters, n, p, stream);
+  }
+  return launch_u<T, VEC, LPR, 1>(mode, only_dots, iters, n, p, stream);
 }
 
 template <typename T, int VEC, int LPR>
@@ -639,10 +733,13 @@ int variants_for(int width, int value_type, KernelShape* out, int max_out) {
   // each; 2: twice as many, half as long).  Rows per workgroup are set by the
   // plan from the kernel's occupancy and the device's CU count.
   auto add = [&](int lpr, int vec) {
-    for (int rounds = 1; rounds <= 2; ++rounds) {
-      if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, rounds, 0};
-      ++n;
-    }
+    // order: the measured-best shape on the 2M-site graphene block first
+    const int max_unroll = lpr >= 32 ? 2 : 1;
+    for (int unroll = 1; unroll <= max_unroll; ++unroll)
+      for (int rounds = 2; rounds >= 1; --rounds) {
+        if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, rounds, unroll, 0};
+        ++n;
+      }
   };
   if (value_type == KPM_C64) {
     if (width >= 2) add(width / 2, 2);
@@ -657,10 +754,19 @@ namespace {
 
 struct OccQuery {
   int* out;
+  int unroll;
   template <typename T, int V, int LP>
   int operator()() {
+    if constexpr (LP >= 32) {
+      if (unroll == 2)
+        return check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             out, cheb_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING, 2>,
+                             kWaves * 64, 0),
+                         "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    }
     return check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         out, cheb_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING>, kWaves * 64, 0),
+                         out, cheb_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING, 1>, kWaves * 64,
+                         0),
                      "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   }
 };
@@ -668,7 +774,7 @@ struct OccQuery {
 }  // namespace
 
 int blocks_per_cu(int value_type, const KernelShape& ks, int* out) {
-  OccQuery q{out};
+  OccQuery q{out, ks.unroll};
   if (value_type == KPM_C64) return dispatch_shape<float>(ks.vec, ks.lpr, q);
   if (value_type == KPM_C128) return dispatch_shape<double>(ks.vec, ks.lpr, q);
   return fail(KPM_ERR_ARG, "unknown value type");
@@ -681,11 +787,12 @@ struct ChebLauncher {
   bool only_dots;
   int iters;
   int n;
+  int unroll;
   const StepParams& p;
   hipStream_t s;
   template <typename T, int V, int LP>
   int operator()() {
-    return launch_one<T, V, LP>(mode, only_dots, iters, n, p, s);
+    return launch_one<T, V, LP>(mode, only_dots, iters, n, unroll, p, s);
   }
 };
 
@@ -707,10 +814,16 @@ struct SpmmLauncher {
 
 int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots, int n,
                 const StepParams& p, hipStream_t stream) {
-  ChebLauncher l{mode, only_dots, ks.iters, n, p, stream};
+  ChebLauncher l{mode, only_dots, ks.iters, n, ks.unroll, p, stream};
   if (value_type == KPM_C64) return dispatch_shape<float>(ks.vec, ks.lpr, l);
   if (value_type == KPM_C128) return dispatch_shape<double>(ks.vec, ks.lpr, l);
   return fail(KPM_ERR_ARG, "unknown value type");
+}
+
+int launch_reduce(int mode, int width, const StepParams& p, hipStream_t stream) {
+  if (mode == KPM_DOTS_DOUBLING) return launch_reduce_mode<KPM_DOTS_DOUBLING>(p, width, stream);
+  if (mode == KPM_DOTS_PROJECT) return launch_reduce_mode<KPM_DOTS_PROJECT>(p, width, stream);
+  return KPM_OK;
 }
 
 int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x, int64_t x_stride,

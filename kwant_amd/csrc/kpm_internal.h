@@ -23,6 +23,7 @@ struct KernelShape {
   int vec;    // complex values per lane
   int waves;  // wavefronts per workgroup
   int rounds; // residency rounds the grid is sized for
+  int unroll; // row slots per wave iteration (scalar-metadata shapes)
   int iters;  // row iterations per workgroup (set by the plan)
 };
 
@@ -40,7 +41,9 @@ struct StepParams {
   const int32_t* blk_seg;  // logical block -> segment
   const int32_t* seg_blk;  // segment -> first logical block (nseg + 1)
   const double* scale;     // segment -> (a, b)
-  double* part;            // block partials, (sum, error) pairs
+  double* part;            // this launch's block partials, (sum, error) pairs
+  const double* prev_part; // previous step's partials to reduce first (or null)
+  double* prev_dots;       // ... into this record (nseg x W x 4 doubles)
   int32_t nblocks;
   int32_t nblocks_launch;
   int32_t nseg;
@@ -49,9 +52,12 @@ struct StepParams {
 
 // Launchers implemented in kpm_cheb.hip.  mode: dots mode; n: the step (the
 // launch computes a_{n+1}); only_dots: no matrix product, dots of ring slot n
-// (kpm_block_dots).  Dots modes add a cheb_reduce launch after the step.
+// (kpm_block_dots).  A dots-mode launch writes workgroup partials to p.part
+// and, when p.prev_part is set, first reduces those of the previous step into
+// p.prev_dots; launch_reduce reduces p.part into p.dots on its own.
 int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots, int n,
                 const StepParams& p, hipStream_t stream);
+int launch_reduce(int mode, int width, const StepParams& p, hipStream_t stream);
 int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x,
                 int64_t x_stride, void* y, int64_t y_stride, int n_slots,
                 hipStream_t stream);
