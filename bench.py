@@ -128,6 +128,14 @@ def main():
     log("vectors ready (%.1fs)" % (time.time() - t0))
 
     m64 = eng.DeviceCSR(h, "complex64")
+    if args.variant < 0:
+        from kwant_amd import autotune
+        args.variant = autotune.pick_variant(m64, (a, b), width, eng.DOTS_DOUBLING, force=True)
+        log("autotuned launch shape: variant %d (ms/step per variant: %s)" % (
+            args.variant, ", ".join("%.3f" % t for t in
+                                    autotune._CACHE[autotune.key_of(m64, width,
+                                                                    eng.DOTS_DOUBLING)][
+                                        "ms_per_step"])))
     m_steps = args.steps
     check = {}
     if not args.no_check:
@@ -172,7 +180,7 @@ def main():
     bstep = bytes_per_step(n, nnz, width, 8, 8)
     achieved = bstep / (ms_step * 1e-3) / 1e9
     lpr, vec, waves, rounds, unroll = eng.variant_info(width, eng.KPM_C64,
-                                                       max(args.variant, 0))
+                                                       args.variant)
     log("%.4f ms/step, %.1f steps/s, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
         % (ms_step, steps_s, achieved, 100 * achieved / HBM_PEAK_GBS))
 

@@ -1,0 +1,99 @@
+"""Launch-shape autotuner for the Chebyshev recursion kernel.
+
+The library exposes several launch shapes per (block width, value type):
+lanes per matrix row, complex values per lane, residency rounds and row
+slots per wavefront iteration (``kpm_variant_info``).  Which is fastest
+depends on the matrix (rows, nonzeros per row, neighbour distances) and the
+width, so for large problems the engine times each shape for a few steps on
+the actual matrix and keeps the fastest, per (rows, nnz, width, value type,
+dots mode).  Results are identical bitwise whatever shape runs (the per-row
+arithmetic is the same and the FP64 dot sums are order-independent), so
+tuning never changes the moments.
+
+Results are cached in memory and, when ``KWANT_AMD_TUNE_CACHE`` names a file,
+in that JSON file across processes.
+"""
+import json
+import os
+
+from . import _engine as eng
+
+_CACHE = {}
+MIN_ELEMENTS = 1 << 20  # smaller problems use the default shape
+TIMED_STEPS = 6
+
+
+def _cache_file():
+    return os.environ.get("KWANT_AMD_TUNE_CACHE")
+
+
+def _load():
+    path = _cache_file()
+    if path and os.path.exists(path):
+        try:
+            with open(path) as f:
+                for k, v in json.load(f).items():
+                    _CACHE.setdefault(k, v)
+        except (OSError, ValueError):
+            pass
+
+
+def _store():
+    path = _cache_file()
+    if not path:
+        return
+    try:
+        with open(path, "w") as f:
+            json.dump(_CACHE, f)
+    except OSError:
+        pass
+
+
+def key_of(dmat, width, mode):
+    return "%d:%d:%d:%d:%d" % (dmat.shape[0], dmat.nnz, width, dmat.value_type, mode)
+
+
+def time_variant(dmat, ab, width, mode, variant, block, steps=TIMED_STEPS):
+    """Milliseconds per step of one launch shape on ``block`` (N x width)."""
+    import torch
+    rec = eng.BlockRecursion(dmat, ab, block, width, mode, variant=variant)
+    dots = None
+    if mode != eng.DOTS_NONE:
+        dots = torch.empty((steps + 2, 1, width, 4), dtype=torch.float64, device=block.device)
+    stride = dmat.shape[0] * width
+    rec.plan.steps(dmat, rec.ring, stride, 2, 2, mode, dots=dots, out_base=0)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    rec.plan.steps(dmat, rec.ring, stride, 2, steps, mode, dots=dots, out_base=0)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+def pick_variant(dmat, ab, width, mode=eng.DOTS_DOUBLING, force=False):
+    """The fastest launch shape for this matrix and width (0 below the size
+    threshold unless ``force``)."""
+    import torch
+    n_var = eng.variant_count(width, dmat.value_type)
+    if n_var <= 1:
+        return 0
+    if not force and dmat.shape[0] * width < MIN_ELEMENTS:
+        return 0
+    if not _CACHE:
+        _load()
+    key = key_of(dmat, width, mode)
+    if key in _CACHE:
+        return int(_CACHE[key]["variant"])
+    cdt, _ = eng.complex_dtype(dmat.precision)
+    dev = eng.torch_device()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    phase = torch.rand((dmat.shape[0], width), generator=gen, device=dev, dtype=torch.float64)
+    block = torch.exp(2j * 3.141592653589793 * phase).to(cdt)
+    times = [time_variant(dmat, ab, width, mode, v, block) for v in range(n_var)]
+    best = min(range(n_var), key=lambda v: times[v])
+    _CACHE[key] = {"variant": best, "ms_per_step": times}
+    _store()
+    return best

@@ -36,6 +36,7 @@ from scipy.sparse import coo_matrix, csr_matrix
 from scipy.sparse.linalg import LinearOperator
 
 from . import _engine as eng
+from . import autotune
 from .lanczos import lanczos_bounds
 
 __all__ = ['SpectralDensity', 'Correlator', 'conductivity',
@@ -520,15 +521,21 @@ class SpectralDensity:
                 self._expand_more(blk, self.num_moments, n_moments)
             self._moments_list = [m for blk in self._blocks for m in blk.moments]
 
+    def _launch_shape(self, width, mode):
+        """The requested kernel variant, or the autotuned one (variant=-1)."""
+        if self._variant >= 0:
+            return self._variant
+        return autotune.pick_variant(self._dmat, (self._a, self._b), width, mode)
+
     def _new_block(self, vectors, width):
         ab = (self._a, self._b)
         if self.operator is None:
             return _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_DOUBLING,
-                             variant=self._variant)
+                             variant=self._launch_shape(width, eng.DOTS_DOUBLING))
         if self._op_matrix is not None:
             import torch
             blk = _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_PROJECT,
-                            variant=self._variant)
+                            variant=self._launch_shape(width, eng.DOTS_PROJECT))
             # w = O^H alpha_0, so <w|alpha_n> = vdot(alpha_0, O alpha_n)
             op_h = eng.DeviceCSR(self._op_matrix.conj().T.tocsr(), self._dmat.precision)
             w = torch.empty_like(blk.rec.ring[0])
@@ -537,7 +544,7 @@ class SpectralDensity:
             blk._op_h = op_h
             return blk
         return _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_NONE,
-                         variant=self._variant)
+                         variant=self._launch_shape(width, eng.DOTS_NONE))
 
     def _expand_new(self, blk, n_moments):
         rec = blk.rec
