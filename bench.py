@@ -83,6 +83,21 @@ def cpu_baseline(h, a, b, vec, steps):
         return (time.perf_counter() - t0) / (steps + 1)
 
 
+def calibrate():
+    """Kernels of known HBM traffic: copy 2 GiB (2 GiB read + 2 GiB written)
+    and a sum over 2 GiB (2 GiB read), three times each."""
+    import torch
+    from kwant_amd import _engine as eng
+    dev = eng.torch_device()
+    x = torch.ones(2 ** 28, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+        float(x.sum())
+    torch.cuda.synchronize()
+    log("calibration kernels done: copy 2 GiB, sum 2 GiB (x3 each)")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,7 +109,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run a 2 GiB copy and a 2 GiB read (known traffic) for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate()
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
