@@ -173,6 +173,67 @@ int kpm_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb,
                   int64_t k, int32_t precision, int32_t k_chunk,
                   double* work, int64_t work_elems, kpm_stream_t stream);
 
+/* ---- moment surrogate (no counterpart in kwant/kpm.py) ------------------
+ * A small MLP that maps Hamiltonian parameters to Jackson-damped moments,
+ * trained on moments from kpm_cheb_steps and evaluated in place of the
+ * recursion inside parameter sweeps.  All buffers are FP32 device memory. */
+
+/* Batched strided GEMM, z = 0 .. batch-1:
+ *   C[z][m, n] = sum_k A[z*a_bs + m*a_rs + k*a_cs] * B[z*b_bs + n*b_rs + k*b_cs]
+ *                (+ bias[z*bias_bs + n])
+ * operands rounded to BF16, FP32 accumulation on the matrix cores.  flags =
+ * one epilogue | optional modifiers:
+ *   KPM_MLP_PLAIN     : C = acc + bias
+ *   KPM_MLP_BIAS_GELU : C = Z = acc + bias, C2 = gelu(Z) (exact erf form),
+ *                       C2 with the strides of C
+ *   KPM_MLP_GELU_GRAD : C = acc * gelu'(aux[z*aux_bs + m*aux_rs + n])
+ *   KPM_MLP_BF16X3    : split each operand x = hi + lo (both BF16) and form
+ *                       hi*hi + hi*lo + lo*hi (about 16 significant bits)
+ *   KPM_MLP_ACCUMULATE: C += acc + bias (plain epilogue only)
+ * C element (z, m, n) is at c + z*c_bs + m*c_rs + n.  bias may be NULL.
+ * Asynchronous on stream. */
+#define KPM_MLP_PLAIN 0
+#define KPM_MLP_BIAS_GELU 1
+#define KPM_MLP_GELU_GRAD 2
+#define KPM_MLP_BF16X3 16
+#define KPM_MLP_ACCUMULATE 32
+typedef struct kpm_mlp_gemm_desc {
+  const float* a;
+  int64_t a_rs, a_cs, a_bs;
+  const float* b;
+  int64_t b_rs, b_cs, b_bs;
+  float* c;
+  int64_t c_rs, c_bs;
+  float* c2;
+  const float* bias;
+  int64_t bias_bs;
+  const float* aux;
+  int64_t aux_rs, aux_bs;
+  int32_t m, n, k, batch;
+  int32_t flags;
+} kpm_mlp_gemm_desc;
+int kpm_mlp_gemm(const kpm_mlp_gemm_desc* desc, kpm_stream_t stream);
+/* out[z*out_bs + n] = sum_m A[z*a_bs + m*lda + n] (bias gradients), rows
+ * summed in order. */
+int kpm_mlp_colsum(const float* A, int64_t lda, int64_t a_bs, int32_t M, int32_t N,
+                   int32_t batch, float* out, int64_t out_bs, kpm_stream_t stream);
+/* dy = scale * (y - t) over n elements; partial[b] = sum over workgroup b of
+ * (y - t)^2 in FP64 (n_partial workgroups; the loss is their sum). */
+int kpm_mlp_mse(const float* y, const float* t, float* dy, int64_t n, float scale,
+                double* partial, int32_t n_partial, kpm_stream_t stream);
+/* Ensemble statistics per row p of y[z*z_stride + p*row_stride + n]
+ * (members z < members, n < width):
+ *   mean[p*mean_stride + n] = average over members       (mean may be NULL)
+ *   dev[p] = max over members of (1/width) sum_n |y - mean| (dev may be NULL)
+ * With y = densities at Chebyshev nodes, dev is the largest L1 distance of a
+ * member's density from the ensemble mean (the surrogate's fallback gate). */
+int kpm_mlp_ensemble(const float* y, int64_t z_stride, int64_t row_stride,
+                     int32_t members, int64_t rows, int32_t width, float* mean,
+                     int64_t mean_stride, float* dev, kpm_stream_t stream);
+/* One Adam update of n parameters at step t >= 1 (bias-corrected). */
+int kpm_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
+                 float b1, float b2, float eps, int32_t t, kpm_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,13 +25,19 @@ GEMM_FP64 = 0
 GEMM_FP32 = 1
 GEMM_FP32X3 = 2
 GEMM_VECTOR_ALU = 16
+MLP_PLAIN = 0
+MLP_BIAS_GELU = 1
+MLP_GELU_GRAD = 2
+MLP_BF16X3 = 16
+MLP_ACCUMULATE = 32
 
 # Exported symbols, in the order include/kpm_amd.h declares them.
 EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
     "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
-    "kpm_spmm", "kpm_corr_gemm",
+    "kpm_spmm", "kpm_corr_gemm", "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
+    "kpm_mlp_ensemble", "kpm_mlp_adam",
 )
 
 
@@ -44,6 +50,21 @@ class KpmCsr(ctypes.Structure):
         ("indices", ctypes.c_void_p),
         ("values", ctypes.c_void_p),
         ("value_type", ctypes.c_int32),
+    ]
+
+
+class MlpGemmDesc(ctypes.Structure):
+    _fields_ = [
+        ("a", ctypes.c_void_p), ("a_rs", ctypes.c_int64), ("a_cs", ctypes.c_int64),
+        ("a_bs", ctypes.c_int64),
+        ("b", ctypes.c_void_p), ("b_rs", ctypes.c_int64), ("b_cs", ctypes.c_int64),
+        ("b_bs", ctypes.c_int64),
+        ("c", ctypes.c_void_p), ("c_rs", ctypes.c_int64), ("c_bs", ctypes.c_int64),
+        ("c2", ctypes.c_void_p),
+        ("bias", ctypes.c_void_p), ("bias_bs", ctypes.c_int64),
+        ("aux", ctypes.c_void_p), ("aux_rs", ctypes.c_int64), ("aux_bs", ctypes.c_int64),
+        ("m", ctypes.c_int32), ("n", ctypes.c_int32), ("k", ctypes.c_int32),
+        ("batch", ctypes.c_int32), ("flags", ctypes.c_int32),
     ]
 
 
@@ -62,6 +83,7 @@ def load_library(path=LIB_PATH):
     import torch  # noqa: F401  -- loads torch's HIP runtime first
     lib = ctypes.CDLL(path)
     i32, i64, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    f32 = ctypes.c_float
     pi32 = ctypes.POINTER(ctypes.c_int32)
     sig = {
         "kpm_last_error": (ctypes.c_char_p, []),
@@ -81,6 +103,11 @@ def load_library(path=LIB_PATH):
         "kpm_spmm": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, i64, vp, i64, i32, i32, vp]),
         "kpm_corr_gemm": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
                                          i32, vp, i64, vp]),
+        "kpm_mlp_gemm": (ctypes.c_int, [ctypes.POINTER(MlpGemmDesc), vp]),
+        "kpm_mlp_colsum": (ctypes.c_int, [vp, i64, i64, i32, i32, i32, vp, i64, vp]),
+        "kpm_mlp_mse": (ctypes.c_int, [vp, vp, vp, i64, f32, vp, i32, vp]),
+        "kpm_mlp_ensemble": (ctypes.c_int, [vp, i64, i64, i32, i64, i32, vp, i64, vp, vp]),
+        "kpm_mlp_adam": (ctypes.c_int, [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -354,3 +381,93 @@ def corr_gemm(a, b, mu, precision, k_chunk=256, work=None):
         a.data_ptr(), a.stride(0), b.data_ptr(), b.stride(0), mu.data_ptr(), mu.stride(0),
         m, n, k, precision, k_chunk, work.data_ptr(), work.numel(), stream_handle()),
         "kpm_corr_gemm")
+
+
+def _f32_view(t, name, dims=3):
+    import torch
+    if t.dtype != torch.float32 or t.dim() != dims:
+        raise ValueError("%s must be a %d-d float32 tensor" % (name, dims))
+    return t
+
+
+def mlp_gemm(a, b, c, bias=None, flags=MLP_PLAIN, c2=None, aux=None):
+    """c[z] = a[z] @ b[z].T (+ bias[z]) with the epilogue/modifiers of
+    ``flags`` on the matrix cores (BF16 operands, FP32 accumulation).
+
+    a: (Z, M, K), b: (Z, N, K), c / c2 / aux: (Z, M, N) float32 views with
+    any strides (stride 0 along Z broadcasts one operand to every batch);
+    c, c2 and aux need unit stride along N; bias: (Z, N) with unit stride."""
+    _f32_view(a, "a"), _f32_view(b, "b"), _f32_view(c, "c")
+    z, m, k = a.shape
+    n = b.shape[1]
+    if b.shape[0] != z or b.shape[2] != k or tuple(c.shape) != (z, m, n):
+        raise ValueError("mlp_gemm shape mismatch")
+    if c.stride(2) != 1:
+        raise ValueError("c needs unit stride along N")
+    d = MlpGemmDesc()
+    d.a, d.a_bs, d.a_rs, d.a_cs = a.data_ptr(), *a.stride()
+    d.b, d.b_bs, d.b_rs, d.b_cs = b.data_ptr(), *b.stride()
+    d.c, d.c_bs, d.c_rs = c.data_ptr(), c.stride(0), c.stride(1)
+    if c2 is not None:
+        if c2.shape != c.shape or c2.stride() != c.stride():
+            raise ValueError("c2 must have the shape and strides of c")
+        d.c2 = c2.data_ptr()
+    if bias is not None:
+        if tuple(bias.shape) != (z, n) or bias.stride(1) != 1:
+            raise ValueError("bias must be (Z, N) with unit stride")
+        d.bias, d.bias_bs = bias.data_ptr(), bias.stride(0)
+    if aux is not None:
+        if aux.shape != c.shape or aux.stride(2) != 1:
+            raise ValueError("aux must match c with unit stride along N")
+        d.aux, d.aux_bs, d.aux_rs = aux.data_ptr(), aux.stride(0), aux.stride(1)
+    d.m, d.n, d.k, d.batch, d.flags = m, n, k, z, flags
+    check(load_library().kpm_mlp_gemm(ctypes.byref(d), stream_handle()), "kpm_mlp_gemm")
+
+
+def mlp_colsum(a, out):
+    """out[z, n] = sum_m a[z, m, n] (a: (Z, M, N), unit stride along N)."""
+    _f32_view(a, "a")
+    z, m, n = a.shape
+    if a.stride(2) != 1 or tuple(out.shape) != (z, n) or out.stride(1) != 1:
+        raise ValueError("mlp_colsum layout")
+    check(load_library().kpm_mlp_colsum(a.data_ptr(), a.stride(1), a.stride(0), m, n, z,
+                                        out.data_ptr(), out.stride(0), stream_handle()),
+          "kpm_mlp_colsum")
+
+
+def mlp_mse(y, t, dy, scale, partial):
+    """dy = scale (y - t) elementwise over contiguous tensors; partial (FP64,
+    one per workgroup) receives sums of (y - t)^2."""
+    for x in (y, t, dy):
+        if not x.is_contiguous() or x.numel() != y.numel():
+            raise ValueError("mlp_mse needs contiguous tensors of one size")
+    check(load_library().kpm_mlp_mse(y.data_ptr(), t.data_ptr(), dy.data_ptr(), y.numel(),
+                                     float(scale), partial.data_ptr(), partial.numel(),
+                                     stream_handle()), "kpm_mlp_mse")
+
+
+def mlp_ensemble(y, mean=None, dev=None):
+    """Ensemble mean and largest member L1 deviation per row of y (Z, P, K)
+    (unit stride along K); mean: (P, K) or None, dev: (P,) or None."""
+    _f32_view(y, "y")
+    z, rows, width = y.shape
+    if y.stride(2) != 1:
+        raise ValueError("y needs unit stride along its last axis")
+    if mean is not None and (tuple(mean.shape) != (rows, width) or mean.stride(1) != 1):
+        raise ValueError("mean must be (P, K) with unit stride")
+    if dev is not None and (dev.numel() != rows or not dev.is_contiguous()):
+        raise ValueError("dev must be a contiguous (P,) tensor")
+    check(load_library().kpm_mlp_ensemble(
+        y.data_ptr(), y.stride(0), y.stride(1), z, rows, width,
+        None if mean is None else mean.data_ptr(), 0 if mean is None else mean.stride(0),
+        None if dev is None else dev.data_ptr(), stream_handle()), "kpm_mlp_ensemble")
+
+
+def mlp_adam(p, g, m, v, lr, b1, b2, eps, t):
+    """One bias-corrected Adam update of the flat parameter buffer p."""
+    for x in (p, g, m, v):
+        if not x.is_contiguous() or x.numel() != p.numel():
+            raise ValueError("mlp_adam needs contiguous buffers of one size")
+    check(load_library().kpm_mlp_adam(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(),
+                                      p.numel(), float(lr), float(b1), float(b2), float(eps),
+                                      int(t), stream_handle()), "kpm_mlp_adam")

@@ -56,6 +56,47 @@ def square(lx, ly=None, t=1.0, disorder=0.0, seed=None, shift=0.0, flux=0.0):
     return csr, pos
 
 
+class SquarePattern:
+    """Sparsity pattern of :func:`square` (every diagonal entry kept) with the
+    per-entry data that turns (disorder W, flux phi, shift mu) into values:
+
+        value = exp(2 pi i phi peierls[e])     on hoppings (t = 1)
+        value = W u[row] + mu                  on the diagonal
+
+    with u = RandomState(seed).uniform(-0.5, 0.5, n), the draw of
+    :func:`square`.  ``values(W, phi, mu)`` equals ``square(l, disorder=W,
+    seed=seed, shift=mu, flux=phi)`` entry for entry wherever that matrix
+    keeps the entry (it drops exact zeros)."""
+
+    def __init__(self, l, seed=None):
+        n = l * l
+        x, y = np.divmod(np.arange(n), l)
+        hx = np.nonzero(x < l - 1)[0]
+        hy = np.nonzero(y < l - 1)[0]
+        rows = np.concatenate([hx + l, hx, hy + 1, hy, np.arange(n)])
+        cols = np.concatenate([hx, hx + l, hy, hy + 1, np.arange(n)])
+        m = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+        m.sort_indices()
+        self.l, self.n, self.nnz = l, n, m.nnz
+        self.indptr = m.indptr.astype(np.int32)
+        self.indices = m.indices.astype(np.int32)
+        r = np.repeat(np.arange(n), np.diff(m.indptr))
+        c = self.indices.astype(np.int64)
+        self.is_diag = r == c
+        # Landau gauge of square(): (x+1, y) <- (x, y) carries exp(-2 pi i phi y)
+        self.peierls = np.where(np.abs(x[r] - x[c]) == 1, -(x[r] - x[c]) * y[r], 0).astype(float)
+        self.u = np.random.RandomState(seed).uniform(-0.5, 0.5, n)
+        self.u_entry = np.where(self.is_diag, self.u[r], 0.0)
+
+    def values(self, disorder, flux, shift):
+        hop = np.exp(2j * np.pi * flux * self.peierls)
+        return np.where(self.is_diag, disorder * self.u_entry + shift, hop)
+
+    def matrix(self, disorder, flux, shift):
+        return sp.csr_matrix((self.values(disorder, flux, shift), self.indices, self.indptr),
+                             shape=(self.n, self.n))
+
+
 def honeycomb(l1, l2=None, t=1.0, disorder=0.0, seed=None):
     """Graphene parallelogram of l1 x l2 unit cells with kwant's honeycomb
     geometry (primitive vectors (1, 0), (1/2, sqrt3/2); basis (0, 0),

@@ -1,0 +1,377 @@
+"""Moment surrogate for parameter sweeps: a small MLP ensemble in place of the
+KPM recursion, with plain KPM as the fallback and as the judge.
+
+The family is the square-lattice Hamiltonian H(W, phi, mu) of
+:class:`kwant_amd.lattices.SquarePattern` (L x L sites, hopping 1 with a
+Peierls flux phi per plaquette in Landau gauge, Anderson disorder
+W u_i, u_i ~ U[-1/2, 1/2] with a fixed draw, uniform on-site shift mu), with
+one fixed spectral window (bounds (-7, 7) for every member of the family, so
+that the rescaled moments carry the shift and the disorder).  Its output is
+the Jackson-damped (kwant/kpm.py:1106-1123), normalised DOS moments
+g_n mu_n / mu_0 that kwant's SpectralDensity averages over random vectors
+(kwant/kpm.py:436-477 for the recursion).
+
+* ``SquareFamily`` computes those moments with the GPU recursion for many
+  parameter points at once: ``batch`` systems are the diagonal blocks of one
+  CSR matrix, one segment each, so one launch per Chebyshev step advances all
+  of them (each segment has its own dot products).
+* ``MomentMLP`` is an ensemble of MLPs 3 -> 256 -> 256 -> 256 -> 256 -> 512
+  (GELU) whose forward, backward and Adam update run on the kernels of
+  libkpm_amd (BF16 matrix cores, FP32 accumulation); all members train in the
+  same launches (one batched GEMM per layer).
+* ``MomentSurrogate`` trains the ensemble on moments from ``SquareFamily`` and
+  answers sweeps: where the members' densities differ from their mean by more
+  than ``tol`` in L1 (the spread gate) the point is recomputed with KPM.
+
+DOS L1 distance of two damped moment vectors d = g (mu - mu'):
+    L1 = int |rho - rho'| dx = (1/K) sum_k |d_0 + 2 sum_n d_n cos(n theta_k)|,
+theta_k = pi (k + 1/2) / K: Gauss-Chebyshev quadrature, whose weight
+cancels the density's 1 / (pi sqrt(1 - x^2)); it is exact for int rho itself
+and, with K = 2M nodes, resolves |rho - rho'| at the kernel's resolution.
+With mu_0 = 1 the distance is relative to int rho.
+"""
+import time
+
+import numpy as np
+
+from . import _engine as eng
+from .kpm import jackson_kernel, _scale_from_bounds
+from .lattices import SquarePattern
+
+PARAM_NAMES = ("disorder", "flux", "shift")
+PARAM_LO = np.array([0.0, 0.0, -1.0])
+PARAM_HI = np.array([4.0, 0.02, 1.0])
+LAYERS = (3, 256, 256, 256, 256, 512)
+
+
+def jackson_factors(count):
+    """g_n of the Jackson kernel for ``count`` moments."""
+    return jackson_kernel(np.ones(count))
+
+
+def normalize_params(params, lo=PARAM_LO, hi=PARAM_HI):
+    """Map parameter points (P, 3) from [lo, hi] to [-1, 1]."""
+    params = np.asarray(params, dtype=np.float64)
+    return 2.0 * (params - lo) / (hi - lo) - 1.0
+
+
+def dos_l1_matrix(num_moments, nodes=None):
+    """(K, M) matrix C with C[k, n] = w_n cos(n theta_k) / K, w_0 = 1,
+    w_n = 2: the L1 distance of damped moments d is sum_k |(C d)_k|."""
+    k = 2 * num_moments if nodes is None else nodes
+    theta = np.pi * (np.arange(k) + 0.5) / k
+    c = np.cos(np.outer(theta, np.arange(num_moments)))
+    c[:, 1:] *= 2.0
+    return c / k
+
+
+def dos_l1(d_a, d_b, nodes=None):
+    """Host L1 distance between densities given by damped moments (..., M)."""
+    d = np.asarray(d_a, dtype=np.float64) - np.asarray(d_b, dtype=np.float64)
+    c = dos_l1_matrix(d.shape[-1], nodes)
+    return np.abs(d @ c.T).sum(axis=-1)
+
+
+class SquareFamily:
+    """Damped DOS moments of H(W, phi, mu) from the GPU recursion, ``batch``
+    parameter points per launch.
+
+    The random vectors are those kwant's SpectralDensity(rng=vector_seed)
+    draws: one random_sample(N) for its Lanczos start vector
+    (kwant/kpm.py:203), then one exp(2 pi i random_sample(N)) per vector
+    (kwant/kpm.py:932-951), the same vectors for every parameter point."""
+
+    def __init__(self, l=512, num_moments=512, num_vectors=32, bounds=(-7.0, 7.0), eps=0.05,
+                 disorder_seed=0, vector_seed=1, batch=8, precision="complex64", variant=-1):
+        import torch
+        if num_moments < 2 or num_moments % 2:
+            raise ValueError("num_moments must be even and >= 2")
+        dev = eng.torch_device()
+        self.pattern = pat = SquarePattern(l, seed=disorder_seed)
+        self.l, self.n, self.batch = l, pat.n, batch
+        self.num_moments, self.num_vectors = num_moments, num_vectors
+        self.a, self.b = _scale_from_bounds(bounds[0], bounds[1], eps)
+        self.bounds = bounds
+        n, nnz = pat.n, pat.nnz
+        offs = np.arange(batch, dtype=np.int64)
+        indptr = np.concatenate([(pat.indptr[None, :-1] + nnz * offs[:, None]).ravel(),
+                                 [batch * nnz]])
+        indices = (pat.indices[None, :] + n * offs[:, None]).ravel()
+        if batch * nnz >= 2 ** 31:
+            raise ValueError("batch too large for int32 CSR indices")
+        self.cdt, vt = eng.complex_dtype(precision)
+        self._indptr = torch.from_numpy(indptr.astype(np.int32)).to(dev)
+        self._indices = torch.from_numpy(indices.astype(np.int32)).to(dev)
+        self._values = torch.zeros(batch * nnz, dtype=self.cdt, device=dev)
+        self.mat = eng.DeviceCSR.from_device_arrays((batch * n, batch * n), self._indptr,
+                                                    self._indices, self._values, precision)
+        self._peierls = torch.from_numpy(pat.peierls).to(dev)
+        self._u = torch.from_numpy(pat.u_entry).to(dev)
+        self._diag = torch.from_numpy(pat.is_diag).to(dev)
+        rng = np.random.RandomState(vector_seed)
+        rng.random_sample(n)
+        vecs = np.stack([np.exp(2j * np.pi * rng.random_sample(n))
+                         for _ in range(num_vectors)], axis=1)
+        self._vectors = torch.from_numpy(vecs).to(dev).to(self.cdt)
+        if variant < 0:
+            from . import autotune
+            variant = autotune.pick_variant(self.mat, (self.a, self.b), num_vectors,
+                                            eng.DOTS_DOUBLING)
+        self.plan = eng.Plan(batch * n, num_vectors, vt, seg_rows=n * np.arange(batch + 1),
+                             variant=variant)
+        self.plan.set_scale([self.a, self.b] * batch)
+        self.ring = torch.empty((2, batch * n, num_vectors), dtype=self.cdt, device=dev)
+        self.steps = num_moments // 2
+        self.g = jackson_factors(num_moments)
+
+    def _load(self, params):
+        """Write the values of the ``batch`` systems into the block matrix."""
+        import torch
+        p = torch.from_numpy(np.asarray(params, dtype=np.float64)).to(self._u.device)
+        w, phi, mu = p[:, 0:1], p[:, 1:2], p[:, 2:3]
+        hop = torch.polar(torch.ones_like(self._peierls),
+                          2 * np.pi * phi * self._peierls[None, :])
+        diag = (w * self._u[None, :] + mu).to(torch.complex128)
+        vals = torch.where(self._diag[None, :], diag, hop)
+        self._values.view(self.batch, -1).copy_(vals)
+
+    def dots(self, params):
+        """Launch the recursion for all points; returns the FP64 dot records
+        (batches, steps, batch, R, 4) on the device (asynchronous)."""
+        import torch
+        params = np.asarray(params, dtype=np.float64).reshape(-1, 3)
+        nb = -(-len(params) // self.batch)
+        pad = np.concatenate([params, np.repeat(params[-1:], nb * self.batch - len(params), 0)])
+        out = torch.empty((nb, self.steps, self.batch, self.num_vectors, 4),
+                          dtype=torch.float64, device=self._u.device)
+        n, r = self.n, self.num_vectors
+        for i in range(nb):
+            self._load(pad[i * self.batch:(i + 1) * self.batch])
+            self.ring[0].view(self.batch, n, r).copy_(self._vectors[None].expand(self.batch, n, r))
+            self.plan.set_step(0)
+            self.plan.steps(self.mat, self.ring, self.batch * n * r, 2, self.steps,
+                            eng.DOTS_DOUBLING, dots=out[i])
+        return out
+
+    def moments(self, params):
+        """Normalised moments mu_n / mu_0 averaged over vectors, (P, M)."""
+        params = np.asarray(params, dtype=np.float64).reshape(-1, 3)
+        # the doubling identity is linear in the dots, so average over the
+        # vectors first (kwant/kpm.py:470-473 per vector, then the mean)
+        d = self.dots(params).mean(dim=3).cpu().numpy()
+        nb, steps, batch, _ = d.shape
+        d = d.transpose(0, 2, 1, 3).reshape(nb * batch, steps, 4)[:len(params)]
+        out = np.empty((len(params), self.num_moments))
+        mu0 = d[:, 0, 0]
+        out[:, 0] = mu0
+        out[:, 1] = d[:, 0, 2]                     # Re conj(<a1|a0>)
+        out[:, 2::2] = 2 * d[:, 1:, 0] - mu0[:, None]
+        out[:, 3::2] = 2 * d[:, 1:, 2] - d[:, 0:1, 2]
+        return out / mu0[:, None]
+
+    def damped_moments(self, params):
+        """Jackson-damped normalised moments g_n mu_n / mu_0, (P, M)."""
+        return self.moments(params) * self.g
+
+
+class MomentMLP:
+    """An ensemble of ``members`` MLPs ``layers[0] -> ... -> layers[-1]``
+    (GELU between layers, linear output) on the GPU.
+
+    Parameters live in one flat FP32 buffer, layer by layer: W_l as
+    (members, out, in) then b_l as (members, out); gradients and the two
+    Adam moments have the same layout, so one Adam launch updates everything.
+    Every GEMM is one batched launch over the members.  ``split`` lists, per
+    layer, whether its forward product uses the three-term BF16 split
+    (input resolution matters in the first layer)."""
+
+    def __init__(self, layers=LAYERS, members=5, seed=0, split=None):
+        import torch
+        dev = eng.torch_device()
+        self.layers, self.members = tuple(layers), members
+        self.split = [True] + [False] * (len(layers) - 2) if split is None else list(split)
+        if len(self.split) != len(layers) - 1:
+            raise ValueError("split needs one flag per layer")
+        self.shapes, off = [], 0
+        for fan_in, fan_out in zip(layers[:-1], layers[1:]):
+            self.shapes.append((off, off + members * fan_out * fan_in, fan_in, fan_out))
+            off += members * fan_out * (fan_in + 1)
+        self.size = off
+        flat = np.zeros(off, dtype=np.float32)
+        for z in range(members):
+            rng = np.random.RandomState(1000 * seed + z)
+            for li, (w0, b0, fi, fo) in enumerate(self.shapes):
+                gain = np.sqrt(2.0 / fi) if li < len(self.shapes) - 1 else np.sqrt(1.0 / fi)
+                w = (rng.standard_normal((fo, fi)) * gain).astype(np.float32)
+                flat[w0:b0].reshape(members, fo, fi)[z] = w
+        self.params = torch.from_numpy(flat).to(dev)
+        self.grads = torch.zeros_like(self.params)
+        self.adam_m = torch.zeros_like(self.params)
+        self.adam_v = torch.zeros_like(self.params)
+        self.t = 0
+        self._ws = {}
+
+    def weight(self, li, buf=None):
+        w0, b0, fi, fo = self.shapes[li]
+        return (self.params if buf is None else buf)[w0:b0].view(self.members, fo, fi)
+
+    def bias(self, li, buf=None):
+        w0, b0, fi, fo = self.shapes[li]
+        return (self.params if buf is None else buf)[b0:b0 + self.members * fo].view(
+            self.members, fo)
+
+    def _workspace(self, batch):
+        import torch
+        ws = self._ws.get(batch)
+        if ws is None:
+            dev, z = self.params.device, self.members
+            ws = {"z": [torch.empty((z, batch, fo), dtype=torch.float32, device=dev)
+                        for (_, _, _, fo) in self.shapes],
+                  "a": [torch.empty((z, batch, fo), dtype=torch.float32, device=dev)
+                        for (_, _, _, fo) in self.shapes[:-1]],
+                  "g": [torch.empty((z, batch, fo), dtype=torch.float32, device=dev)
+                        for (_, _, _, fo) in self.shapes[:-1]],
+                  "dy": torch.empty((z, batch, self.layers[-1]), dtype=torch.float32,
+                                    device=dev),
+                  "partial": torch.empty(256, dtype=torch.float64, device=dev)}
+            self._ws[batch] = ws
+        return ws
+
+    def forward(self, x):
+        """x: (members or 1, B, layers[0]) float32; returns (members, B, out)
+        (a workspace view, valid until the next call with this B)."""
+        z, b, _ = x.shape
+        if z == 1 and self.members > 1:
+            x = x.expand(self.members, b, x.shape[2])
+        ws = self._workspace(b)
+        act = x
+        last = len(self.shapes) - 1
+        for li in range(len(self.shapes)):
+            flags = eng.MLP_BF16X3 if self.split[li] else 0
+            if li < last:
+                eng.mlp_gemm(act, self.weight(li), ws["z"][li], bias=self.bias(li),
+                             flags=eng.MLP_BIAS_GELU | flags, c2=ws["a"][li])
+                act = ws["a"][li]
+            else:
+                eng.mlp_gemm(act, self.weight(li), ws["z"][li], bias=self.bias(li),
+                             flags=eng.MLP_PLAIN | flags)
+        self._x = x
+        return ws["z"][last]
+
+    def backward(self, dy):
+        """Gradients of the loss whose output gradient is dy (members, B,
+        out) for the inputs of the last forward call, into ``self.grads``."""
+        b = dy.shape[1]
+        ws = self._workspace(b)
+        g = dy
+        for li in range(len(self.shapes) - 1, -1, -1):
+            prev = self._x if li == 0 else ws["a"][li - 1]
+            eng.mlp_gemm(g.transpose(1, 2), prev.transpose(1, 2), self.weight(li, self.grads))
+            eng.mlp_colsum(g, self.bias(li, self.grads))
+            if li > 0:
+                eng.mlp_gemm(g, self.weight(li).transpose(1, 2), ws["g"][li - 1],
+                             flags=eng.MLP_GELU_GRAD, aux=ws["z"][li - 1])
+                g = ws["g"][li - 1]
+
+    def loss_grad(self, y, target):
+        """dy = d(mean squared error per member)/dy into the workspace;
+        returns (dy, FP64 partial sums of squared errors)."""
+        ws = self._workspace(y.shape[1])
+        scale = 2.0 / (y.shape[1] * y.shape[2])
+        eng.mlp_mse(y, target, ws["dy"], scale, ws["partial"])
+        return ws["dy"], ws["partial"]
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+        self.t += 1
+        eng.mlp_adam(self.params, self.grads, self.adam_m, self.adam_v, lr, b1, b2, eps, self.t)
+
+
+class MomentSurrogate:
+    """Ensemble surrogate of ``family.damped_moments`` with a KPM fallback."""
+
+    def __init__(self, family, members=5, layers=None, seed=0, tol=1e-3, split=None):
+        self.family = family
+        m = family.num_moments
+        layers = LAYERS[:-1] + (m,) if layers is None else tuple(layers)
+        if layers[0] != 3 or layers[-1] != m:
+            raise ValueError("layers must map 3 parameters to the family's moments")
+        self.mlp = MomentMLP(layers, members, seed, split)
+        self.tol = tol
+        import torch
+        c = dos_l1_matrix(m)
+        # node densities c[k, n] d_n, row-scaled by K so that values are O(1)
+        self._l1 = torch.from_numpy((c * c.shape[0]).astype(np.float32)).to(
+            self.mlp.params.device)
+        self.history = []
+
+    def fit(self, params, targets, epochs=400, batch=256, lr=2e-3, lr_min=2e-5, seed=0,
+            log=None):
+        """Train every member on (params, targets) with Adam and a cosine
+        learning-rate schedule; members draw their own mini-batch order."""
+        import torch
+        dev = self.mlp.params.device
+        x = torch.from_numpy(normalize_params(params).astype(np.float32)).to(dev)
+        t = torch.from_numpy(np.asarray(targets, dtype=np.float32)).to(dev)
+        count = len(x)
+        per_epoch = max(1, count // batch)
+        total = epochs * per_epoch
+        gen = torch.Generator(device="cpu")
+        gen.manual_seed(seed)
+        z = self.mlp.members
+        it = 0
+        for ep in range(epochs):
+            order = torch.stack([torch.randperm(count, generator=gen)
+                                 for _ in range(z)]).to(dev)
+            ep_loss = torch.zeros((), dtype=torch.float64, device=dev)
+            for s in range(per_epoch):
+                idx = order[:, s * batch:(s + 1) * batch]
+                xb, tb = x[idx], t[idx]
+                y = self.mlp.forward(xb)
+                dy, partial = self.mlp.loss_grad(y, tb)
+                ep_loss += partial.sum()
+                self.mlp.backward(dy)
+                frac = it / max(1, total - 1)
+                self.mlp.adam(lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * frac)))
+                it += 1
+            if log is not None and (ep % max(1, epochs // 10) == 0 or ep == epochs - 1):
+                mse = float(ep_loss) / (per_epoch * batch * z * t.shape[1])
+                self.history.append((ep, mse))
+                log("epoch %d: mse %.3e" % (ep, mse))
+
+    def predict(self, params, chunk=4096):
+        """Ensemble mean moments (P, M) and spread (P,) on the device: the
+        spread is the largest DOS L1 distance of a member from the mean."""
+        import torch
+        dev = self.mlp.params.device
+        x = torch.from_numpy(normalize_params(params).astype(np.float32)).to(dev)
+        count, m = len(x), self.family.num_moments
+        mean = torch.empty((count, m), dtype=torch.float32, device=dev)
+        spread = torch.empty(count, dtype=torch.float32, device=dev)
+        k = self._l1.shape[0]
+        nodes = torch.empty((self.mlp.members, min(chunk, count), k), dtype=torch.float32,
+                            device=dev)
+        for s in range(0, count, chunk):
+            e = min(count, s + chunk)
+            y = self.mlp.forward(x[None, s:e])
+            eng.mlp_ensemble(y, mean=mean[s:e])
+            nd = nodes[:, :e - s]
+            eng.mlp_gemm(y, self._l1[None].expand(self.mlp.members, k, m), nd,
+                         flags=eng.MLP_BF16X3)
+            eng.mlp_ensemble(nd, dev=spread[s:e])
+        return mean, spread
+
+    def sweep(self, params):
+        """Damped moments for every parameter point: the ensemble mean where
+        the spread is within ``tol``, KPM elsewhere.  Returns (moments (P, M)
+        FP64 host array, fallback mask, seconds)."""
+        import torch
+        params = np.asarray(params, dtype=np.float64).reshape(-1, 3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mean, spread = self.predict(params)
+        out = mean.double().cpu().numpy()
+        fall = spread.cpu().numpy() > self.tol
+        if fall.any():
+            out[fall] = self.family.damped_moments(params[fall])
+        torch.cuda.synchronize()
+        return out, fall, time.perf_counter() - t0

@@ -1,0 +1,183 @@
+"""GPU kernels of the moment surrogate against the CPU oracle
+(oracle/mlp_oracle.py), and the batched square-lattice moment generator
+against the KPM oracle (oracle/kpm_oracle.py)."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+
+
+def _close(got, want, tol):
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    scale = max(np.max(np.abs(want)), 1e-30)
+    err = np.max(np.abs(got - want)) / scale
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("m,n,k", [(64, 64, 32), (70, 130, 33), (5, 3, 300), (257, 96, 3)])
+@pytest.mark.parametrize("split", [False, True])
+def test_gemm_plain(m, n, k, split):
+    """C = A B^T (+ bias) with BF16 operands and FP32 accumulation."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(m + n + k)
+    z = 3
+    a = rng.standard_normal((z, m, k)).astype(np.float32)
+    b = rng.standard_normal((z, n, k)).astype(np.float32)
+    bias = rng.standard_normal((z, n)).astype(np.float32)
+    c = torch.full((z, m, n), np.nan, dtype=torch.float32, device="cuda")
+    eng.mlp_gemm(_t(a), _t(b), c, bias=_t(bias), flags=eng.MLP_BF16X3 if split else 0)
+    got = c.cpu().numpy()
+    for i in range(z):
+        want = mo.gemm(a[i], b[i], split) + bias[i]
+        _close(got[i], want, 2e-6 if not split else 1e-5)
+        if split:  # the split product is close to the FP32 product
+            _close(got[i], a[i].astype(np.float64) @ b[i].T.astype(np.float64) + bias[i], 1e-4)
+
+
+def test_gemm_strides_broadcast_accumulate():
+    """Transposed operands (unit stride along rows), a broadcast operand
+    (stride 0 over the batch) and accumulation into C."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(7)
+    z, m, n, k = 2, 50, 40, 77
+    at = rng.standard_normal((z, k, m)).astype(np.float32)     # A^T stored
+    bt = rng.standard_normal((k, n)).astype(np.float32)        # B^T stored, shared
+    c0 = rng.standard_normal((z, m, n)).astype(np.float32)
+    c = _t(c0)
+    a_view = _t(at).transpose(1, 2)
+    b_view = _t(bt).t()[None].expand(z, n, k)
+    eng.mlp_gemm(a_view, b_view, c, flags=eng.MLP_ACCUMULATE)
+    got = c.cpu().numpy()
+    for i in range(z):
+        _close(got[i], c0[i] + mo.gemm(at[i].T, bt.T), 2e-6)
+
+
+def test_gemm_epilogues():
+    """BIAS_GELU writes Z and gelu(Z); GELU_GRAD multiplies by gelu'(aux)."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(11)
+    z, m, n, k = 2, 33, 65, 40
+    a = rng.standard_normal((z, m, k)).astype(np.float32)
+    b = (rng.standard_normal((z, n, k)) / 6).astype(np.float32)
+    bias = rng.standard_normal((z, n)).astype(np.float32)
+    aux = rng.standard_normal((z, m, n)).astype(np.float32)
+    zc = torch.empty((z, m, n), dtype=torch.float32, device="cuda")
+    ac = torch.empty_like(zc)
+    eng.mlp_gemm(_t(a), _t(b), zc, bias=_t(bias), flags=eng.MLP_BIAS_GELU, c2=ac)
+    gc = torch.empty_like(zc)
+    eng.mlp_gemm(_t(a), _t(b), gc, flags=eng.MLP_GELU_GRAD, aux=_t(aux))
+    for i in range(z):
+        want = mo.gemm(a[i], b[i]) + bias[i]
+        _close(zc[i].cpu().numpy(), want, 2e-6)
+        _close(ac[i].cpu().numpy(), mo.gelu(want), 2e-6)
+        _close(gc[i].cpu().numpy(), mo.gemm(a[i], b[i]) * mo.gelu_grad(aux[i]), 2e-6)
+
+
+def test_gemm_argument_errors():
+    import torch
+    from kwant_amd import _engine as eng
+    a = torch.zeros((1, 4, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError):
+        eng.mlp_gemm(a, a, a, flags=eng.MLP_BIAS_GELU)            # no activation output
+    with pytest.raises(ValueError):
+        eng.mlp_gemm(a, a, a, flags=eng.MLP_GELU_GRAD)            # no pre-activation
+    with pytest.raises(ValueError):
+        eng.mlp_gemm(a, a, a, flags=eng.MLP_BIAS_GELU | eng.MLP_ACCUMULATE, c2=a)
+    with pytest.raises(ValueError):
+        eng.mlp_gemm(a, a, a, flags=64)
+
+
+def test_colsum_mse_ensemble_adam():
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(5)
+    g = rng.standard_normal((3, 37, 70)).astype(np.float32)
+    out = torch.empty((3, 70), dtype=torch.float32, device="cuda")
+    eng.mlp_colsum(_t(g), out)
+    _close(out.cpu().numpy(), g.astype(np.float64).sum(axis=1), 1e-6)
+
+    y = rng.standard_normal(5000).astype(np.float32)
+    t = rng.standard_normal(5000).astype(np.float32)
+    dy = torch.empty(5000, dtype=torch.float32, device="cuda")
+    part = torch.empty(7, dtype=torch.float64, device="cuda")
+    eng.mlp_mse(_t(y), _t(t), dy, 2.0 / 5000, part)
+    loss, grad = mo.mse(y, t)
+    _close(dy.cpu().numpy(), grad, 1e-6)
+    assert abs(float(part.sum()) / 5000 - loss) < 1e-6 * loss
+
+    ens = rng.standard_normal((5, 21, 130)).astype(np.float32)
+    mean = torch.empty((21, 130), dtype=torch.float32, device="cuda")
+    dev = torch.empty(21, dtype=torch.float32, device="cuda")
+    eng.mlp_ensemble(_t(ens), mean=mean, dev=dev)
+    m_ref = ens.astype(np.float64).mean(axis=0)
+    _close(mean.cpu().numpy(), m_ref, 1e-6)
+    _close(dev.cpu().numpy(), np.abs(ens - m_ref).mean(axis=2).max(axis=0), 1e-5)
+
+    p = rng.standard_normal(1000).astype(np.float32)
+    pm, pv = np.zeros_like(p), np.zeros_like(p)
+    tp, tm, tv = _t(p), _t(pm), _t(pv)
+    for step in range(1, 4):
+        gr = rng.standard_normal(1000).astype(np.float32)
+        eng.mlp_adam(tp, _t(gr), tm, tv, 1e-2, 0.9, 0.999, 1e-8, step)
+        p, pm, pv = mo.adam(p.astype(np.float64), gr, pm, pv, 1e-2, 0.9, 0.999, 1e-8, step)
+    _close(tp.cpu().numpy(), p, 1e-5)
+
+
+def test_mlp_forward_backward_vs_oracle():
+    """The ensemble network's output and every gradient, member by member."""
+    import torch
+    from kwant_amd.surrogate import MomentMLP
+    layers = (3, 64, 48, 80)
+    z, batch = 3, 40
+    net = MomentMLP(layers, members=z, seed=4)
+    rng = np.random.RandomState(9)
+    x = rng.uniform(-1, 1, (z, batch, 3)).astype(np.float32)
+    t = rng.standard_normal((z, batch, layers[-1])).astype(np.float32) * 0.1
+    y = net.forward(_t(x))
+    dy, _ = net.loss_grad(y, _t(t))
+    net.backward(dy)
+    y = y.cpu().numpy()
+    for i in range(z):
+        ws = [net.weight(l)[i].cpu().numpy() for l in range(len(layers) - 1)]
+        bs = [net.bias(l)[i].cpu().numpy() for l in range(len(layers) - 1)]
+        zs, acts, yo = mo.forward(ws, bs, x[i], split=net.split)
+        _close(y[i], yo, 1e-3)
+        # per-member loss: mean over that member's B x O entries
+        _, dyo = mo.mse(yo, t[i])
+        dws, dbs = mo.backward(ws, zs, acts, dyo)
+        for l in range(len(layers) - 1):
+            _close(net.weight(l, net.grads)[i].cpu().numpy(), dws[l], 2e-3)
+            _close(net.bias(l, net.grads)[i].cpu().numpy(), dbs[l], 2e-3)
+
+
+@pytest.mark.parametrize("precision,tol", [("complex128", 1e-11), ("complex64", 1e-5)])
+def test_square_family_vs_oracle(precision, tol):
+    """Batched block-diagonal moments equal kwant's SpectralDensity moments
+    (oracle restatement) of each system with the same seeded vectors."""
+    from oracle import kpm_oracle as ko
+    from kwant_amd.lattices import square
+    from kwant_amd.surrogate import SquareFamily
+    l, m, r = 12, 40, 4
+    fam = SquareFamily(l=l, num_moments=m, num_vectors=r, disorder_seed=4, vector_seed=6,
+                       batch=3, precision=precision, variant=0)
+    params = np.array([[1.5, 0.013, 0.3], [0.0, 0.0, -0.8], [3.9, 0.02, 1.0],
+                       [2.2, 0.005, 0.0], [0.7, 0.017, -0.2]])
+    got = fam.moments(params)
+    for p, (w, phi, mu) in enumerate(params):
+        h = square(l, disorder=w, seed=4, shift=mu, flux=phi)[0]
+        sd = ko.SpectralDensity(h, num_vectors=r, num_moments=m, bounds=(-7.0, 7.0),
+                                rng=6, kernel=lambda x: x)
+        ref = np.real(sd._moments())
+        ref = ref / ref[0]
+        err = np.max(np.abs(got[p] - ref))
+        assert err < tol, (p, err)
