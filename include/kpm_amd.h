@@ -173,6 +173,22 @@ int kpm_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb,
                   int64_t k, int32_t precision, int32_t k_chunk,
                   double* work, int64_t work_elems, kpm_stream_t stream);
 
+/* Exact moments after a uniform on-site shift (no counterpart in kwant/kpm.py;
+ * it acts on the moments kwant/kpm.py:436-477 produces).  nu holds moments
+ * <T_k(t)> in a window t = (E - b') / a'; the moments in the window
+ * y = alpha t + beta (for H + mu: alpha = a'/a, beta = (b' + mu - b)/a) are
+ *   mu_n = out_scale[n] * sum_{k<=n} C_nk in_scale[k] nu_k,
+ *   T_n(alpha t + beta) = sum_k C_nk T_k(t)   (three-term recurrence in FP64).
+ * Element (z, p, k) of nu is nu[z*nu_bs + p*nu_rs + k], likewise out; beta is a
+ * device array of rows doubles; in_scale / out_scale are device arrays of
+ * num_moments doubles or NULL (1).  Requires |alpha| + |beta[p]| <= 1 (the
+ * caller checks beta): then |C_nk| <= 2 and the recurrence is stable.
+ * num_moments <= 1024.  Asynchronous on stream. */
+int kpm_moment_shift(const float* nu, int64_t nu_bs, int64_t nu_rs, int32_t batch,
+                     int64_t rows, int32_t num_moments, const double* in_scale,
+                     const double* out_scale, double alpha, const double* beta,
+                     float* out, int64_t out_bs, int64_t out_rs, kpm_stream_t stream);
+
 /* ---- moment surrogate (no counterpart in kwant/kpm.py) ------------------
  * A small MLP that maps Hamiltonian parameters to Jackson-damped moments,
  * trained on moments from kpm_cheb_steps and evaluated in place of the

@@ -36,7 +36,7 @@ EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
     "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
-    "kpm_spmm", "kpm_corr_gemm", "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
+    "kpm_spmm", "kpm_corr_gemm", "kpm_moment_shift", "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
     "kpm_mlp_ensemble", "kpm_mlp_adam",
 )
 
@@ -103,6 +103,8 @@ def load_library(path=LIB_PATH):
         "kpm_spmm": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, i64, vp, i64, i32, i32, vp]),
         "kpm_corr_gemm": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
                                          i32, vp, i64, vp]),
+        "kpm_moment_shift": (ctypes.c_int, [vp, i64, i64, i32, i64, i32, vp, vp,
+                                            ctypes.c_double, vp, vp, i64, i64, vp]),
         "kpm_mlp_gemm": (ctypes.c_int, [ctypes.POINTER(MlpGemmDesc), vp]),
         "kpm_mlp_colsum": (ctypes.c_int, [vp, i64, i64, i32, i32, i32, vp, i64, vp]),
         "kpm_mlp_mse": (ctypes.c_int, [vp, vp, vp, i64, f32, vp, i32, vp]),
@@ -381,6 +383,29 @@ def corr_gemm(a, b, mu, precision, k_chunk=256, work=None):
         a.data_ptr(), a.stride(0), b.data_ptr(), b.stride(0), mu.data_ptr(), mu.stride(0),
         m, n, k, precision, k_chunk, work.data_ptr(), work.numel(), stream_handle()),
         "kpm_corr_gemm")
+
+
+def moment_shift(nu, alpha, beta, out, in_scale=None, out_scale=None):
+    """Moments of H + mu from those of H (see kpm_moment_shift): nu, out:
+    (Z, P, M) float32 with unit stride along M; beta: (P,) float64 device
+    tensor with |alpha| + |beta| <= 1 (checked by the caller); scales: (M,)
+    float64 device tensors or None."""
+    import torch
+    if nu.dtype != torch.float32 or out.dtype != torch.float32 or nu.dim() != 3:
+        raise ValueError("nu and out must be (Z, P, M) float32")
+    z, p, m = nu.shape
+    if tuple(out.shape) != (z, p, m) or nu.stride(2) != 1 or out.stride(2) != 1:
+        raise ValueError("moment_shift layout")
+    if beta.dtype != torch.float64 or beta.numel() != p or not beta.is_contiguous():
+        raise ValueError("beta must be a contiguous (P,) float64 tensor")
+    for sc in (in_scale, out_scale):
+        if sc is not None and (sc.dtype != torch.float64 or sc.numel() != m):
+            raise ValueError("scales must be (M,) float64")
+    check(load_library().kpm_moment_shift(
+        nu.data_ptr(), nu.stride(0), nu.stride(1), z, p, m,
+        None if in_scale is None else in_scale.data_ptr(),
+        None if out_scale is None else out_scale.data_ptr(), float(alpha), beta.data_ptr(),
+        out.data_ptr(), out.stride(0), out.stride(1), stream_handle()), "kpm_moment_shift")
 
 
 def _f32_view(t, name, dims=3):

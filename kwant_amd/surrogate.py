@@ -3,25 +3,31 @@ KPM recursion, with plain KPM as the fallback and as the judge.
 
 The family is the square-lattice Hamiltonian H(W, phi, mu) of
 :class:`kwant_amd.lattices.SquarePattern` (L x L sites, hopping 1 with a
-Peierls flux phi per plaquette in Landau gauge, Anderson disorder
-W u_i, u_i ~ U[-1/2, 1/2] with a fixed draw, uniform on-site shift mu), with
-one fixed spectral window (bounds (-7, 7) for every member of the family, so
-that the rescaled moments carry the shift and the disorder).  Its output is
-the Jackson-damped (kwant/kpm.py:1106-1123), normalised DOS moments
+Peierls flux phi per plaquette in Landau gauge, Anderson disorder W u_i,
+u_i ~ U[-1/2, 1/2] with a fixed draw, uniform on-site shift mu).  A sweep
+asks for the Jackson-damped (kwant/kpm.py:1106-1123), normalised DOS moments
 g_n mu_n / mu_0 that kwant's SpectralDensity averages over random vectors
-(kwant/kpm.py:436-477 for the recursion).
+(kwant/kpm.py:436-477), in one spectral window common to the whole family
+(bounds (-7, 7), so that shift and disorder show in the rescaled moments).
 
-* ``SquareFamily`` computes those moments with the GPU recursion for many
-  parameter points at once: ``batch`` systems are the diagonal blocks of one
-  CSR matrix, one segment each, so one launch per Chebyshev step advances all
-  of them (each segment has its own dot products).
-* ``MomentMLP`` is an ensemble of MLPs 3 -> 256 -> 256 -> 256 -> 256 -> 512
-  (GELU) whose forward, backward and Adam update run on the kernels of
-  libkpm_amd (BF16 matrix cores, FP32 accumulation); all members train in the
-  same launches (one batched GEMM per layer).
-* ``MomentSurrogate`` trains the ensemble on moments from ``SquareFamily`` and
-  answers sweeps: where the members' densities differ from their mean by more
-  than ``tol`` in L1 (the spread gate) the point is recomputed with KPM.
+The shift mu enters exactly: H(W, phi, mu) = H(W, phi, 0) + mu, so the
+moments in the common window are a fixed linear function (kpm_moment_shift)
+of the moments of H(W, phi, 0) in a tighter window that contains every
+unshifted spectrum (bounds (-6, 6)).  The identity holds per random vector,
+so it reproduces the recursion's own estimate, not only its mean.  The
+network therefore learns the (W, phi) dependence only; mu is applied by the
+exact shift layer after it.
+
+* ``SquareFamily`` computes moments with the GPU recursion for many parameter
+  points at once: ``batch`` systems are the diagonal blocks of one CSR matrix,
+  one segment each, so one launch per Chebyshev step advances all of them.
+* ``MomentMLP`` is an ensemble of MLPs (GELU, linear output) whose forward,
+  backward and Adam update run on the kernels of libkpm_amd (BF16 matrix
+  cores, FP32 accumulation); all members train in the same launches.
+* ``MomentSurrogate`` trains the ensemble on tight-window moments of
+  H(W, phi, 0) and answers sweeps over (W, phi, mu): where the members'
+  densities differ from their mean by more than ``tol`` in L1 (the spread
+  gate) the point is recomputed with plain KPM.
 
 DOS L1 distance of two damped moment vectors d = g (mu - mu'):
     L1 = int |rho - rho'| dx = (1/K) sum_k |d_0 + 2 sum_n d_n cos(n theta_k)|,
@@ -41,7 +47,9 @@ from .lattices import SquarePattern
 PARAM_NAMES = ("disorder", "flux", "shift")
 PARAM_LO = np.array([0.0, 0.0, -1.0])
 PARAM_HI = np.array([4.0, 0.02, 1.0])
-LAYERS = (3, 256, 256, 256, 256, 512)
+LAYERS = (2, 256, 256, 256, 256, 512)
+WIDE_BOUNDS = (-7.0, 7.0)     # common window of the sweep's moments
+TIGHT_BOUNDS = (-6.0, 6.0)    # contains every spectrum of H(W <= 4, phi, 0)
 
 
 def jackson_factors(count):
@@ -81,7 +89,7 @@ class SquareFamily:
     (kwant/kpm.py:203), then one exp(2 pi i random_sample(N)) per vector
     (kwant/kpm.py:932-951), the same vectors for every parameter point."""
 
-    def __init__(self, l=512, num_moments=512, num_vectors=32, bounds=(-7.0, 7.0), eps=0.05,
+    def __init__(self, l=512, num_moments=512, num_vectors=32, bounds=WIDE_BOUNDS, eps=0.05,
                  disorder_seed=0, vector_seed=1, batch=8, precision="complex64", variant=-1):
         import torch
         if num_moments < 2 or num_moments % 2:
@@ -135,11 +143,20 @@ class SquareFamily:
         vals = torch.where(self._diag[None, :], diag, hop)
         self._values.view(self.batch, -1).copy_(vals)
 
-    def dots(self, params):
+    def dots(self, params, fresh_seed=None):
         """Launch the recursion for all points; returns the FP64 dot records
-        (batches, steps, batch, R, 4) on the device (asynchronous)."""
+        (batches, steps, batch, R, 4) on the device (asynchronous).
+
+        fresh_seed=None: every point uses the seeded kwant vectors; otherwise
+        every point gets its own random-phase vectors exp(2 pi i u) from the
+        device generator seeded with ``fresh_seed`` (independent stochastic
+        errors across points, as training data wants)."""
         import torch
         params = np.asarray(params, dtype=np.float64).reshape(-1, 3)
+        gen = None
+        if fresh_seed is not None:
+            gen = torch.Generator(device=self._u.device)
+            gen.manual_seed(int(fresh_seed))
         nb = -(-len(params) // self.batch)
         pad = np.concatenate([params, np.repeat(params[-1:], nb * self.batch - len(params), 0)])
         out = torch.empty((nb, self.steps, self.batch, self.num_vectors, 4),
@@ -147,18 +164,24 @@ class SquareFamily:
         n, r = self.n, self.num_vectors
         for i in range(nb):
             self._load(pad[i * self.batch:(i + 1) * self.batch])
-            self.ring[0].view(self.batch, n, r).copy_(self._vectors[None].expand(self.batch, n, r))
+            if gen is None:
+                self.ring[0].view(self.batch, n, r).copy_(
+                    self._vectors[None].expand(self.batch, n, r))
+            else:
+                u = torch.rand((self.batch * n, r), generator=gen, device=self._u.device,
+                               dtype=torch.float64)
+                self.ring[0].copy_(torch.polar(torch.ones_like(u), 2 * np.pi * u))
             self.plan.set_step(0)
             self.plan.steps(self.mat, self.ring, self.batch * n * r, 2, self.steps,
                             eng.DOTS_DOUBLING, dots=out[i])
         return out
 
-    def moments(self, params):
+    def moments(self, params, fresh_seed=None):
         """Normalised moments mu_n / mu_0 averaged over vectors, (P, M)."""
         params = np.asarray(params, dtype=np.float64).reshape(-1, 3)
         # the doubling identity is linear in the dots, so average over the
         # vectors first (kwant/kpm.py:470-473 per vector, then the mean)
-        d = self.dots(params).mean(dim=3).cpu().numpy()
+        d = self.dots(params, fresh_seed).mean(dim=3).cpu().numpy()
         nb, steps, batch, _ = d.shape
         d = d.transpose(0, 2, 1, 3).reshape(nb * batch, steps, 4)[:len(params)]
         out = np.empty((len(params), self.num_moments))
@@ -169,9 +192,9 @@ class SquareFamily:
         out[:, 3::2] = 2 * d[:, 1:, 2] - d[:, 0:1, 2]
         return out / mu0[:, None]
 
-    def damped_moments(self, params):
+    def damped_moments(self, params, fresh_seed=None):
         """Jackson-damped normalised moments g_n mu_n / mu_0, (P, M)."""
-        return self.moments(params) * self.g
+        return self.moments(params, fresh_seed) * self.g
 
 
 class MomentMLP:
@@ -182,14 +205,16 @@ class MomentMLP:
     (members, out, in) then b_l as (members, out); gradients and the two
     Adam moments have the same layout, so one Adam launch updates everything.
     Every GEMM is one batched launch over the members.  ``split`` lists, per
-    layer, whether its forward product uses the three-term BF16 split
-    (input resolution matters in the first layer)."""
+    layer, whether its forward product uses the three-term BF16 split; by
+    default every forward product does (outputs accurate to ~1e-5 relative,
+    which moment regression needs), while the backward pass runs plain BF16
+    (gradients tolerate the rounding)."""
 
     def __init__(self, layers=LAYERS, members=5, seed=0, split=None):
         import torch
         dev = eng.torch_device()
         self.layers, self.members = tuple(layers), members
-        self.split = [True] + [False] * (len(layers) - 2) if split is None else list(split)
+        self.split = [True] * (len(layers) - 1) if split is None else list(split)
         if len(self.split) != len(layers) - 1:
             raise ValueError("split needs one flag per layer")
         self.shapes, off = [], 0
@@ -287,47 +312,70 @@ class MomentMLP:
 
 
 class MomentSurrogate:
-    """Ensemble surrogate of ``family.damped_moments`` with a KPM fallback."""
+    """Ensemble surrogate of the damped moments of H(W, phi, mu) in the
+    window of ``family`` (the plain-KPM family that also serves the
+    fallback), learned from moments of H(W, phi, 0) in the tighter window of
+    ``train_family`` and carried to any mu by the exact shift."""
 
-    def __init__(self, family, members=5, layers=None, seed=0, tol=1e-3, split=None):
-        self.family = family
-        m = family.num_moments
-        layers = LAYERS[:-1] + (m,) if layers is None else tuple(layers)
-        if layers[0] != 3 or layers[-1] != m:
-            raise ValueError("layers must map 3 parameters to the family's moments")
-        self.mlp = MomentMLP(layers, members, seed, split)
-        self.tol = tol
+    def __init__(self, family, train_family, members=5, layers=LAYERS, seed=0, tol=1e-3,
+                 split=None):
         import torch
+        m = family.num_moments
+        layers = tuple(layers)
+        if layers[0] != 2 or layers[-1] != m or train_family.num_moments != m:
+            raise ValueError("layers must map (W, phi) to the family's moments")
+        self.family, self.train_family = family, train_family
+        self.tol = tol
+        self.alpha = train_family.a / family.a
+        # H(W, phi, mu) in the wide window = alpha t + beta, t the tight variable
+        self._beta0 = (train_family.b - family.b) / family.a
+        # the largest |mu| whose shifted window stays inside the tight one
+        self.max_shift = (1.0 - self.alpha) * family.a - abs(train_family.b - family.b)
+        self.mlp = MomentMLP(layers, members, seed, split)
+        dev = self.mlp.params.device
+        self._in_scale = torch.from_numpy(1.0 / train_family.g).to(dev)
+        self._out_scale = torch.from_numpy(family.g).to(dev)
         c = dos_l1_matrix(m)
         # node densities c[k, n] d_n, row-scaled by K so that values are O(1)
-        self._l1 = torch.from_numpy((c * c.shape[0]).astype(np.float32)).to(
-            self.mlp.params.device)
+        self._l1 = torch.from_numpy((c * c.shape[0]).astype(np.float32)).to(dev)
         self.history = []
 
-    def fit(self, params, targets, epochs=400, batch=256, lr=2e-3, lr_min=2e-5, seed=0,
+    def _inputs(self, params):
+        """Normalised (W, phi) of parameter rows (P, 2) or (P, 3)."""
+        import torch
+        p = np.atleast_2d(np.asarray(params, dtype=np.float64))[:, :2]
+        x = normalize_params(p, PARAM_LO[:2], PARAM_HI[:2]).astype(np.float32)
+        return torch.from_numpy(x).to(self.mlp.params.device)
+
+    def training_targets(self, params, fresh_seed=1):
+        """Damped tight-window moments of H(W, phi, 0) for params (P, >= 2)
+        (any shift column is ignored), from the GPU recursion with fresh
+        random vectors per point."""
+        p = np.asarray(params, dtype=np.float64)
+        p = np.concatenate([p[:, :2], np.zeros((len(p), 1))], axis=1)
+        return self.train_family.damped_moments(p, fresh_seed=fresh_seed)
+
+    def fit(self, params, targets, epochs=400, batch=64, lr=3e-3, lr_min=3e-6, seed=0,
             log=None):
         """Train every member on (params, targets) with Adam and a cosine
         learning-rate schedule; members draw their own mini-batch order."""
         import torch
         dev = self.mlp.params.device
-        x = torch.from_numpy(normalize_params(params).astype(np.float32)).to(dev)
+        x = self._inputs(params)
         t = torch.from_numpy(np.asarray(targets, dtype=np.float32)).to(dev)
-        count = len(x)
+        count, z = len(x), self.mlp.members
         per_epoch = max(1, count // batch)
         total = epochs * per_epoch
-        gen = torch.Generator(device="cpu")
+        gen = torch.Generator(device=dev)
         gen.manual_seed(seed)
-        z = self.mlp.members
         it = 0
         for ep in range(epochs):
-            order = torch.stack([torch.randperm(count, generator=gen)
-                                 for _ in range(z)]).to(dev)
+            order = torch.argsort(torch.rand((z, count), generator=gen, device=dev), dim=1)
             ep_loss = torch.zeros((), dtype=torch.float64, device=dev)
             for s in range(per_epoch):
                 idx = order[:, s * batch:(s + 1) * batch]
-                xb, tb = x[idx], t[idx]
-                y = self.mlp.forward(xb)
-                dy, partial = self.mlp.loss_grad(y, tb)
+                y = self.mlp.forward(x[idx])
+                dy, partial = self.mlp.loss_grad(y, t[idx])
                 ep_loss += partial.sum()
                 self.mlp.backward(dy)
                 frac = it / max(1, total - 1)
@@ -338,32 +386,46 @@ class MomentSurrogate:
                 self.history.append((ep, mse))
                 log("epoch %d: mse %.3e" % (ep, mse))
 
-    def predict(self, params, chunk=4096):
-        """Ensemble mean moments (P, M) and spread (P,) on the device: the
-        spread is the largest DOS L1 distance of a member from the mean."""
+    def predict(self, params, chunk=4096, members=False):
+        """Ensemble mean of the damped wide-window moments (P, M) and the
+        spread (P,) on the device: the spread is the largest DOS L1 distance
+        of a member from the mean.  members=True also returns the per-member
+        moments (Z, P, M)."""
         import torch
+        p = np.asarray(params, dtype=np.float64).reshape(-1, 3)
+        if np.any(np.abs(p[:, 2]) > self.max_shift + 1e-12):
+            raise ValueError("shift outside the exact range |mu| <= %.4f" % self.max_shift)
         dev = self.mlp.params.device
-        x = torch.from_numpy(normalize_params(params).astype(np.float32)).to(dev)
-        count, m = len(x), self.family.num_moments
+        x = self._inputs(p)
+        beta = torch.from_numpy(self._beta0 + p[:, 2] / self.family.a).to(dev)
+        count, m, z = len(x), self.family.num_moments, self.mlp.members
         mean = torch.empty((count, m), dtype=torch.float32, device=dev)
         spread = torch.empty(count, dtype=torch.float32, device=dev)
+        per = torch.empty((z, count, m), dtype=torch.float32, device=dev) if members else None
         k = self._l1.shape[0]
-        nodes = torch.empty((self.mlp.members, min(chunk, count), k), dtype=torch.float32,
-                            device=dev)
+        step = min(chunk, count)
+        shifted = torch.empty((z, step, m), dtype=torch.float32, device=dev)
+        nodes = torch.empty((z, step, k), dtype=torch.float32, device=dev)
         for s in range(0, count, chunk):
             e = min(count, s + chunk)
             y = self.mlp.forward(x[None, s:e])
-            eng.mlp_ensemble(y, mean=mean[s:e])
+            sh = shifted[:, :e - s]
+            eng.moment_shift(y, self.alpha, beta[s:e], sh, in_scale=self._in_scale,
+                             out_scale=self._out_scale)
+            if per is not None:
+                per[:, s:e].copy_(sh)
+            eng.mlp_ensemble(sh, mean=mean[s:e])
             nd = nodes[:, :e - s]
-            eng.mlp_gemm(y, self._l1[None].expand(self.mlp.members, k, m), nd,
-                         flags=eng.MLP_BF16X3)
+            eng.mlp_gemm(sh, self._l1[None].expand(z, k, m), nd, flags=eng.MLP_BF16X3)
             eng.mlp_ensemble(nd, dev=spread[s:e])
+        if members:
+            return mean, spread, per
         return mean, spread
 
     def sweep(self, params):
         """Damped moments for every parameter point: the ensemble mean where
-        the spread is within ``tol``, KPM elsewhere.  Returns (moments (P, M)
-        FP64 host array, fallback mask, seconds)."""
+        the spread is within ``tol``, plain KPM (``family``) elsewhere.
+        Returns (moments (P, M) FP64 host array, fallback mask, seconds)."""
         import torch
         params = np.asarray(params, dtype=np.float64).reshape(-1, 3)
         torch.cuda.synchronize()
