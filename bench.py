@@ -83,6 +83,223 @@ def cpu_baseline(h, a, b, vec, steps):
         return (time.perf_counter() - t0) / (steps + 1)
 
 
+def surrogate_bench(epochs, tol, points, judge_points, judge_sets, log_fn):
+    """The moment surrogate on the square-lattice family (512 x 512 sites,
+    W in [0, 4], flux in [0, 0.02], mu in [-1, 1], 512 moments, 32 vectors):
+    train on 4096 GPU-KPM runs, sweep ``points`` random parameter points with
+    the spread gate and KPM fallback, and judge the result against converged
+    KPM (``judge_sets`` x 32 vectors) on ``judge_points`` of them."""
+    import numpy as np
+    import torch
+    from scipy.stats import qmc
+    from kwant_amd.surrogate import (SquareFamily, MomentSurrogate, dos_l1, PARAM_LO,
+                                     PARAM_HI, TIGHT_BOUNDS, WIDE_BOUNDS)
+    out = {}
+    wide = SquareFamily(l=512, num_moments=512, num_vectors=32, batch=8, bounds=WIDE_BOUNDS)
+    tight = SquareFamily(l=512, num_moments=512, num_vectors=32, batch=8, bounds=TIGHT_BOUNDS)
+    sur = MomentSurrogate(wide, tight, members=5, seed=0, tol=tol)
+    train = qmc.scale(qmc.Sobol(2, scramble=True, seed=8).random(4096), PARAM_LO[:2],
+                      PARAM_HI[:2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    targets = np.concatenate([sur.training_targets(train[s:s + 512], fresh_seed=100 + s)
+                              for s in range(0, 4096, 512)])
+    t_data = time.perf_counter() - t0
+    log_fn("surrogate: 4096 training runs in %.1f s" % t_data)
+    t0 = time.perf_counter()
+    sur.fit(train, targets, epochs=epochs, batch=64, lr=3e-3)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    log_fn("surrogate: %d epochs x 64 Adam steps, 5 members, in %.1f s" % (epochs, t_fit))
+    sweep = PARAM_LO + (PARAM_HI - PARAM_LO) * np.random.RandomState(7).random_sample((points, 3))
+    sur.predict(sweep[:64])
+    moments, fall, t_sweep = sur.sweep(sweep)
+    log_fn("surrogate: sweep of %d points in %.3f s (%.2f%% KPM fallback)"
+           % (points, t_sweep, 100 * fall.mean()))
+    sample = sweep[:64]
+    wide.damped_moments(sample[:8])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kpm = wide.damped_moments(sample)
+    t_point = (time.perf_counter() - t0) / len(sample)
+    t_plain = t_point * points
+    log_fn("plain KPM: %.2f ms per point -> %.1f s for the sweep" % (1e3 * t_point, t_plain))
+    judge = np.zeros((judge_points, 512))
+    for k in range(judge_sets):
+        rng = np.random.RandomState(1000 + k)
+        rng.random_sample(wide.n)
+        vecs = np.stack([np.exp(2j * np.pi * rng.random_sample(wide.n)) for _ in range(32)], 1)
+        wide._vectors = torch.from_numpy(vecs).to(wide._vectors.device).to(wide.cdt)
+        judge += wide.damped_moments(sweep[:judge_points]) / judge_sets
+    err = dos_l1(moments[:judge_points], judge)
+    err_kpm = dos_l1(kpm[:judge_points], judge)
+    log_fn("DOS L1 vs converged KPM (%d vectors): sweep median %.2e p90 %.2e max %.2e; plain "
+           "KPM(32) median %.2e" % (32 * judge_sets, np.median(err), np.percentile(err, 90),
+                                   err.max(), np.median(err_kpm)))
+    out.update({
+        "workload": "square 512x512, W in [0,4], flux in [0,0.02], mu in [-1,1], 512 moments, "
+                    "32 vectors; MLP (W, flux) -> 4x256 GELU -> 512 + exact shift in mu; "
+                    "5-member ensemble; %d sweep points" % points,
+        "speedup_vs_plain_kpm": t_plain / t_sweep,
+        "sweep_s": t_sweep, "plain_kpm_sweep_s": t_plain, "plain_kpm_ms_per_point": 1e3 * t_point,
+        "fallback_fraction": float(fall.mean()), "spread_tol": tol,
+        "dos_l1_vs_converged_median": float(np.median(err)),
+        "dos_l1_vs_converged_p90": float(np.percentile(err, 90)),
+        "dos_l1_vs_converged_max": float(err.max()),
+        "plain_kpm32_dos_l1_vs_converged_median": float(np.median(err_kpm)),
+        "judge": "KPM with %d random vectors on %d sweep points" % (32 * judge_sets,
+                                                                   judge_points),
+        "training_data_s": t_data, "training_s": t_fit, "epochs": epochs,
+    })
+    return out
+
+
+def ensemble_bench(count, moments, width, log_fn):
+    """The batched disorder ensemble: ``count`` cubic 4-orbital systems with
+    L ~ U{20..60}, generated on the GPU, ``moments`` moments with ``width``
+    vectors each, every step one launch over all systems, the step sequence
+    replayed from a captured HIP graph."""
+    import torch
+    from kwant_amd.ensemble import CubicEnsemble, ensemble_sizes
+    sizes = ensemble_sizes(count, 20, 60, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens = CubicEnsemble(sizes, seed=0, num_moments=moments, num_vectors=width)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    t_eager = ens.run(graph=False)
+    ens.run(graph=True)                      # capture + first replay
+    t_graph = min(ens.run(graph=True) for _ in range(2))
+    bstep = ens.bytes_per_step()
+    ms_step = 1e3 * t_graph / ens.steps
+    achieved = bstep / (ms_step * 1e-3) / 1e9
+    log_fn("ensemble: %d systems, N=%d, nnz=%d, W=%d: %.3f ms/step (graph), eager %.3f "
+           "ms/step, %.0f GB/s (%.1f%% of 8 TB/s); setup %.1f s"
+           % (count, ens.n, ens.nnz, width, ms_step, 1e3 * t_eager / ens.steps, achieved,
+              100 * achieved / HBM_PEAK_GBS, t_setup))
+    out = {"workload": "%d cubic 4-orbital systems, L ~ U{20..60}, sigma 0.3, t 1, %d moments, "
+                       "%d vectors each, complex64, one launch per step for all systems"
+                       % (count, moments, width),
+           "rows": ens.n, "nnz": ens.nnz, "steps": ens.steps, "ms_per_step": ms_step,
+           "ms_per_step_eager": 1e3 * t_eager / ens.steps,
+           "expansion_s": t_graph, "setup_s": t_setup,
+           "algorithmic_bytes_per_step": bstep, "achieved_GBs": achieved,
+           "frac_of_hbm_peak": achieved / HBM_PEAK_GBS, "variant": ens.variant}
+    del ens
+    torch.cuda.empty_cache()
+    return out
+
+
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+
+
+def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
+    """Kubo-Bastin sigma_xy of the four-band model (cells x cells sites,
+    A = 1, B = -1, M = -2, flux 1/cells per plaquette, Anderson W = 1):
+    the Gamma_nm contraction (moments x moments from ``vectors`` random
+    vectors) timed as one MFMA GEMM per precision at the real shape, and the
+    full conductivity in each precision against FP64."""
+    import numpy as np
+    import torch
+    from kwant_amd import _engine as eng
+    from kwant_amd.lattices import dirac_4band
+    from kwant_amd.correlator import _velocity, conductivity
+    t0 = time.perf_counter()
+    h, pos = dirac_4band(cells, A=1.0, B=-1.0, M=-2.0, flux=1.0 / cells, disorder=1.0, seed=4)
+    n = h.shape[0]
+    log_fn("four-band model %dx%d: N=%d nnz=%d (%.1f s)" % (cells, cells, n, h.nnz,
+                                                        time.perf_counter() - t0))
+    bound = float(np.max(np.abs(h).sum(axis=1)))
+    eps = 0.05
+    a, b = 2 * bound / (2 - eps), 0.0
+    vx = _velocity(h, None, 'x', pos)
+    vy = _velocity(h, None, 'y', pos)
+    dev = eng.torch_device()
+    m128, dvx, dvy = (eng.DeviceCSR(x, "complex128") for x in (h, vx, vy))
+    rng = np.random.RandomState(5)
+    block = torch.from_numpy(np.exp(2j * np.pi * rng.random_sample((n, width)))).to(dev)
+    # right expansion, Psi = v_y T_n|r>
+    rec_r = eng.BlockRecursion(m128, (a, b), block, width, eng.DOTS_NONE, store_slots=moments)
+    rec_r.step(moments - 1, want_dots=False)
+    psi = torch.empty_like(rec_r.ring)
+    dvy.spmm(rec_r.ring, psi, width, n_slots=moments, x_stride=n * width, y_stride=n * width)
+    del rec_r
+    start = torch.empty_like(block)
+    dvx.spmm(block, start, width)
+    rec_l = eng.BlockRecursion(m128, (a, b), start, width, eng.DOTS_NONE, store_slots=moments)
+    rec_l.step(moments - 1, want_dots=False)
+    omega = rec_l.ring
+    k = n * width
+    flops = 8.0 * moments * moments * k
+    out = {"workload": "four-band Dirac model %dx%d (N=%d), flux 1/%d, W=1; Gamma_nm %dx%d "
+                       "from %d vectors; GEMM timed on one %d-vector block (K = N x %d)"
+                       % (cells, cells, n, cells, moments, moments, vectors, width, width),
+           "gemm": {}}
+    ref = None
+    for name, prec, peak in (("fp64", eng.GEMM_FP64, "fp64"), ("fp32x3", eng.GEMM_FP32X3, "fp32"),
+                             ("fp32", eng.GEMM_FP32, "fp32")):
+        if prec == eng.GEMM_FP32:
+            a_op = omega.to(torch.complex64).reshape(moments, k)
+            b_op = psi.to(torch.complex64).reshape(moments, k)
+        else:
+            a_op, b_op = omega.reshape(moments, k), psi.reshape(moments, k)
+        mu = torch.zeros((moments, moments), dtype=torch.complex128, device=dev)
+        eng.corr_gemm(a_op, b_op, mu, prec)
+        torch.cuda.synchronize()
+        reps = 3
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            mu.zero_()
+            eng.corr_gemm(a_op, b_op, mu, prec)
+        e.record()
+        torch.cuda.synchronize()
+        sec = s.elapsed_time(e) / reps / 1e3
+        gam = mu.cpu().numpy()
+        if ref is None:
+            ref = gam
+        rel = float(np.linalg.norm(gam - ref) / np.linalg.norm(ref))
+        mult = 3 if name == "fp32x3" else 1
+        tflops = flops * mult / sec / 1e12
+        out["gemm"][name] = {"seconds": sec, "algorithmic_TFLOPs": flops / sec / 1e12,
+                             "mfma_TFLOPs": tflops, "mfma_peak_TFLOPs": MFMA_PEAK_TFLOPS[peak],
+                             "frac_of_mfma_peak": tflops / MFMA_PEAK_TFLOPS[peak],
+                             "rel_frobenius_err_vs_fp64": rel}
+        log_fn("Gamma GEMM %s: %.3f s, %.1f TFLOP/s on MFMA (%.1f%% of %.1f), rel err vs FP64 "
+               "%.2e" % (name, sec, tflops, 100 * tflops / MFMA_PEAK_TFLOPS[peak],
+                         MFMA_PEAK_TFLOPS[peak], rel))
+        del a_op, b_op, mu
+    del omega, psi, rec_l, start, block
+    torch.cuda.empty_cache()
+    if full:
+        sig = {}
+        for name, prec in (("fp64", "complex128"), ("fp32x3", "complex128"),
+                           ("fp32", "complex64")):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            c = conductivity(h, alpha='x', beta='y', positions=pos, num_moments=moments,
+                             num_vectors=vectors, bounds=(-bound, bound), rng=6,
+                             precision=prec, gemm_precision=name)
+            e_grid, s_grid = c.energies, np.array([c(mu=x, temperature=0.0)
+                                                    for x in (-1.0, -0.5, 0.0, 0.5, 1.0)])
+            torch.cuda.synchronize()
+            sig[name] = (s_grid, time.perf_counter() - t1)
+            del c
+            torch.cuda.empty_cache()
+        s64 = sig["fp64"][0]
+        out["sigma_xy"] = {"mu": [-1.0, -0.5, 0.0, 0.5, 1.0],
+                           "fp64": [float(np.real(v)) for v in s64]}
+        for name in ("fp32x3", "fp32"):
+            d = float(np.max(np.abs(sig[name][0] - s64)) / np.max(np.abs(s64)))
+            out["sigma_xy"][name + "_max_rel_dev"] = d
+        for name in sig:
+            out["sigma_xy"][name + "_seconds"] = sig[name][1]
+        log_fn("sigma_xy (FP64) at mu = -1..1: %s; max rel dev FP32X3 %.2e, FP32 %.2e"
+               % (np.round(np.real(s64), 4), out["sigma_xy"]["fp32x3_max_rel_dev"],
+                  out["sigma_xy"]["fp32_max_rel_dev"]))
+    return out
+
+
 def calibrate():
     """Kernels of known HBM traffic: copy 2 GiB (2 GiB read + 2 GiB written)
     and a sum over 2 GiB (2 GiB read), three times each."""
@@ -109,12 +326,30 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the batched disorder-ensemble measurement")
+    ap.add_argument("--no-correlator", action="store_true",
+                    help="skip the Kubo-Bastin Gamma_nm measurement")
+    ap.add_argument("--only", choices=["correlator", "ensemble", "surrogate"],
+                    help="run only one of the secondary measurements (for profiling)")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the moment-surrogate sweep measurement")
+    ap.add_argument("--sur-epochs", type=int, default=3000)
+    ap.add_argument("--sur-tol", type=float, default=1e-3)
+    ap.add_argument("--sur-points", type=int, default=4096)
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a 2 GiB copy and a 2 GiB read (known traffic) for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.calibrate:
         calibrate()
+        return
+    if args.only:
+        fn = {"correlator": lambda: correlator_bench(512, 1024, 32, 2, log, full=False),
+              "ensemble": lambda: ensemble_bench(256, 512, 16, log),
+              "surrogate": lambda: surrogate_bench(args.sur_epochs, args.sur_tol,
+                                                   args.sur_points, 32, 32, log)}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
         return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -219,6 +454,18 @@ def main():
         log("CPU: %.1f ms per vector-step -> %.4f block steps/s"
             % (1e3 * t_vs, cpu["value"]))
 
+    del rec, plan, dots, m64
+    torch.cuda.empty_cache()
+    correlator = None
+    if not args.no_correlator:
+        correlator = correlator_bench(512, 1024, 32, 2, log)
+    ensemble = None
+    if not args.no_ensemble:
+        ensemble = ensemble_bench(256, 512, 16, log)
+    surrogate = None
+    if not args.no_surrogate:
+        surrogate = surrogate_bench(args.sur_epochs, args.sur_tol, args.sur_points, 32, 32, log)
+
     traffic = read_traffic()
     line = {
         "metric": "KPM Chebyshev steps/s (% of HBM peak); surrogate speedup at 1e-3 DOS error",
@@ -244,6 +491,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_step": bstep},
         "cpu_baseline": cpu,
+        "correlator": correlator,
+        "ensemble": ensemble,
+        "surrogate": surrogate,
     }
     line.update(check)
     print(json.dumps(line), flush=True)

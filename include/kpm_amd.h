@@ -189,6 +189,24 @@ int kpm_moment_shift(const float* nu, int64_t nu_bs, int64_t nu_rs, int32_t batc
                      const double* out_scale, double alpha, const double* beta,
                      float* out, int64_t out_bs, int64_t out_rs, kpm_stream_t stream);
 
+/* ---- cubic 4-orbital disorder ensemble generator --------------------------
+ * Replaces the host assembly kwant would do through hamiltonian_submatrix
+ * (kwant/kpm.py:177-184 -> kwant/_system.pyx:233-338) for the batched
+ * ensemble: a system of side L (4 L^3 orbitals, sites s = (x L + y) L + z,
+ * rows 4 s + o) with random Hermitian on-site blocks A_s = (X + X^H)/2,
+ * X_ij = sigma (g + i g'), g standard normals from the counter-based stream of
+ * `seed`, and hopping t * I_4 to the 6 nearest neighbours (open boundaries).
+ * kpm_cubic4_nnz gives its entry count, 40 L^3 - 24 L^2.  kpm_gen_cubic4
+ * writes the system's rows [row0, row0 + 4L^3) of a block-diagonal CSR whose
+ * entries for it start at nz0 (indices are global: row0 + local column), and
+ * raises *radius_bits (a non-negative float stored as its bits, zeroed by the
+ * caller) to at least the Gershgorin radius max_r sum_j |H_rj| (spectrum in
+ * [-radius, radius]).  Asynchronous on stream. */
+int kpm_cubic4_nnz(int32_t L, int64_t* nnz);
+int kpm_gen_cubic4(int32_t L, uint64_t seed, double sigma, double t, int64_t row0,
+                   int64_t nz0, int32_t* indptr, int32_t* indices, void* values,
+                   int32_t value_type, unsigned int* radius_bits, kpm_stream_t stream);
+
 /* ---- moment surrogate (no counterpart in kwant/kpm.py) ------------------
  * A small MLP that maps Hamiltonian parameters to Jackson-damped moments,
  * trained on moments from kpm_cheb_steps and evaluated in place of the
@@ -239,16 +257,21 @@ int kpm_mlp_mse(const float* y, const float* t, float* dy, int64_t n, float scal
                 double* partial, int32_t n_partial, kpm_stream_t stream);
 /* Ensemble statistics per row p of y[z*z_stride + p*row_stride + n]
  * (members z < members, n < width):
- *   mean[p*mean_stride + n] = average over members       (mean may be NULL)
- *   dev[p] = max over members of (1/width) sum_n |y - mean| (dev may be NULL)
- * With y = densities at Chebyshev nodes, dev is the largest L1 distance of a
- * member's density from the ensemble mean (the surrogate's fallback gate). */
+ *   mean[p*mean_stride + n] = average over members           (mean may be NULL)
+ *   dev[p] = (1/width) sum_n (sample standard deviation over members)
+ *                                                             (dev may be NULL)
+ * With y = densities at Chebyshev nodes, dev is the L1 norm of the ensemble's
+ * pointwise standard deviation: the surrogate's spread (fallback gate). */
 int kpm_mlp_ensemble(const float* y, int64_t z_stride, int64_t row_stride,
                      int32_t members, int64_t rows, int32_t width, float* mean,
                      int64_t mean_stride, float* dev, kpm_stream_t stream);
-/* One Adam update of n parameters at step t >= 1 (bias-corrected). */
+/* One Adam update of n parameters at step t >= 1 (bias-corrected).  With
+ * sched != NULL (device memory: lr, 1/(1-b1^t), 1/(1-b2^t)) lr and t are
+ * ignored and the kernel reads them when it runs, so one captured graph can
+ * replay many steps of a schedule. */
 int kpm_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
-                 float b1, float b2, float eps, int32_t t, kpm_stream_t stream);
+                 float b1, float b2, float eps, int32_t t, const float* sched,
+                 kpm_stream_t stream);
 
 #ifdef __cplusplus
 }

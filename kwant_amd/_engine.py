@@ -36,7 +36,8 @@ EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
     "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
-    "kpm_spmm", "kpm_corr_gemm", "kpm_moment_shift", "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
+    "kpm_spmm", "kpm_corr_gemm", "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
+    "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
     "kpm_mlp_ensemble", "kpm_mlp_adam",
 )
 
@@ -105,11 +106,14 @@ def load_library(path=LIB_PATH):
                                          i32, vp, i64, vp]),
         "kpm_moment_shift": (ctypes.c_int, [vp, i64, i64, i32, i64, i32, vp, vp,
                                             ctypes.c_double, vp, vp, i64, i64, vp]),
+        "kpm_cubic4_nnz": (ctypes.c_int, [i32, ctypes.POINTER(ctypes.c_int64)]),
+        "kpm_gen_cubic4": (ctypes.c_int, [i32, ctypes.c_uint64, ctypes.c_double,
+                                          ctypes.c_double, i64, i64, vp, vp, vp, i32, vp, vp]),
         "kpm_mlp_gemm": (ctypes.c_int, [ctypes.POINTER(MlpGemmDesc), vp]),
         "kpm_mlp_colsum": (ctypes.c_int, [vp, i64, i64, i32, i32, i32, vp, i64, vp]),
         "kpm_mlp_mse": (ctypes.c_int, [vp, vp, vp, i64, f32, vp, i32, vp]),
         "kpm_mlp_ensemble": (ctypes.c_int, [vp, i64, i64, i32, i64, i32, vp, i64, vp, vp]),
-        "kpm_mlp_adam": (ctypes.c_int, [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp]),
+        "kpm_mlp_adam": (ctypes.c_int, [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -146,9 +150,9 @@ def stream_handle():
 
 def complex_dtype(precision):
     import torch
-    if precision in ("c128", "complex128", np.complex128):
+    if precision in ("c128", "complex128", np.complex128, torch.complex128):
         return torch.complex128, KPM_C128
-    if precision in ("c64", "complex64", np.complex64):
+    if precision in ("c64", "complex64", np.complex64, torch.complex64):
         return torch.complex64, KPM_C64
     raise ValueError("precision must be 'complex128' or 'complex64'")
 
@@ -408,6 +412,33 @@ def moment_shift(nu, alpha, beta, out, in_scale=None, out_scale=None):
         out.data_ptr(), out.stride(0), out.stride(1), stream_handle()), "kpm_moment_shift")
 
 
+def cubic4_nnz(L):
+    """Entries of the CSR of one cubic 4-orbital system of side L."""
+    out = ctypes.c_int64()
+    check(load_library().kpm_cubic4_nnz(int(L), ctypes.byref(out)), "kpm_cubic4_nnz")
+    return out.value
+
+
+def gen_cubic4(L, seed, sigma, t, row0, nz0, indptr, indices, values, radius_bits):
+    """Write one cubic 4-orbital system into its diagonal block of a
+    block-diagonal CSR on the device (see kpm_gen_cubic4)."""
+    import torch
+    if indptr.dtype != torch.int32 or indices.dtype != torch.int32:
+        raise ValueError("indptr and indices must be int32")
+    if radius_bits.dtype != torch.int32 or radius_bits.numel() != 1:
+        raise ValueError("radius_bits must be one int32 element")
+    _, vt = complex_dtype(values.dtype)
+    n = 4 * L ** 3
+    if row0 + n + 1 > indptr.numel() or nz0 + cubic4_nnz(L) > indices.numel() \
+            or nz0 + cubic4_nnz(L) > values.numel():
+        raise ValueError("cubic4 block outside the CSR buffers")
+    check(load_library().kpm_gen_cubic4(int(L), int(seed) & (2 ** 64 - 1), float(sigma),
+                                        float(t), int(row0), int(nz0), indptr.data_ptr(),
+                                        indices.data_ptr(), values.data_ptr(), vt,
+                                        radius_bits.data_ptr(), stream_handle()),
+          "kpm_gen_cubic4")
+
+
 def _f32_view(t, name, dims=3):
     import torch
     if t.dtype != torch.float32 or t.dim() != dims:
@@ -488,11 +519,16 @@ def mlp_ensemble(y, mean=None, dev=None):
         None if dev is None else dev.data_ptr(), stream_handle()), "kpm_mlp_ensemble")
 
 
-def mlp_adam(p, g, m, v, lr, b1, b2, eps, t):
-    """One bias-corrected Adam update of the flat parameter buffer p."""
+def mlp_adam(p, g, m, v, lr, b1, b2, eps, t, sched=None):
+    """One bias-corrected Adam update of the flat parameter buffer p; with
+    ``sched`` (a float32 device tensor holding lr, 1/(1-b1^t), 1/(1-b2^t))
+    the kernel reads those when it runs and ignores lr and t."""
     for x in (p, g, m, v):
         if not x.is_contiguous() or x.numel() != p.numel():
             raise ValueError("mlp_adam needs contiguous buffers of one size")
+    if sched is not None and (sched.numel() < 3 or not sched.is_contiguous()):
+        raise ValueError("sched needs three contiguous floats")
     check(load_library().kpm_mlp_adam(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(),
                                       p.numel(), float(lr), float(b1), float(b2), float(eps),
-                                      int(t), stream_handle()), "kpm_mlp_adam")
+                                      int(t), None if sched is None else sched.data_ptr(),
+                                      stream_handle()), "kpm_mlp_adam")

@@ -216,7 +216,8 @@ class Correlator:
         cdt, _ = eng.complex_dtype(dmat.precision)
         n, m_count = dmat.shape[0], self.num_moments
         r0 = self.num_vectors
-        widths = [1] * count if not self.mean else eng.pow2_widths(count, self._max_width)
+        widths = [1] * count if not self.mean else eng.pow2_widths(
+            count, self._width_cap(n, m_count, cdt))
         start = r0
         for width in widths:
             vecs = [factory[r] for r in range(start, start + width)]
@@ -252,6 +253,21 @@ class Correlator:
             self.moments_matrix = self._mu_sum.cpu().numpy() / self.num_vectors
         else:
             self.moments_matrix = np.array(self._mu_list)
+
+    def _width_cap(self, n, m_count, cdt):
+        """Largest power-of-two block width (<= max_block_width) whose two
+        live rings of m_count slots (T_n|r> or Omega, and Psi) take at most
+        half of the free device memory."""
+        import torch
+        free, _ = torch.cuda.mem_get_info()
+        per_width = 2 * m_count * n * torch.empty((), dtype=cdt).element_size()
+        width = self._max_width
+        while width > 1 and width * per_width > free // 2:
+            width //= 2
+        if per_width > free // 2:
+            raise MemoryError("one vector's %d Chebyshev vectors of %d sites do not fit in "
+                              "device memory" % (m_count, n))
+        return width
 
     def _build_integral_factor(self):
         """sum_mn mu~_mn Gamma_mn(x_k) at the 2M nodes (kwant/kpm.py:720-760)

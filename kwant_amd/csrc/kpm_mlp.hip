@@ -26,7 +26,11 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4a __attribute__((ext_vector_type(4)));
 
 constexpr int GB = 64;       // rows / cols per workgroup tile
-constexpr int GK = 32;       // k per stage (one MFMA)
+#ifndef KPM_MLP_GK
+#define KPM_MLP_GK 32
+#endif
+constexpr int GK = KPM_MLP_GK;  // k per stage (a multiple of 32, one MFMA k-step each);
+                                // deeper stages mean fewer load-to-use waits
 constexpr int GLD = GK + 8;  // LDS row stride (bf16), keeps 16-byte alignment
 
 __device__ inline float gelu(float x) {
@@ -39,21 +43,34 @@ __device__ inline float gelu_grad(float x) {
 
 // Stage a GB x GK tile of a strided operand into LDS as bf16 (hi and, for
 // SPLIT, lo parts).  Consecutive threads walk k when k has unit stride and
-// rows otherwise, so global loads coalesce either way.
-template <bool SPLIT>
-__device__ inline void stage(const float* __restrict__ X, int64_t rs, int64_t cs, int r0,
-                             int rows, int k0, int K, __bf16* hi, __bf16* lo, int tid) {
+// rows otherwise, so global loads coalesce either way.  All loads of the tile
+// are issued before the first conversion, so their latencies overlap.
+constexpr int kStageLoads = GB * GK / 256;
+
+__device__ inline void stage_load(const float* __restrict__ X, int64_t rs, int64_t cs, int r0,
+                                  int rows, int k0, int K, int tid, float (&v)[kStageLoads]) {
   const bool kfast = cs == 1;
 #pragma unroll
-  for (int q = 0; q < GB * GK / 256; ++q) {
+  for (int q = 0; q < kStageLoads; ++q) {
     const int e = tid + 256 * q;
     const int r = kfast ? e / GK : e % GB;
     const int c = kfast ? e % GK : e / GB;
-    float v = 0.0f;
-    if (r0 + r < rows && k0 + c < K) v = X[int64_t(r0 + r) * rs + int64_t(k0 + c) * cs];
-    const __bf16 h = (__bf16)v;
+    v[q] = (r0 + r < rows && k0 + c < K) ? X[int64_t(r0 + r) * rs + int64_t(k0 + c) * cs]
+                                         : 0.0f;
+  }
+}
+
+template <bool SPLIT>
+__device__ inline void stage_store(const float (&v)[kStageLoads], bool kfast, __bf16* hi,
+                                   __bf16* lo, int tid) {
+#pragma unroll
+  for (int q = 0; q < kStageLoads; ++q) {
+    const int e = tid + 256 * q;
+    const int r = kfast ? e / GK : e % GB;
+    const int c = kfast ? e % GK : e / GB;
+    const __bf16 h = (__bf16)v[q];
     hi[r * GLD + c] = h;
-    if (SPLIT) lo[r * GLD + c] = (__bf16)(v - (float)h);
+    if (SPLIT) lo[r * GLD + c] = (__bf16)(v[q] - (float)h);
   }
 }
 
@@ -78,43 +95,54 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(kpm_mlp_gemm_desc d, int 
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0f;
 
+  // software pipeline: the global loads of stage k0 + GK are in flight while
+  // stage k0 is multiplied out of LDS
+  float va[kStageLoads], vb[kStageLoads];
+  stage_load(A, d.a_rs, d.a_cs, m0, d.m, 0, d.k, tid, va);
+  stage_load(B, d.b_rs, d.b_cs, n0, d.n, 0, d.k, tid, vb);
   for (int k0 = 0; k0 < d.k; k0 += GK) {
-    stage<SPLIT>(A, d.a_rs, d.a_cs, m0, d.m, k0, d.k, ah, al, tid);
-    stage<SPLIT>(B, d.b_rs, d.b_cs, n0, d.n, k0, d.k, bh, bl, tid);
+    stage_store<SPLIT>(va, d.a_cs == 1, ah, al, tid);
+    stage_store<SPLIT>(vb, d.b_cs == 1, bh, bl, tid);
     __syncthreads();
-    bf8 a[2], b[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int ra = (wm * 32 + i * 16 + (lane & 15)) * GLD + 8 * (lane >> 4);
-      const int rb = (wn * 32 + i * 16 + (lane & 15)) * GLD + 8 * (lane >> 4);
-      a[i] = *reinterpret_cast<const bf8*>(&ah[ra]);
-      b[i] = *reinterpret_cast<const bf8*>(&bh[rb]);
+    if (k0 + GK < d.k) {
+      stage_load(A, d.a_rs, d.a_cs, m0, d.m, k0 + GK, d.k, tid, va);
+      stage_load(B, d.b_rs, d.b_cs, n0, d.n, k0 + GK, d.k, tid, vb);
     }
+    const int steps = (d.k - k0 + 31) / 32 < GK / 32 ? (d.k - k0 + 31) / 32 : GK / 32;
+    for (int kk = 0; kk < steps; ++kk) {
+      const int col = kk * 32 + 8 * (lane >> 4);
+      bf8 a[2], b[2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    if (SPLIT) {
-      bf8 x[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-        x[i] = *reinterpret_cast<const bf8*>(
-            &bl[(wn * 32 + i * 16 + (lane & 15)) * GLD + 8 * (lane >> 4)]);
+      for (int i = 0; i < 2; ++i) {
+        a[i] = *reinterpret_cast<const bf8*>(&ah[(wm * 32 + i * 16 + (lane & 15)) * GLD + col]);
+        b[i] = *reinterpret_cast<const bf8*>(&bh[(wn * 32 + i * 16 + (lane & 15)) * GLD + col]);
+      }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], x[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      if (SPLIT) {
+        bf8 x[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-        x[i] = *reinterpret_cast<const bf8*>(
-            &al[(wm * 32 + i * 16 + (lane & 15)) * GLD + 8 * (lane >> 4)]);
+        for (int i = 0; i < 2; ++i)
+          x[i] = *reinterpret_cast<const bf8*>(
+              &bl[(wn * 32 + i * 16 + (lane & 15)) * GLD + col]);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[i], b[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], x[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          x[i] = *reinterpret_cast<const bf8*>(
+              &al[(wm * 32 + i * 16 + (lane & 15)) * GLD + col]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[i], b[j], acc[i][j], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
@@ -144,15 +172,22 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(kpm_mlp_gemm_desc d, int 
       }
 }
 
-// out[z][n] = sum_m A[z*bs + m*lda + n], m in order (bias gradients).
-__global__ void colsum_kernel(const float* __restrict__ A, int64_t lda, int64_t bs, int M,
-                              int N, float* __restrict__ out, int64_t out_bs) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// out[z][n] = sum_m A[z*bs + m*lda + n] (bias gradients): 64 columns per
+// workgroup, 4 row groups each summing rows g, g+4, ... in order, then the 4
+// partials added in group order (deterministic).
+__global__ __launch_bounds__(256) void colsum_kernel(const float* __restrict__ A, int64_t lda,
+                                                     int64_t bs, int M, int N,
+                                                     float* __restrict__ out, int64_t out_bs) {
+  __shared__ float part[4][64];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + c;
   const float* a = A + blockIdx.y * bs;
   float s = 0.0f;
-  for (int m = 0; m < M; ++m) s += a[int64_t(m) * lda + n];
-  out[blockIdx.y * out_bs + n] = s;
+  if (n < N)
+    for (int m = g; m < M; m += 4) s += a[int64_t(m) * lda + n];
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && n < N) out[blockIdx.y * out_bs + n] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
 }
 
 // dy = scale * (y - t); per-workgroup partial sums of (y - t)^2 in FP64.
@@ -177,9 +212,18 @@ __global__ void mse_kernel(const float* __restrict__ y, const float* __restrict_
 }
 
 // Adam (Kingma & Ba 2015) with bias corrections bc1 = 1/(1-b1^t), bc2 = 1/(1-b2^t).
+// With sched != NULL the step reads (lr, bc1, bc2) from device memory, so a
+// captured graph of many steps follows a schedule the host writes between
+// replays.
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
                             float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
-                            float b1, float b2, float eps, float bc1, float bc2) {
+                            float b1, float b2, float eps, float bc1, float bc2,
+                            const float* __restrict__ sched) {
+  if (sched) {
+    lr = sched[0];
+    bc1 = sched[1];
+    bc2 = sched[2];
+  }
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     const float gi = g[i];
@@ -192,8 +236,10 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
 }
 
 // Per row p of an ensemble y[z][p][:] (members z = 0 .. Z-1, `width` values):
-//   mean[p][n] = (1/Z) sum_z y[z][p][n]                       (if mean != NULL)
-//   dev[p]     = max_z (1/width) sum_n |y[z][p][n] - mean_n|  (if dev != NULL)
+//   mean[p][n] = (1/Z) sum_z y[z][p][n]                             (if mean)
+//   dev[p]     = (1/width) sum_n sqrt(sum_z (y[z][p][n] - mean_n)^2 / (Z-1))
+//                                                                   (if dev)
+// i.e. the L1 norm of the pointwise sample standard deviation (0 for Z = 1).
 // One wavefront per row; members summed in order, lanes reduced by a fixed
 // butterfly, so the result is deterministic.
 __global__ __launch_bounds__(256) void ensemble_kernel(const float* __restrict__ y, int64_t z_stride,
@@ -206,29 +252,25 @@ __global__ __launch_bounds__(256) void ensemble_kernel(const float* __restrict__
   const int64_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= rows) return;
   const float* yp = y + p * row_stride;
-  float worst = 0.0f;
-  for (int z = -1; z < members; ++z) {
-    // z = -1: the mean pass; z >= 0: the deviation of member z
-    float s = 0.0f;
-    for (int n = lane; n < width; n += 64) {
-      float mu = 0.0f;
-      for (int q = 0; q < members; ++q) mu += yp[q * z_stride + n];
-      mu *= 1.0f / float(members);
-      if (z < 0) {
-        if (mean) mean[p * mean_stride + n] = mu;
-      } else {
-        s += fabsf(yp[z * z_stride + n] - mu);
+  float s = 0.0f;
+  for (int n = lane; n < width; n += 64) {
+    float mu = 0.0f;
+    for (int q = 0; q < members; ++q) mu += yp[q * z_stride + n];
+    mu *= 1.0f / float(members);
+    if (mean) mean[p * mean_stride + n] = mu;
+    if (dev && members > 1) {
+      float var = 0.0f;
+      for (int q = 0; q < members; ++q) {
+        const float d = yp[q * z_stride + n] - mu;
+        var += d * d;
       }
+      s += sqrtf(var / float(members - 1));
     }
-    if (z < 0) {
-      if (!dev) break;
-      continue;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    worst = fmaxf(worst, s / float(width));
   }
-  if (dev && lane == 0) dev[p] = worst;
+  if (!dev) return;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) dev[p] = s / float(width);
 }
 
 template <int EPI>
@@ -285,7 +327,7 @@ int kpm_mlp_colsum(const float* A, int64_t lda, int64_t a_bs, int32_t M, int32_t
                    int32_t batch, float* out, int64_t out_bs, kpm_stream_t stream) {
   if (!A || !out || M <= 0 || N <= 0 || batch <= 0 || batch > 65535 || lda < N)
     return fail(KPM_ERR_ARG, "bad colsum arguments");
-  hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256, batch), dim3(256), 0,
+  hipLaunchKernelGGL(colsum_kernel, dim3((N + 63) / 64, batch), dim3(256), 0,
                      (hipStream_t)stream, A, lda, a_bs, M, N, out, out_bs);
   return check_hip(hipGetLastError(), "colsum launch");
 }
@@ -313,15 +355,17 @@ int kpm_mlp_ensemble(const float* y, int64_t z_stride, int64_t row_stride, int32
 }
 
 int kpm_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
-                 float b2, float eps, int32_t t, kpm_stream_t stream) {
-  if (!p || !g || !m || !v || n <= 0 || t <= 0) return fail(KPM_ERR_ARG, "bad adam arguments");
+                 float b2, float eps, int32_t t, const float* sched, kpm_stream_t stream) {
+  if (!p || !g || !m || !v || n <= 0 || (t <= 0 && !sched))
+    return fail(KPM_ERR_ARG, "bad adam arguments");
+  if (sched) t = 1;
   if (!(b1 >= 0.0f && b1 < 1.0f && b2 >= 0.0f && b2 < 1.0f && lr >= 0.0f && eps > 0.0f))
     return fail(KPM_ERR_ARG, "bad adam hyper-parameters");
   const float bc1 = float(1.0 / (1.0 - std::pow(double(b1), double(t))));
   const float bc2 = float(1.0 / (1.0 - std::pow(double(b2), double(t))));
   const int64_t blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
   hipLaunchKernelGGL(adam_kernel, dim3(unsigned(blocks)), dim3(256), 0, (hipStream_t)stream, p,
-                     g, m, v, n, lr, b1, b2, eps, bc1, bc2);
+                     g, m, v, n, lr, b1, b2, eps, bc1, bc2, sched);
   return check_hip(hipGetLastError(), "adam launch");
 }
 

@@ -1,0 +1,133 @@
+"""Batched disorder ensembles: many systems of uneven size advanced by one
+kernel launch per Chebyshev step.
+
+The systems are the diagonal blocks of one CSR matrix, generated on the GPU
+(``kpm_gen_cubic4``: no host assembly or upload) and run with one recursion
+plan whose segments are the systems.  Each segment has its own rescaling
+(a, b) from its own Gershgorin radius and its own FP64 dot products, so one
+launch per step produces every system's moments, which kwant would compute
+with one ``SpectralDensity`` per system (kwant/kpm.py:163-252, the recursion
+of kwant/kpm.py:436-477 looping over systems and vectors).  The whole step
+sequence of an expansion is captured once as a HIP graph and replayed.
+"""
+import time
+
+import numpy as np
+
+from . import _engine as eng
+from .kpm import _scale_from_bounds, jackson_kernel
+
+
+def ensemble_sizes(count, lo=20, hi=60, seed=0):
+    """Side lengths L ~ U{lo..hi} of ``count`` systems (numpy MT19937)."""
+    return np.random.RandomState(seed).randint(lo, hi + 1, size=count)
+
+
+class CubicEnsemble:
+    """Disorder ensemble of cubic L x L x L lattices with 4 orbitals per site
+    (random Hermitian on-site blocks, entries sigma N(0, 1) + i sigma N(0, 1)
+    symmetrised, hopping t I_4 to the 6 neighbours), one system per side
+    length in ``sizes``, ``num_vectors`` random-phase vectors each."""
+
+    def __init__(self, sizes, seed=0, sigma=0.3, t=1.0, num_moments=512, num_vectors=16,
+                 precision="complex64", eps=0.05, variant=-1, vector_seed=1):
+        import torch
+        if num_moments < 2 or num_moments % 2:
+            raise ValueError("num_moments must be even and >= 2")
+        dev = eng.torch_device()
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        if self.sizes.ndim != 1 or len(self.sizes) == 0 or self.sizes.min() < 1:
+            raise ValueError("sizes must be a non-empty list of positive side lengths")
+        self.num_moments, self.num_vectors = num_moments, num_vectors
+        self.rows = 4 * self.sizes ** 3
+        self.nnzs = np.array([eng.cubic4_nnz(int(L)) for L in self.sizes], dtype=np.int64)
+        self.row_off = np.concatenate([[0], np.cumsum(self.rows)])
+        self.nz_off = np.concatenate([[0], np.cumsum(self.nnzs)])
+        n, nnz = int(self.row_off[-1]), int(self.nz_off[-1])
+        if n >= 2 ** 31 or nnz >= 2 ** 31:
+            raise ValueError("ensemble too large for int32 CSR indices")
+        self.n, self.nnz = n, nnz
+        self.cdt, vt = eng.complex_dtype(precision)
+        self.indptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        self.indices = torch.empty(nnz, dtype=torch.int32, device=dev)
+        self.values = torch.empty(nnz, dtype=self.cdt, device=dev)
+        radius = torch.zeros(len(self.sizes), dtype=torch.int32, device=dev)
+        self.seeds = [(int(seed) << 20) + s for s in range(len(self.sizes))]
+        for s, L in enumerate(self.sizes):
+            eng.gen_cubic4(int(L), self.seeds[s], sigma, t, int(self.row_off[s]),
+                           int(self.nz_off[s]), self.indptr, self.indices, self.values,
+                           radius[s:s + 1])
+        self.radius = radius.cpu().numpy().view(np.float32).astype(np.float64)
+        self.scales = np.array([_scale_from_bounds(-r, r, eps) for r in self.radius])
+        self.mat = eng.DeviceCSR.from_device_arrays((n, n), self.indptr, self.indices,
+                                                    self.values, precision)
+        if variant < 0:
+            from . import autotune
+            variant = autotune.pick_variant(self.mat, tuple(self.scales[0]), num_vectors,
+                                            eng.DOTS_DOUBLING)
+        self.variant = variant
+        self.plan = eng.Plan(n, num_vectors, vt, seg_rows=self.row_off.astype(np.int32),
+                             variant=variant)
+        self.plan.set_scale(self.scales.ravel())
+        self.ring = torch.empty((2, n, num_vectors), dtype=self.cdt, device=dev)
+        self.vectors = torch.empty((n, num_vectors), dtype=self.cdt, device=dev)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(vector_seed))
+        chunk = 1 << 22
+        for r0 in range(0, n, chunk):
+            r1 = min(n, r0 + chunk)
+            u = torch.rand((r1 - r0, num_vectors), generator=gen, device=dev,
+                           dtype=torch.float64)
+            self.vectors[r0:r1] = torch.polar(torch.ones_like(u), 2 * np.pi * u)
+        self.steps = num_moments // 2
+        self.dots = torch.empty((self.steps, len(self.sizes), num_vectors, 4),
+                                dtype=torch.float64, device=dev)
+        self._graph = None
+
+    def bytes_per_step(self):
+        """Algorithmic HBM bytes of one step: a_n, a_{n-1} read, a_{n+1}
+        written, the CSR read once."""
+        s_v = 8 if self.cdt.itemsize == 8 else 16
+        return 3 * self.num_vectors * self.n * s_v + self.nnz * (s_v + 4) + 4 * (self.n + 1)
+
+    def _launch(self):
+        self.plan.set_step(0)
+        self.plan.steps(self.mat, self.ring, self.n * self.num_vectors, 2, self.steps,
+                        eng.DOTS_DOUBLING, dots=self.dots)
+
+    def run(self, graph=True):
+        """One expansion of every system; returns the seconds from the first
+        launch to completion (the moments stay on the device, see moments()).
+        With ``graph`` the step sequence is captured on the first call and
+        replayed afterwards."""
+        import torch
+        self.ring[0].copy_(self.vectors)
+        if graph and self._graph is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._launch()
+            self._graph = g
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if graph:
+            self._graph.replay()
+        else:
+            self._launch()
+        torch.cuda.synchronize()
+        self.plan.set_step(self.steps)
+        return time.perf_counter() - t0
+
+    def moments(self, damped=True):
+        """Per-system moments mu_n / mu_0 averaged over the vectors, (S, M);
+        Jackson-damped (kwant/kpm.py:1106-1123) unless damped=False."""
+        d = self.dots.mean(dim=2).cpu().numpy()   # (steps, S, 4)
+        m = np.empty((d.shape[1], self.num_moments))
+        mu0 = d[0, :, 0]
+        m[:, 0] = mu0
+        m[:, 1] = d[0, :, 2]
+        m[:, 2::2] = (2 * d[1:, :, 0] - mu0[None, :]).T
+        m[:, 3::2] = (2 * d[1:, :, 2] - d[0:1, :, 2]).T
+        m /= mu0[:, None]
+        if damped:
+            m = jackson_kernel(m.T).T
+        return m

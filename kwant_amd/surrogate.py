@@ -25,9 +25,9 @@ exact shift layer after it.
   backward and Adam update run on the kernels of libkpm_amd (BF16 matrix
   cores, FP32 accumulation); all members train in the same launches.
 * ``MomentSurrogate`` trains the ensemble on tight-window moments of
-  H(W, phi, 0) and answers sweeps over (W, phi, mu): where the members'
-  densities differ from their mean by more than ``tol`` in L1 (the spread
-  gate) the point is recomputed with plain KPM.
+  H(W, phi, 0) and answers sweeps over (W, phi, mu): where the L1 norm of
+  the members' pointwise density standard deviation exceeds ``tol`` (the
+  spread gate) the point is recomputed with plain KPM.
 
 DOS L1 distance of two damped moment vectors d = g (mu - mu'):
     L1 = int |rho - rho'| dx = (1/K) sum_k |d_0 + 2 sum_n d_n cos(n theta_k)|,
@@ -306,9 +306,12 @@ class MomentMLP:
         eng.mlp_mse(y, target, ws["dy"], scale, ws["partial"])
         return ws["dy"], ws["partial"]
 
-    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, sched=None):
+        """One Adam step; with ``sched`` (device lr, bias corrections) the
+        step reads its schedule from device memory (graph replay)."""
         self.t += 1
-        eng.mlp_adam(self.params, self.grads, self.adam_m, self.adam_v, lr, b1, b2, eps, self.t)
+        eng.mlp_adam(self.params, self.grads, self.adam_m, self.adam_v, lr, b1, b2, eps, self.t,
+                     sched=sched)
 
 
 class MomentSurrogate:
@@ -356,41 +359,78 @@ class MomentSurrogate:
         return self.train_family.damped_moments(p, fresh_seed=fresh_seed)
 
     def fit(self, params, targets, epochs=400, batch=64, lr=3e-3, lr_min=3e-6, seed=0,
-            log=None):
+            log=None, graph=True, b1=0.9, b2=0.999):
         """Train every member on (params, targets) with Adam and a cosine
-        learning-rate schedule; members draw their own mini-batch order."""
+        learning-rate schedule; members draw their own mini-batch order.
+
+        An epoch is a fixed sequence of launches over fixed buffers (the
+        epoch's shuffled copy of the data, one schedule row per iteration), so
+        with ``graph`` it is captured once as a HIP graph and replayed every
+        epoch; the host only reshuffles and writes the schedule between
+        replays.  graph=False runs the same launches eagerly (bitwise the same
+        result)."""
         import torch
         dev = self.mlp.params.device
         x = self._inputs(params)
         t = torch.from_numpy(np.asarray(targets, dtype=np.float32)).to(dev)
         count, z = len(x), self.mlp.members
         per_epoch = max(1, count // batch)
+        used = per_epoch * batch
         total = epochs * per_epoch
+        # iteration s reads xs[s], ts[s]: contiguous (members, batch, features)
+        xs = torch.empty((per_epoch, z, batch, x.shape[1]), dtype=torch.float32, device=dev)
+        ts = torch.empty((per_epoch, z, batch, t.shape[1]), dtype=torch.float32, device=dev)
+        sched = torch.empty((per_epoch, 4), dtype=torch.float32, device=dev)
         gen = torch.Generator(device=dev)
         gen.manual_seed(seed)
+        loss = torch.zeros((), dtype=torch.float64, device=dev)
+
+        def epoch_body():
+            for s in range(per_epoch):
+                y = self.mlp.forward(xs[s])
+                dy, partial = self.mlp.loss_grad(y, ts[s])
+                self.mlp.backward(dy)
+                self.mlp.adam(0.0, b1, b2, sched=sched[s])
+            return partial
+
+        g = None
+        if graph:
+            # capture reads only the buffers above; nothing is allocated or cleared inside
+            self.mlp._workspace(batch)
+            g = torch.cuda.CUDAGraph()
+            t_saved = self.mlp.t
+            with torch.cuda.graph(g):
+                last_partial = epoch_body()
+            self.mlp.t = t_saved
         it = 0
         for ep in range(epochs):
             order = torch.argsort(torch.rand((z, count), generator=gen, device=dev), dim=1)
-            ep_loss = torch.zeros((), dtype=torch.float64, device=dev)
+            order = order[:, :used].reshape(z, per_epoch, batch).transpose(0, 1)
+            xs.copy_(x[order])
+            ts.copy_(t[order])
+            rows = np.empty((per_epoch, 4), dtype=np.float32)
             for s in range(per_epoch):
-                idx = order[:, s * batch:(s + 1) * batch]
-                y = self.mlp.forward(x[idx])
-                dy, partial = self.mlp.loss_grad(y, t[idx])
-                ep_loss += partial.sum()
-                self.mlp.backward(dy)
-                frac = it / max(1, total - 1)
-                self.mlp.adam(lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * frac)))
-                it += 1
+                frac = (it + s) / max(1, total - 1)
+                step = self.mlp.t + s + 1
+                rows[s] = (lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * frac)),
+                           1.0 / (1.0 - b1 ** step), 1.0 / (1.0 - b2 ** step), 0.0)
+            sched.copy_(torch.from_numpy(rows))
+            if g is not None:
+                g.replay()
+                self.mlp.t += per_epoch
+            else:
+                last_partial = epoch_body()
+            it += per_epoch
             if log is not None and (ep % max(1, epochs // 10) == 0 or ep == epochs - 1):
-                mse = float(ep_loss) / (per_epoch * batch * z * t.shape[1])
+                mse = float(last_partial.sum()) / (batch * z * t.shape[1])
                 self.history.append((ep, mse))
-                log("epoch %d: mse %.3e" % (ep, mse))
+                log("epoch %d: mse %.3e (last mini-batch)" % (ep, mse))
 
     def predict(self, params, chunk=4096, members=False):
         """Ensemble mean of the damped wide-window moments (P, M) and the
-        spread (P,) on the device: the spread is the largest DOS L1 distance
-        of a member from the mean.  members=True also returns the per-member
-        moments (Z, P, M)."""
+        spread (P,) on the device: the spread is the L1 norm (over energy)
+        of the pointwise standard deviation of the members' densities.
+        members=True also returns the per-member moments (Z, P, M)."""
         import torch
         p = np.asarray(params, dtype=np.float64).reshape(-1, 3)
         if np.any(np.abs(p[:, 2]) > self.max_shift + 1e-12):

@@ -121,7 +121,7 @@ def test_colsum_mse_ensemble_adam():
     eng.mlp_ensemble(_t(ens), mean=mean, dev=dev)
     m_ref = ens.astype(np.float64).mean(axis=0)
     _close(mean.cpu().numpy(), m_ref, 1e-6)
-    _close(dev.cpu().numpy(), np.abs(ens - m_ref).mean(axis=2).max(axis=0), 1e-5)
+    _close(dev.cpu().numpy(), ens.astype(np.float64).std(axis=0, ddof=1).mean(axis=1), 1e-5)
 
     p = rng.standard_normal(1000).astype(np.float32)
     pm, pv = np.zeros_like(p), np.zeros_like(p)
@@ -295,3 +295,28 @@ def test_surrogate_gain_over_plain_kpm():
           % (t_kpm, t_sur, 100 * fall.mean()))
     assert mom.shape == (1024, m)
     assert t_sur < t_kpm
+
+
+def test_fit_graph_replay_matches_eager():
+    """Training epochs replayed from one captured HIP graph give bitwise the
+    parameters of the same launches run eagerly (several replays, with the
+    host's reshuffle and schedule upload between them)."""
+    from kwant_amd.surrogate import MomentSurrogate
+
+    class Fam:
+        num_moments = 48
+        a, b = 7.2, 0.0
+        g = np.linspace(1.0, 0.05, 48)
+
+    rng = np.random.RandomState(2)
+    params = rng.uniform([0, 0, -1], [4, 0.02, 1], (96, 3))
+    targets = rng.standard_normal((96, 48)) * 0.1
+    out = []
+    for graph in (True, False):
+        tight = Fam()
+        tight.a = 6.1
+        sur = MomentSurrogate(Fam(), tight, members=3, layers=(2, 32, 32, 48), seed=3)
+        sur.fit(params, targets, epochs=4, batch=32, lr=1e-2, graph=graph)
+        out.append(sur.mlp.params.cpu().numpy())
+        assert sur.mlp.t == 4 * 3
+    assert np.array_equal(out[0], out[1])
