@@ -173,6 +173,22 @@ int kpm_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb,
                   int64_t k, int32_t precision, int32_t k_chunk,
                   double* work, int64_t work_elems, kpm_stream_t stream);
 
+/* The same product, mu[m, n] += sum_k conj(A[m, k]) B[n, k], from
+ * complex128 A and B on the BF16 matrix cores, exact in every term kept:
+ * per panel of k and per operand row, each real component is cut into
+ * `slices` (1..4) integer slices of <= 8 bits against the row's largest
+ * exponent; slice products and their level sums over 256 k' are exact in the
+ * FP32 accumulator and are folded into FP64.  The error is the slice tail and
+ * the dropped cross levels: about 2^(-8 slices) of max|A_m| max|B_n| per term
+ * (slices = 4: FP64-class accuracy for well-scaled rows).  work: device
+ * buffer of at least kpm_corr_gemm_sliced_work() bytes.  Asynchronous. */
+int kpm_corr_gemm_sliced_work(int32_t m_rows, int32_t n_rows, int64_t k, int32_t slices,
+                              int64_t* bytes);
+int kpm_corr_gemm_sliced(const void* A, int64_t lda, const void* B, int64_t ldb,
+                         double* mu, int64_t ldc, int32_t m_rows, int32_t n_rows,
+                         int64_t k, int32_t slices, void* work, int64_t work_bytes,
+                         kpm_stream_t stream);
+
 /* Exact moments after a uniform on-site shift (no counterpart in kwant/kpm.py;
  * it acts on the moments kwant/kpm.py:436-477 produces).  nu holds moments
  * <T_k(t)> in a window t = (E - b') / a'; the moments in the window
@@ -266,12 +282,13 @@ int kpm_mlp_ensemble(const float* y, int64_t z_stride, int64_t row_stride,
                      int32_t members, int64_t rows, int32_t width, float* mean,
                      int64_t mean_stride, float* dev, kpm_stream_t stream);
 /* One Adam update of n parameters at step t >= 1 (bias-corrected).  With
- * sched != NULL (device memory: lr, 1/(1-b1^t), 1/(1-b2^t)) lr and t are
- * ignored and the kernel reads them when it runs, so one captured graph can
- * replay many steps of a schedule. */
+ * sched != NULL (device memory: lr, 1/(1-b1^t), 1/(1-b2^t), ema decay) lr, t
+ * and ema_decay are ignored and the kernel reads them when it runs, so one
+ * captured graph can replay many steps of a schedule.  With ema != NULL the
+ * updated parameters are folded into ema = d ema + (1 - d) p. */
 int kpm_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
                  float b1, float b2, float eps, int32_t t, const float* sched,
-                 kpm_stream_t stream);
+                 float* ema, float ema_decay, kpm_stream_t stream);
 
 #ifdef __cplusplus
 }

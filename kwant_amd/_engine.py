@@ -36,7 +36,8 @@ EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
     "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
-    "kpm_spmm", "kpm_corr_gemm", "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
+    "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
+    "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
     "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
     "kpm_mlp_ensemble", "kpm_mlp_adam",
 )
@@ -104,6 +105,10 @@ def load_library(path=LIB_PATH):
         "kpm_spmm": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, i64, vp, i64, i32, i32, vp]),
         "kpm_corr_gemm": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
                                          i32, vp, i64, vp]),
+        "kpm_corr_gemm_sliced_work": (ctypes.c_int, [i32, i32, i64, i32,
+                                                     ctypes.POINTER(ctypes.c_int64)]),
+        "kpm_corr_gemm_sliced": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
+                                                vp, i64, vp]),
         "kpm_moment_shift": (ctypes.c_int, [vp, i64, i64, i32, i64, i32, vp, vp,
                                             ctypes.c_double, vp, vp, i64, i64, vp]),
         "kpm_cubic4_nnz": (ctypes.c_int, [i32, ctypes.POINTER(ctypes.c_int64)]),
@@ -113,7 +118,8 @@ def load_library(path=LIB_PATH):
         "kpm_mlp_colsum": (ctypes.c_int, [vp, i64, i64, i32, i32, i32, vp, i64, vp]),
         "kpm_mlp_mse": (ctypes.c_int, [vp, vp, vp, i64, f32, vp, i32, vp]),
         "kpm_mlp_ensemble": (ctypes.c_int, [vp, i64, i64, i32, i64, i32, vp, i64, vp, vp]),
-        "kpm_mlp_adam": (ctypes.c_int, [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp, vp]),
+        "kpm_mlp_adam": (ctypes.c_int, [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp, vp,
+                                        f32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -389,6 +395,31 @@ def corr_gemm(a, b, mu, precision, k_chunk=256, work=None):
         "kpm_corr_gemm")
 
 
+def corr_gemm_sliced(a, b, mu, slices=4, work=None):
+    """mu += conj(a) @ b.T from complex128 a (m x k) and b (n x k) on the
+    BF16 matrix cores with ``slices`` exact 8-bit slices per component."""
+    import torch
+    m, k = a.shape
+    n = b.shape[0]
+    if b.shape[1] != k or tuple(mu.shape) != (m, n):
+        raise ValueError("corr_gemm_sliced shape mismatch")
+    if a.dtype != torch.complex128 or b.dtype != torch.complex128 or mu.dtype != torch.complex128:
+        raise ValueError("corr_gemm_sliced takes complex128 operands and output")
+    if a.stride(1) != 1 or b.stride(1) != 1 or mu.stride(1) != 1:
+        raise ValueError("operands must be contiguous along k")
+    lib = load_library()
+    need = ctypes.c_int64()
+    check(lib.kpm_corr_gemm_sliced_work(m, n, k, int(slices), ctypes.byref(need)),
+          "kpm_corr_gemm_sliced_work")
+    if work is None or work.numel() * work.element_size() < need.value:
+        work = torch.empty(need.value, dtype=torch.uint8, device=mu.device)
+    check(lib.kpm_corr_gemm_sliced(a.data_ptr(), a.stride(0), b.data_ptr(), b.stride(0),
+                                   mu.data_ptr(), mu.stride(0), m, n, k, int(slices),
+                                   work.data_ptr(), work.numel() * work.element_size(),
+                                   stream_handle()), "kpm_corr_gemm_sliced")
+    return work
+
+
 def moment_shift(nu, alpha, beta, out, in_scale=None, out_scale=None):
     """Moments of H + mu from those of H (see kpm_moment_shift): nu, out:
     (Z, P, M) float32 with unit stride along M; beta: (P,) float64 device
@@ -519,16 +550,18 @@ def mlp_ensemble(y, mean=None, dev=None):
         None if dev is None else dev.data_ptr(), stream_handle()), "kpm_mlp_ensemble")
 
 
-def mlp_adam(p, g, m, v, lr, b1, b2, eps, t, sched=None):
+def mlp_adam(p, g, m, v, lr, b1, b2, eps, t, sched=None, ema=None, ema_decay=0.0):
     """One bias-corrected Adam update of the flat parameter buffer p; with
-    ``sched`` (a float32 device tensor holding lr, 1/(1-b1^t), 1/(1-b2^t))
-    the kernel reads those when it runs and ignores lr and t."""
-    for x in (p, g, m, v):
+    ``sched`` (a float32 device tensor holding lr, 1/(1-b1^t), 1/(1-b2^t),
+    ema decay) the kernel reads those when it runs and ignores lr, t and
+    ema_decay.  With ``ema`` the new parameters are folded into it."""
+    for x in (p, g, m, v) + (() if ema is None else (ema,)):
         if not x.is_contiguous() or x.numel() != p.numel():
             raise ValueError("mlp_adam needs contiguous buffers of one size")
-    if sched is not None and (sched.numel() < 3 or not sched.is_contiguous()):
-        raise ValueError("sched needs three contiguous floats")
+    if sched is not None and (sched.numel() < 4 or not sched.is_contiguous()):
+        raise ValueError("sched needs four contiguous floats")
     check(load_library().kpm_mlp_adam(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(),
                                       p.numel(), float(lr), float(b1), float(b2), float(eps),
                                       int(t), None if sched is None else sched.data_ptr(),
-                                      stream_handle()), "kpm_mlp_adam")
+                                      None if ema is None else ema.data_ptr(),
+                                      float(ema_decay), stream_handle()), "kpm_mlp_adam")

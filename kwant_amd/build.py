@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkpm_amd.so")
 SOURCES = ["kpm_capi.hip", "kpm_cheb.hip", "kpm_gemm.hip", "kpm_mlp.hip", "kpm_shift.hip",
-           "kpm_lattice.hip"]
+           "kpm_lattice.hip", "kpm_sliced.hip"]
 ARCH = os.environ.get("KPM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

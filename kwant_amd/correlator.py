@@ -29,7 +29,8 @@ from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _is_l
 from collections.abc import Iterable
 import math
 
-_GEMM = {'fp64': eng.GEMM_FP64, 'fp32': eng.GEMM_FP32, 'fp32x3': eng.GEMM_FP32X3}
+_GEMM = {'fp64': eng.GEMM_FP64, 'fp32': eng.GEMM_FP32, 'fp32x3': eng.GEMM_FP32X3,
+         'sliced3': 'sliced3', 'sliced4': 'sliced4'}
 
 
 def _normalize_operator(op, params):
@@ -109,7 +110,10 @@ class Correlator:
         if gp is None:
             gp = 'fp64' if precision in ('complex128', 'c128') else 'fp32'
         if gp not in _GEMM:
-            raise ValueError("gemm_precision must be 'fp64', 'fp32' or 'fp32x3'")
+            raise ValueError("gemm_precision must be 'fp64', 'fp32', 'fp32x3', 'sliced3' "
+                             "or 'sliced4'")
+        if gp.startswith('sliced') and precision not in ('complex128', 'c128'):
+            raise ValueError("the sliced GEMM takes complex128 expansions")
         self._gemm_precision = gp
         self._max_width = kwargs.pop('max_block_width', 16)
         kwargs.pop('variant', None)
@@ -241,8 +245,13 @@ class Correlator:
                 rec_l.step(m_count - 1, want_dots=False)
             mu = torch.zeros((m_count, m_count), dtype=torch.complex128, device=dev)
             k = n * width
-            eng.corr_gemm(rec_l.ring.reshape(m_count, k), psi.reshape(m_count, k), mu,
-                          _GEMM[self._gemm_precision])
+            gp = _GEMM[self._gemm_precision]
+            if isinstance(gp, str):
+                self._gemm_work = eng.corr_gemm_sliced(
+                    rec_l.ring.reshape(m_count, k), psi.reshape(m_count, k), mu,
+                    slices=int(gp[-1]), work=getattr(self, '_gemm_work', None))
+            else:
+                eng.corr_gemm(rec_l.ring.reshape(m_count, k), psi.reshape(m_count, k), mu, gp)
             del rec_l, psi
             if self.mean:
                 self._mu_sum = mu if self._mu_sum is None else self._mu_sum + mu

@@ -212,17 +212,21 @@ __global__ void mse_kernel(const float* __restrict__ y, const float* __restrict_
 }
 
 // Adam (Kingma & Ba 2015) with bias corrections bc1 = 1/(1-b1^t), bc2 = 1/(1-b2^t).
-// With sched != NULL the step reads (lr, bc1, bc2) from device memory, so a
-// captured graph of many steps follows a schedule the host writes between
-// replays.
+// With sched != NULL the step reads (lr, bc1, bc2, ema decay) from device
+// memory, so a captured graph of many steps follows a schedule the host
+// writes between replays.  With ema != NULL the updated parameters are also
+// folded into an exponential moving average, ema = d ema + (1 - d) p
+// (d = 0 copies p).
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
                             float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
                             float b1, float b2, float eps, float bc1, float bc2,
-                            const float* __restrict__ sched) {
+                            const float* __restrict__ sched, float* __restrict__ ema,
+                            float decay) {
   if (sched) {
     lr = sched[0];
     bc1 = sched[1];
     bc2 = sched[2];
+    decay = sched[3];
   }
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
@@ -231,7 +235,9 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    p[i] -= lr * (mi * bc1) / (sqrtf(vi * bc2) + eps);
+    const float pi = p[i] - lr * (mi * bc1) / (sqrtf(vi * bc2) + eps);
+    p[i] = pi;
+    if (ema) ema[i] = decay * ema[i] + (1.0f - decay) * pi;
   }
 }
 
@@ -355,7 +361,8 @@ int kpm_mlp_ensemble(const float* y, int64_t z_stride, int64_t row_stride, int32
 }
 
 int kpm_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
-                 float b2, float eps, int32_t t, const float* sched, kpm_stream_t stream) {
+                 float b2, float eps, int32_t t, const float* sched, float* ema,
+                 float ema_decay, kpm_stream_t stream) {
   if (!p || !g || !m || !v || n <= 0 || (t <= 0 && !sched))
     return fail(KPM_ERR_ARG, "bad adam arguments");
   if (sched) t = 1;
@@ -365,7 +372,7 @@ int kpm_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float 
   const float bc2 = float(1.0 / (1.0 - std::pow(double(b2), double(t))));
   const int64_t blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
   hipLaunchKernelGGL(adam_kernel, dim3(unsigned(blocks)), dim3(256), 0, (hipStream_t)stream, p,
-                     g, m, v, n, lr, b1, b2, eps, bc1, bc2, sched);
+                     g, m, v, n, lr, b1, b2, eps, bc1, bc2, sched, ema, ema_decay);
   return check_hip(hipGetLastError(), "adam launch");
 }
 

@@ -1,0 +1,343 @@
+// Correlator moment matrix mu[m, n] += sum_k conj(A[m, k]) B[n, k] from
+// complex128 operands on the BF16 matrix cores, exact in every term it keeps
+// (a split-integer scheme after Ozaki et al., "Error-free transformations of
+// matrix multiplication", Numer. Algorithms 59 (2012)).
+//
+// The k range is processed in panels.  For each panel and each operand row r
+// the largest component magnitude gives an exponent e_r (max < 2^e_r), and
+// every real component x (re and im interleaved, k' = 2k + {0, 1}) is cut into
+// S integer slices of at most 8 bits:
+//     v = x 2^(7 - e_r);  q_1 = rint(v);  v = (v - q_1) 256;  q_2 = rint(v); ...
+//     x = 2^(e_r - 7) (q_1 + q_2 2^-8 + ... + q_S 2^(8 - 8S)) + O(2^(e_r - 8S - 8))
+// |q_i| <= 128, so each q is exact in BF16 and each product q_i q'_j is an
+// integer of at most 2^14 -- exact in the FP32 accumulator of
+// v_mfma_f32_16x16x32_bf16.  Products are grouped by level l = i + j (0-based
+// slices): a level has at most S pairs, so over 256 real k' its partial sum is
+// an integer of magnitude <= S 256 2^14 <= 2^24 for S <= 4 and stays exact.
+// Every 256 k' the level sums are folded into FP64 as sum_l acc_l 2^(-8l)
+// (exact scaling), and the tile is finally scaled by 2^(e_m + e_n - 14).
+// Levels l >= S are dropped: the result equals the exact product of the
+// S-slice truncations up to those cross terms, about 2^(-8S) of
+// max|A_m| max|B_n| per term.
+//
+// Re mu = sum_k' A'[m, k'] B'[n, k'] with the interleaved real rows, and
+// Im mu = sum_k' A'[m, k'] B''[n, k'] with B'' = (b_im, -b_re), formed from
+// B's slices in registers (swap and negate within a pair: exact).
+
+#include "kpm_internal.h"
+
+namespace kpm {
+
+namespace {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4a __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+
+constexpr int SB = 64;           // output tile (rows and cols)
+constexpr int SK = 64;           // real k' per LDS stage
+constexpr int SLD = SK + 8;      // LDS row stride in bf16 (16-byte aligned rows)
+constexpr int FOLD = 256;        // real k' per exact FP32 interval
+constexpr int kMaxSlices = 4;
+
+// e[r] such that max |x| over the panel's 2 kn components of row r is < 2^e
+// (0 for an all-zero row).  One wavefront per row.
+__global__ __launch_bounds__(256) void row_exponent_kernel(const double* __restrict__ X,
+                                                           int64_t ld, int rows, int64_t k0,
+                                                           int64_t kn, int* __restrict__ e) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= rows) return;
+  const double* x = X + 2 * (int64_t(r) * ld + k0);
+  double m = 0.0;
+  for (int64_t i = lane; i < 2 * kn; i += 64) m = fmax(m, fabs(x[i]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if (lane == 0) {
+    int ex = 0;
+    if (m > 0.0) frexp(m, &ex);
+    e[r] = ex;
+  }
+}
+
+// S slices of every component of the panel, rows padded to kp2 with zeros:
+// out[(s * rows + r) * kp2 + k'].
+__global__ __launch_bounds__(256) void slice_kernel(const double* __restrict__ X, int64_t ld,
+                                                    int rows, int64_t k0, int64_t kn,
+                                                    const int* __restrict__ e, int S,
+                                                    __bf16* __restrict__ out, int64_t kp2) {
+  const int r = blockIdx.y;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= kp2 || r >= rows) return;
+  double v = c < 2 * kn ? X[2 * (int64_t(r) * ld + k0) + c] : 0.0;
+  v = ldexp(v, 7 - e[r]);
+  for (int s = 0; s < S; ++s) {
+    const double q = rint(v);
+    out[(int64_t(s) * rows + r) * kp2 + c] = (__bf16)float(q);
+    v = (v - q) * 256.0;
+  }
+}
+
+__device__ inline bf8 swap_negate(bf8 b) {
+  // (b1, -b0, b3, -b2, ...): the B'' = (im, -re) pairing, exact in bf16
+  s8v u = __builtin_bit_cast(s8v, b);
+  s8v w;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    w[2 * p] = u[2 * p + 1];
+    w[2 * p + 1] = short(u[2 * p] ^ short(0x8000));
+  }
+  return __builtin_bit_cast(bf8, w);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void sliced_gemm_kernel(
+    const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int m_rows, int n_rows,
+    int64_t kp2, int64_t kps, const int* __restrict__ ea, const int* __restrict__ eb,
+    double* __restrict__ out, int64_t ldo, int accumulate) {
+  __shared__ __bf16 la[S][SB * SLD];
+  __shared__ __bf16 lb[S][SB * SLD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * SB, n0 = blockIdx.x * SB;
+  const int64_t kb = int64_t(blockIdx.z) * kps;
+  const int64_t ke = kb + kps < kp2 ? kb + kps : kp2;
+
+  f4a acc_r[S][2][2], acc_i[S][2][2];
+  double dr[2][2][4], di[2][2][4];
+#pragma unroll
+  for (int l = 0; l < S; ++l)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc_r[l][i][j][r] = acc_i[l][i][j][r] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dr[i][j][r] = di[i][j][r] = 0.0;
+
+  // stage S slices of both tiles (64 rows x 64 k' bf16 = 8 x 16 B per row);
+  // the next stage's global loads are in flight during this stage's products
+  bf8 va[S][2], vb[S][2];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int e = tid + 256 * q;  // 512 chunks of 8 bf16 per tile
+        const int row = e >> 3, col = (e & 7) * 8;
+        va[s][q] = bf8{};
+        vb[s][q] = bf8{};
+        if (m0 + row < m_rows)
+          va[s][q] = *reinterpret_cast<const bf8*>(As + (int64_t(s) * m_rows + m0 + row) * kp2 + k0 + col);
+        if (n0 + row < n_rows)
+          vb[s][q] = *reinterpret_cast<const bf8*>(Bs + (int64_t(s) * n_rows + n0 + row) * kp2 + k0 + col);
+      }
+  };
+  if (kb < ke) fetch(kb);
+  int since = 0;
+  for (int64_t k0 = kb; k0 < ke; k0 += SK) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int e = tid + 256 * q;
+        const int row = e >> 3, col = (e & 7) * 8;
+        *reinterpret_cast<bf8*>(&la[s][row * SLD + col]) = va[s][q];
+        *reinterpret_cast<bf8*>(&lb[s][row * SLD + col]) = vb[s][q];
+      }
+    __syncthreads();
+    if (k0 + SK < ke) fetch(k0 + SK);
+#pragma unroll
+    for (int kk = 0; kk < SK; kk += 32) {
+      const int col = kk + 8 * (lane >> 4);
+      // every slice's fragments once per k-step, then all the products
+      bf8 a[S][2], b[S][2], bs[S][2];
+#pragma unroll
+      for (int q = 0; q < S; ++q)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          a[q][t] = *reinterpret_cast<const bf8*>(&la[q][(wm * 32 + t * 16 + (lane & 15)) * SLD + col]);
+          b[q][t] = *reinterpret_cast<const bf8*>(&lb[q][(wn * 32 + t * 16 + (lane & 15)) * SLD + col]);
+        }
+#pragma unroll
+      for (int q = 0; q < S; ++q)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) bs[q][t] = swap_negate(b[q][t]);
+#pragma unroll
+      for (int l = 0; l < S; ++l)
+#pragma unroll
+        for (int sa = 0; sa <= l; ++sa)
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              acc_r[l][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], b[l - sa][j], acc_r[l][i][j], 0, 0, 0);
+              acc_i[l][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], bs[l - sa][j], acc_i[l][i][j], 0, 0, 0);
+            }
+    }
+    __syncthreads();
+    since += SK;
+    if (since >= FOLD || k0 + SK >= ke) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int l = S - 1; l >= 0; --l) {
+              sr += ldexp(double(acc_r[l][i][j][r]), -8 * l);
+              si += ldexp(double(acc_i[l][i][j][r]), -8 * l);
+              acc_r[l][i][j][r] = 0.0f;
+              acc_i[l][i][j][r] = 0.0f;
+            }
+            dr[i][j][r] += sr;
+            di[i][j][r] += si;
+          }
+      since = 0;
+    }
+  }
+
+  double* o = out + int64_t(blockIdx.z) * int64_t(m_rows) * ldo * 2;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (m >= m_rows || n >= n_rows) continue;
+        const int sc = ea[m] + eb[n] - 14;
+        double* p = o + 2 * (int64_t(m) * ldo + n);
+        const double vr = ldexp(dr[i][j][r], sc), vi = ldexp(di[i][j][r], sc);
+        if (accumulate) {
+          p[0] += vr;
+          p[1] += vi;
+        } else {
+          p[0] = vr;
+          p[1] = vi;
+        }
+      }
+}
+
+__global__ void add_partials(const double* __restrict__ work, int slices, double* mu,
+                             int64_t ldc, int m_rows, int n_rows) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(m_rows) * n_rows) return;
+  const int m = int(t / n_rows), n = int(t % n_rows);
+  double sr = 0.0, si = 0.0;
+  for (int z = 0; z < slices; ++z) {
+    const double* p = work + 2 * (int64_t(z) * m_rows * n_rows + t);
+    sr += p[0];
+    si += p[1];
+  }
+  mu[2 * (int64_t(m) * ldc + n)] += sr;
+  mu[2 * (int64_t(m) * ldc + n) + 1] += si;
+}
+
+struct SlicedPlan {
+  int64_t panel;      // complex k per panel
+  int64_t kp2;        // padded real k' per panel row
+  int64_t zs;         // k-splits per panel
+  int64_t kps;        // real k' per split (multiple of SK)
+  int64_t slice_bytes, exp_bytes, part_bytes;
+};
+
+SlicedPlan sliced_plan(int m_rows, int n_rows, int64_t k, int slices) {
+  SlicedPlan p;
+  p.panel = k < (int64_t(1) << 20) ? k : (int64_t(1) << 20);
+  p.kp2 = (2 * p.panel + SK - 1) / SK * SK;
+  const int64_t tiles = int64_t((m_rows + SB - 1) / SB) * ((n_rows + SB - 1) / SB);
+  int64_t zs = (2048 + tiles - 1) / tiles;
+  const int64_t by_k = (p.kp2 + 4 * FOLD - 1) / (4 * FOLD);
+  if (zs > by_k) zs = by_k;
+  if (zs < 1) zs = 1;
+  p.kps = ((p.kp2 + zs - 1) / zs + SK - 1) / SK * SK;
+  p.zs = (p.kp2 + p.kps - 1) / p.kps;
+  p.slice_bytes = int64_t(slices) * (m_rows + n_rows) * p.kp2 * 2;
+  p.exp_bytes = int64_t(m_rows + n_rows) * 4;
+  p.part_bytes = p.zs * int64_t(m_rows) * n_rows * 2 * 8;
+  return p;
+}
+
+template <int S>
+void launch_sliced_gemm(const __bf16* As, const __bf16* Bs, int m_rows, int n_rows,
+                        const SlicedPlan& p, const int* ea, const int* eb, double* part,
+                        int accumulate, hipStream_t s) {
+  dim3 grid((n_rows + SB - 1) / SB, (m_rows + SB - 1) / SB, unsigned(p.zs));
+  hipLaunchKernelGGL(sliced_gemm_kernel<S>, grid, dim3(256), 0, s, As, Bs, m_rows, n_rows, p.kp2,
+                     p.kps, ea, eb, part, int64_t(n_rows), accumulate);
+}
+
+}  // namespace
+
+}  // namespace kpm
+
+using namespace kpm;
+
+extern "C" {
+
+int kpm_corr_gemm_sliced_work(int32_t m_rows, int32_t n_rows, int64_t k, int32_t slices,
+                              int64_t* bytes) {
+  if (!bytes || m_rows <= 0 || n_rows <= 0 || k <= 0 || slices < 1 || slices > kMaxSlices)
+    return fail(KPM_ERR_ARG, "bad sliced GEMM shape");
+  const SlicedPlan p = sliced_plan(m_rows, n_rows, k, slices);
+  *bytes = p.slice_bytes + p.exp_bytes + p.part_bytes + 256;
+  return KPM_OK;
+}
+
+int kpm_corr_gemm_sliced(const void* A, int64_t lda, const void* B, int64_t ldb, double* mu,
+                         int64_t ldc, int32_t m_rows, int32_t n_rows, int64_t k,
+                         int32_t slices, void* work, int64_t work_bytes, kpm_stream_t stream) {
+  if (!A || !B || !mu || !work) return fail(KPM_ERR_ARG, "NULL buffer");
+  if (m_rows <= 0 || n_rows <= 0 || k <= 0) return fail(KPM_ERR_ARG, "empty GEMM");
+  if (slices < 1 || slices > kMaxSlices) return fail(KPM_ERR_ARG, "slices must be 1..4");
+  if (lda < k || ldb < k || ldc < n_rows) return fail(KPM_ERR_ARG, "leading dimension too small");
+  const SlicedPlan p = sliced_plan(m_rows, n_rows, k, slices);
+  if (work_bytes < p.slice_bytes + p.exp_bytes + p.part_bytes + 256)
+    return fail(KPM_ERR_ARG, "work buffer too small (kpm_corr_gemm_sliced_work)");
+  hipStream_t s = (hipStream_t)stream;
+  char* w = static_cast<char*>(work);
+  double* part = reinterpret_cast<double*>(w);
+  int* ea = reinterpret_cast<int*>(w + p.part_bytes);
+  int* eb = ea + m_rows;
+  __bf16* As = reinterpret_cast<__bf16*>(w + ((p.part_bytes + p.exp_bytes + 255) / 256) * 256);
+  __bf16* Bs = As + int64_t(slices) * m_rows * p.kp2;
+  const double* a = static_cast<const double*>(A);
+  const double* b = static_cast<const double*>(B);
+  int panel_no = 0;
+  for (int64_t k0 = 0; k0 < k; k0 += p.panel, ++panel_no) {
+    const int64_t kn = k - k0 < p.panel ? k - k0 : p.panel;
+    hipLaunchKernelGGL(row_exponent_kernel, dim3((m_rows + 3) / 4), dim3(256), 0, s, a, lda,
+                       m_rows, k0, kn, ea);
+    hipLaunchKernelGGL(row_exponent_kernel, dim3((n_rows + 3) / 4), dim3(256), 0, s, b, ldb,
+                       n_rows, k0, kn, eb);
+    const dim3 sg(unsigned((p.kp2 + 255) / 256));
+    hipLaunchKernelGGL(slice_kernel, dim3(sg.x, m_rows), dim3(256), 0, s, a, lda, m_rows, k0,
+                       kn, ea, int(slices), As, p.kp2);
+    hipLaunchKernelGGL(slice_kernel, dim3(sg.x, n_rows), dim3(256), 0, s, b, ldb, n_rows, k0,
+                       kn, eb, int(slices), Bs, p.kp2);
+    const int acc = panel_no > 0;
+    switch (slices) {
+      case 1: launch_sliced_gemm<1>(As, Bs, m_rows, n_rows, p, ea, eb, part, acc, s); break;
+      case 2: launch_sliced_gemm<2>(As, Bs, m_rows, n_rows, p, ea, eb, part, acc, s); break;
+      case 3: launch_sliced_gemm<3>(As, Bs, m_rows, n_rows, p, ea, eb, part, acc, s); break;
+      default: launch_sliced_gemm<4>(As, Bs, m_rows, n_rows, p, ea, eb, part, acc, s); break;
+    }
+    int rc = check_hip(hipGetLastError(), "sliced GEMM launch");
+    if (rc) return rc;
+  }
+  const int64_t total = int64_t(m_rows) * n_rows;
+  hipLaunchKernelGGL(add_partials, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, part,
+                     int(p.zs), mu, ldc, m_rows, n_rows);
+  return check_hip(hipGetLastError(), "sliced GEMM partial sum");
+}
+
+}  // extern "C"

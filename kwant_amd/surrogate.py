@@ -306,12 +306,13 @@ class MomentMLP:
         eng.mlp_mse(y, target, ws["dy"], scale, ws["partial"])
         return ws["dy"], ws["partial"]
 
-    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, sched=None):
-        """One Adam step; with ``sched`` (device lr, bias corrections) the
-        step reads its schedule from device memory (graph replay)."""
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, sched=None, ema=None):
+        """One Adam step; with ``sched`` (device lr, bias corrections, EMA
+        decay) the step reads its schedule from device memory (graph
+        replay); with ``ema`` the parameters' moving average is updated."""
         self.t += 1
         eng.mlp_adam(self.params, self.grads, self.adam_m, self.adam_v, lr, b1, b2, eps, self.t,
-                     sched=sched)
+                     sched=sched, ema=ema)
 
 
 class MomentSurrogate:
@@ -359,7 +360,7 @@ class MomentSurrogate:
         return self.train_family.damped_moments(p, fresh_seed=fresh_seed)
 
     def fit(self, params, targets, epochs=400, batch=64, lr=3e-3, lr_min=3e-6, seed=0,
-            log=None, graph=True, b1=0.9, b2=0.999):
+            log=None, graph=True, b1=0.9, b2=0.999, ema_decay=None, ema_start=0.5):
         """Train every member on (params, targets) with Adam and a cosine
         learning-rate schedule; members draw their own mini-batch order.
 
@@ -368,7 +369,10 @@ class MomentSurrogate:
         with ``graph`` it is captured once as a HIP graph and replayed every
         epoch; the host only reshuffles and writes the schedule between
         replays.  graph=False runs the same launches eagerly (bitwise the same
-        result)."""
+        result).  With ``ema_decay`` the members end with the exponential
+        moving average of their parameters over the iterations after the
+        fraction ``ema_start`` of training (Polyak averaging), which smooths
+        the last optimiser noise out of the fit."""
         import torch
         dev = self.mlp.params.device
         x = self._inputs(params)
@@ -383,14 +387,14 @@ class MomentSurrogate:
         sched = torch.empty((per_epoch, 4), dtype=torch.float32, device=dev)
         gen = torch.Generator(device=dev)
         gen.manual_seed(seed)
-        loss = torch.zeros((), dtype=torch.float64, device=dev)
+        ema = torch.zeros_like(self.mlp.params) if ema_decay else None
 
         def epoch_body():
             for s in range(per_epoch):
                 y = self.mlp.forward(xs[s])
                 dy, partial = self.mlp.loss_grad(y, ts[s])
                 self.mlp.backward(dy)
-                self.mlp.adam(0.0, b1, b2, sched=sched[s])
+                self.mlp.adam(0.0, b1, b2, sched=sched[s], ema=ema)
             return partial
 
         g = None
@@ -412,8 +416,9 @@ class MomentSurrogate:
             for s in range(per_epoch):
                 frac = (it + s) / max(1, total - 1)
                 step = self.mlp.t + s + 1
+                decay = float(ema_decay) if ema_decay and frac >= ema_start else 0.0
                 rows[s] = (lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * frac)),
-                           1.0 / (1.0 - b1 ** step), 1.0 / (1.0 - b2 ** step), 0.0)
+                           1.0 / (1.0 - b1 ** step), 1.0 / (1.0 - b2 ** step), decay)
             sched.copy_(torch.from_numpy(rows))
             if g is not None:
                 g.replay()
@@ -425,6 +430,8 @@ class MomentSurrogate:
                 mse = float(last_partial.sum()) / (batch * z * t.shape[1])
                 self.history.append((ep, mse))
                 log("epoch %d: mse %.3e (last mini-batch)" % (ep, mse))
+        if ema is not None:
+            self.mlp.params.copy_(ema)
 
     def predict(self, params, chunk=4096, members=False):
         """Ensemble mean of the damped wide-window moments (P, M) and the

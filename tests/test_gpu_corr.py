@@ -112,3 +112,68 @@ def test_conductivity_exact_trace_antisymmetric():
     syx = k.conductivity(h, 'y', 'x', pos, vector_factory=k.LocalVectors(h), **kw)
     for mu in (-1.0, 0.7):
         assert abs(sxy(mu, 0.1) + syx(mu, 0.1)) < 1e-11
+
+
+def _cgemm_ref(a, b):
+    return np.conj(a) @ b.T
+
+
+@pytest.mark.gpu
+def test_sliced_gemm_exact_on_small_integers():
+    """Components that are integers of at most 7 bits (times a power of two
+    per row) fit in one slice: every slice count reproduces the exact
+    product bitwise."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(0)
+    m, n, k = 70, 33, 5000
+    a = (rng.randint(-127, 128, (m, k)) + 1j * rng.randint(-127, 128, (m, k))) \
+        * 2.0 ** rng.randint(-20, 20, (m, 1))
+    b = (rng.randint(-127, 128, (n, k)) + 1j * rng.randint(-127, 128, (n, k))) \
+        * 2.0 ** rng.randint(-20, 20, (n, 1))
+    want = _cgemm_ref(a, b)
+    for slices in (1, 2, 4):
+        mu = torch.zeros((m, n), dtype=torch.complex128, device="cuda")
+        eng.corr_gemm_sliced(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mu,
+                             slices=slices)
+        assert np.array_equal(mu.cpu().numpy(), want), slices
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slices,tol", [(2, 3e-4), (3, 2e-6), (4, 1e-8)])
+def test_sliced_gemm_accuracy(slices, tol):
+    """Random complex128 operands with row scales spread over 2^+-30: the
+    error against the FP64 product falls by ~2^-8 per slice."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(slices)
+    m, n, k = 130, 96, 70000
+    a = (rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))) \
+        * 2.0 ** rng.uniform(-30, 30, (m, 1))
+    b = (rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))) \
+        * 2.0 ** rng.uniform(-30, 30, (n, 1))
+    want = _cgemm_ref(a, b)
+    mu = torch.zeros((m, n), dtype=torch.complex128, device="cuda")
+    eng.corr_gemm_sliced(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mu,
+                         slices=slices)
+    got = mu.cpu().numpy()
+    # error per entry relative to the entry's own scale (row x column)
+    scale = np.sqrt(k) * np.abs(a).max(1)[:, None] * np.abs(b).max(1)[None, :]
+    err = np.max(np.abs(got - want) / scale)
+    assert err < tol, err
+
+
+@pytest.mark.gpu
+def test_sliced_gemm_panels_and_accumulate():
+    """k longer than one panel (2^20) and accumulation into an existing mu."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(9)
+    m, n, k = 17, 5, (1 << 20) + 4099
+    a = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
+    b = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    mu0 = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    mu = torch.from_numpy(mu0.copy()).cuda()
+    eng.corr_gemm_sliced(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mu, slices=4)
+    want = mu0 + _cgemm_ref(a, b)
+    assert np.max(np.abs(mu.cpu().numpy() - want)) < 1e-8 * np.sqrt(k) * 16

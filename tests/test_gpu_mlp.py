@@ -320,3 +320,24 @@ def test_fit_graph_replay_matches_eager():
         out.append(sur.mlp.params.cpu().numpy())
         assert sur.mlp.t == 4 * 3
     assert np.array_equal(out[0], out[1])
+
+
+def test_adam_device_schedule_and_ema():
+    """Adam reading (lr, bias corrections, EMA decay) from device memory,
+    with the parameters' moving average, against the oracle step."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(8)
+    n = 777
+    p = rng.standard_normal(n).astype(np.float32)
+    m, v, e = np.zeros(n), np.zeros(n), np.zeros(n)
+    tp, tm, tv, te = _t(p), _t(0 * p), _t(0 * p), _t(0 * p)
+    pp = p.astype(np.float64)
+    for step, (lr, decay) in enumerate([(1e-2, 0.0), (5e-3, 0.0), (4e-3, 0.9), (1e-3, 0.99)], 1):
+        gr = rng.standard_normal(n).astype(np.float32)
+        sched = _t([lr, 1 / (1 - 0.9 ** step), 1 / (1 - 0.999 ** step), decay])
+        eng.mlp_adam(tp, _t(gr), tm, tv, 123.0, 0.9, 0.999, 1e-8, 99, sched=sched, ema=te)
+        pp, m, v = mo.adam(pp, gr, m, v, lr, 0.9, 0.999, 1e-8, step)
+        e = decay * e + (1 - decay) * pp
+    _close(tp.cpu().numpy(), pp, 1e-5)
+    _close(te.cpu().numpy(), e, 1e-5)
