@@ -83,6 +83,80 @@ def cpu_baseline(h, a, b, vec, steps):
         return (time.perf_counter() - t0) / (steps + 1)
 
 
+_POOL_INPUT = None   # (h, a, b), set before the pool forks: inherited, not pickled
+
+
+def _pool_worker(args):
+    seed, steps = args
+    h, a, b = _POOL_INPUT
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    from oracle.kpm_oracle import time_doubling_steps
+    vec = np.exp(2j * np.pi * np.random.RandomState(seed).random_sample(h.shape[0]))
+    with threadpool_limits(limits=1):
+        time_doubling_steps(h, a, b, vec, 2)
+        t0 = time.perf_counter()
+        time_doubling_steps(h, a, b, vec, steps)
+        return (time.perf_counter() - t0) / (steps + 1)
+
+
+def cpu_baseline_pool(h, a, b, steps, width, max_procs=16):
+    """The oracle's per-vector loop on all host cores: one process per core
+    (at most ``max_procs``), each running its own vector, 1 thread each."""
+    import multiprocessing as mp
+    procs = max(1, min(max_procs, len(os.sched_getaffinity(0))))
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    global _POOL_INPUT
+    _POOL_INPUT = (h, a, b)
+    with mp.get_context("fork").Pool(procs) as pool:
+        per = pool.map(_pool_worker, [(100 + i, steps) for i in range(procs)])
+    _POOL_INPUT = None
+    t_vs = sum(per) / len(per)
+    return {"value": procs / (t_vs * width), "unit": "steps/s", "cores": procs,
+            "kind": "port", "cpu_model": model,
+            "sample": "%d processes x 1 vector x %d steps of the oracle loop, %.1f ms per "
+                      "vector-step each, aggregate scaled to %d-vector block steps"
+                      % (procs, steps, 1e3 * t_vs, width)}
+
+
+def c1_bench(log_fn):
+    """The reference's own CPU case in full: square 100 x 100, U[-0.5, 0.5]
+    disorder, 10 random-phase vectors, 100 moments, Jackson kernel -- the
+    GPU engine and the oracle on the same seeded input."""
+    import numpy as np
+    from kwant_amd import kpm
+    from kwant_amd.lattices import square
+    from oracle import kpm_oracle as ko
+    h, _ = square(100, disorder=1.0, seed=1)
+    kw = dict(num_vectors=10, num_moments=100, rng=0)
+    t0 = time.perf_counter()
+    ko.SpectralDensity(h, **kw)
+    t_cpu = time.perf_counter() - t0
+    kpm.SpectralDensity(h, **kw)           # first call: library and plan setup
+    t0 = time.perf_counter()
+    kpm.SpectralDensity(h, **kw)
+    t_gpu = time.perf_counter() - t0
+    bounds = (-4.6, 4.6)
+    g = kpm.SpectralDensity(h, bounds=bounds, **kw)
+    o = ko.SpectralDensity(h, bounds=bounds, **kw)
+    e_g, rho_g = g()
+    e_o, rho_o = o()
+    err = float(np.max(np.abs(rho_g - rho_o)) / np.max(np.abs(rho_o)))
+    log_fn("C1 (square 100x100, 10 vectors, 100 moments): GPU %.3f s, CPU oracle %.3f s "
+           "(wall, bounds by Lanczos / ARPACK); densities with explicit bounds agree to %.1e"
+           % (t_gpu, t_cpu, err))
+    return {"workload": "square 100x100, U[-0.5,0.5], 10 vectors, 100 moments, Jackson, "
+                        "complex128", "gpu_seconds": t_gpu, "cpu_oracle_seconds": t_cpu,
+            "densities_max_rel_diff_explicit_bounds": err}
+
+
 def surrogate_bench(epochs, tol, points, judge_points, judge_sets, log_fn):
     """The moment surrogate on the square-lattice family (512 x 512 sites,
     W in [0, 4], flux in [0, 0.02], mu in [-1, 1], 512 moments, 32 vectors):
@@ -401,13 +475,32 @@ def main():
                                                       time.time() - t0))
     rng = np.random.RandomState(3)
     rng.random_sample(n)  # the v0 draw of kwant/kpm.py:203
-    dev = eng.torch_device()
     block = torch.empty((n, width), dtype=torch.complex128)
     for c in range(width):
         block[:, c] = torch.from_numpy(np.exp(2j * np.pi * rng.random_sample(n)))
-    block = block.to(dev)
-    cpu_vec = block[:, 0].cpu().numpy()
+    cpu_vec = block[:, 0].numpy().copy()
     log("vectors ready (%.1fs)" % (time.time() - t0))
+
+    # CPU baselines first, before the GPU is initialised (the pool forks)
+    cpu = cpu_all = None
+    if not args.no_cpu:
+        log("CPU baseline: oracle loop, 1 vector x %d steps, complex128, 1 thread"
+            % args.cpu_steps)
+        t_vs = cpu_baseline(h, a, b, cpu_vec, args.cpu_steps)
+        cpu = {"value": 1.0 / (t_vs * width), "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": "1 of the 64 vectors x %d steps of the same recursion "
+                         "(oracle restatement of kwant's per-vector loop: scipy csr @ v, "
+                         "numpy temporaries, np.vdot; complex128), %.1f ms per "
+                         "vector-step, scaled x%d vectors per block step"
+                         % (args.cpu_steps, 1e3 * t_vs, width)}
+        log("CPU: %.1f ms per vector-step -> %.4f block steps/s" % (1e3 * t_vs, cpu["value"]))
+        cpu_all = cpu_baseline_pool(h, a, b, args.cpu_steps, width)
+        log("CPU, %d processes: %.4f block steps/s (%s)"
+            % (cpu_all["cores"], cpu_all["value"], cpu_all["cpu_model"]))
+    c1 = c1_bench(log)
+
+    dev = eng.torch_device()
+    block = block.to(dev)
 
     m64 = eng.DeviceCSR(h, "complex64")
     if args.variant < 0:
@@ -466,19 +559,6 @@ def main():
     log("%.4f ms/step, %.1f steps/s, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
         % (ms_step, steps_s, achieved, 100 * achieved / HBM_PEAK_GBS))
 
-    cpu = None
-    if not args.no_cpu:
-        log("CPU baseline: oracle loop, 1 vector x %d steps, complex128, 1 thread"
-            % args.cpu_steps)
-        t_vs = cpu_baseline(h, a, b, cpu_vec, args.cpu_steps)
-        cpu = {"value": 1.0 / (t_vs * width), "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": "1 of the 64 vectors x %d steps of the same recursion "
-                         "(oracle restatement of kwant's per-vector loop: scipy csr @ v, "
-                         "numpy temporaries, np.vdot; complex128), %.1f ms per "
-                         "vector-step, scaled x%d vectors per block step"
-                         % (args.cpu_steps, 1e3 * t_vs, width)}
-        log("CPU: %.1f ms per vector-step -> %.4f block steps/s"
-            % (1e3 * t_vs, cpu["value"]))
 
     del rec, plan, dots, m64
     torch.cuda.empty_cache()
@@ -517,6 +597,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_step": bstep},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
+        "c1": c1,
         "correlator": correlator,
         "ensemble": ensemble,
         "surrogate": surrogate,

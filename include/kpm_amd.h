@@ -110,9 +110,10 @@ int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream);
  * ring + k*slot_stride complex elements, each slot an n_rows x width block.
  * With slots == 2 the update is in place (a_{n+1} overwrites a_{n-1}).
  * The step index n comes from the plan's counter, which each step advances.
- * A step with a dots mode is two launches: the streaming kernel, which
- * writes one (sum, error) partial per workgroup, and a reduction kernel that
- * sums them in workgroup order into the step's record.
+ * Each launch writes one (sum, error) partial per workgroup and column; the
+ * next step's launch first reduces the previous step's partials (in
+ * workgroup order) into that step's record, and one reduction kernel after
+ * the last step of the call finishes the final record.
  *
  * dots_mode KPM_DOTS_DOUBLING writes, per step, segment and column c,
  *   d_dots[((n - out_base) * n_segments + s) * width + c] = 4 doubles
@@ -156,8 +157,10 @@ int kpm_spmm(const kpm_csr* op, const void* x, int64_t x_stride, void* y,
  *   KPM_GEMM_FP32  : complex64 inputs, FP32 MFMA, FP32 partial sums over
  *                    k-chunks of k_chunk, then FP64 accumulation
  *   KPM_GEMM_FP32X3: complex128 inputs split hi+lo into FP32, three FP32 MFMA
- *                    products (hi*hi + hi*lo + lo*hi) = split-precision
- *                    correction, FP64 accumulation across k-chunks
+ *                    products (hi*hi + hi*lo + lo*hi), FP64 accumulation
+ *                    across k-chunks.  The FP32 MFMA rounds products and its
+ *                    accumulator to 24 bits, so this is no more accurate than
+ *                    KPM_GEMM_FP32; kpm_corr_gemm_sliced is the accurate split.
  * k is split into slices to fill the GPU when a workspace is given: work is a
  * device FP64 buffer of work_elems doubles (2 * m_rows * n_rows per slice;
  * NULL = no split).  Slices are added in order (deterministic).

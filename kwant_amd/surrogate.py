@@ -336,6 +336,7 @@ class MomentSurrogate:
         # the largest |mu| whose shifted window stays inside the tight one
         self.max_shift = (1.0 - self.alpha) * family.a - abs(train_family.b - family.b)
         self.mlp = MomentMLP(layers, members, seed, split)
+        self.active = members
         dev = self.mlp.params.device
         self._in_scale = torch.from_numpy(1.0 / train_family.g).to(dev)
         self._out_scale = torch.from_numpy(family.g).to(dev)
@@ -360,7 +361,8 @@ class MomentSurrogate:
         return self.train_family.damped_moments(p, fresh_seed=fresh_seed)
 
     def fit(self, params, targets, epochs=400, batch=64, lr=3e-3, lr_min=3e-6, seed=0,
-            log=None, graph=True, b1=0.9, b2=0.999, ema_decay=None, ema_start=0.5):
+            log=None, graph=True, b1=0.9, b2=0.999, ema_decay=None, ema_start=0.5,
+            warmup=0.02, health=10.0):
         """Train every member on (params, targets) with Adam and a cosine
         learning-rate schedule; members draw their own mini-batch order.
 
@@ -372,7 +374,11 @@ class MomentSurrogate:
         result).  With ``ema_decay`` the members end with the exponential
         moving average of their parameters over the iterations after the
         fraction ``ema_start`` of training (Polyak averaging), which smooths
-        the last optimiser noise out of the fit."""
+        the last optimiser noise out of the fit.  The learning rate ramps up
+        linearly over the first ``warmup`` fraction of the iterations.  After
+        training, a member whose training error exceeds ``health`` times the
+        members' median is left out of the ensemble statistics (a member
+        stuck in a poor minimum would otherwise set the spread everywhere)."""
         import torch
         dev = self.mlp.params.device
         x = self._inputs(params)
@@ -417,8 +423,10 @@ class MomentSurrogate:
                 frac = (it + s) / max(1, total - 1)
                 step = self.mlp.t + s + 1
                 decay = float(ema_decay) if ema_decay and frac >= ema_start else 0.0
-                rows[s] = (lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * frac)),
-                           1.0 / (1.0 - b1 ** step), 1.0 / (1.0 - b2 ** step), decay)
+                rate = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * frac))
+                if warmup and frac < warmup:
+                    rate *= 0.01 + 0.99 * frac / warmup
+                rows[s] = (rate, 1.0 / (1.0 - b1 ** step), 1.0 / (1.0 - b2 ** step), decay)
             sched.copy_(torch.from_numpy(rows))
             if g is not None:
                 g.replay()
@@ -432,6 +440,29 @@ class MomentSurrogate:
                 log("epoch %d: mse %.3e (last mini-batch)" % (ep, mse))
         if ema is not None:
             self.mlp.params.copy_(ema)
+        self._select_members(x, t, health, log)
+
+    def _select_members(self, x, t, health, log=None):
+        """Order the members by training error and keep those within
+        ``health`` times the median (at least one)."""
+        import torch
+        count = min(len(x), 1024)
+        y = self.mlp.forward(x[None, :count])
+        err = ((y - t[None, :count]) ** 2).mean(dim=(1, 2)).double().cpu().numpy()
+        med = float(np.median(err))
+        order = np.argsort(err)
+        keep = int(max(1, np.sum(err <= health * med)))
+        self.member_errors = err
+        self.active = keep
+        if keep < self.mlp.members or np.any(order != np.arange(len(order))):
+            idx = torch.from_numpy(order).to(self.mlp.params.device)
+            for buf in (self.mlp.params, self.mlp.adam_m, self.mlp.adam_v, self.mlp.grads):
+                for li in range(len(self.mlp.shapes)):
+                    for view in (self.mlp.weight(li, buf), self.mlp.bias(li, buf)):
+                        view.copy_(view[idx])
+        if log is not None:
+            log("members' training MSE %s; %d of %d kept"
+                % (np.array2string(err, precision=2), keep, len(err)))
 
     def predict(self, params, chunk=4096, members=False):
         """Ensemble mean of the damped wide-window moments (P, M) and the
@@ -461,10 +492,14 @@ class MomentSurrogate:
                              out_scale=self._out_scale)
             if per is not None:
                 per[:, s:e].copy_(sh)
-            eng.mlp_ensemble(sh, mean=mean[s:e])
-            nd = nodes[:, :e - s]
-            eng.mlp_gemm(sh, self._l1[None].expand(z, k, m), nd, flags=eng.MLP_BF16X3)
+            act = sh[:self.active]
+            eng.mlp_ensemble(act, mean=mean[s:e])
+            nd = nodes[:self.active, :e - s]
+            eng.mlp_gemm(act, self._l1[None].expand(self.active, k, m), nd,
+                         flags=eng.MLP_BF16X3)
             eng.mlp_ensemble(nd, dev=spread[s:e])
+        if self.active < 2:
+            spread.fill_(float("inf"))      # no spread without two healthy members
         if members:
             return mean, spread, per
         return mean, spread

@@ -334,3 +334,4 @@ def test_ensemble_segments_equal_separate_runs():
             for n in range(1, steps):
                 got += [2 * d[n, s, c, 0] - mu0, 2 * (d[n, s, c, 2] + 1j * d[n, s, c, 3]) - mu1]
             assert rel(got, mom, x.shape[0]) < 1e-11, (s, c)
+

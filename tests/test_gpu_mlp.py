@@ -341,3 +341,33 @@ def test_adam_device_schedule_and_ema():
         e = decay * e + (1 - decay) * pp
     _close(tp.cpu().numpy(), pp, 1e-5)
     _close(te.cpu().numpy(), e, 1e-5)
+
+
+def test_failed_member_is_left_out():
+    """A member whose training error is far above the others' median is
+    moved behind the healthy ones and excluded from mean and spread."""
+    import torch
+    from kwant_amd.surrogate import MomentSurrogate
+
+    class Fam:
+        num_moments = 32
+        a, b = 7.2, 0.0
+        g = np.linspace(1.0, 0.05, 32)
+
+    rng = np.random.RandomState(3)
+    params = rng.uniform([0, 0, -1], [4, 0.02, 1], (128, 3))
+    targets = np.tile(np.linspace(0.5, 0.0, 32), (128, 1)) * 0.1
+    tight = Fam()
+    tight.a = 6.1
+    sur = MomentSurrogate(Fam(), tight, members=4, layers=(2, 16, 32), seed=2)
+    sur.fit(params, targets, epochs=30, batch=32, lr=1e-2)
+    assert sur.active == 4
+    # spoil member 1, then re-run the health check on the training data
+    sur.mlp.weight(1)[1].mul_(50.0)
+    x = sur._inputs(params)
+    t = torch.from_numpy(targets.astype(np.float32)).cuda()
+    sur._select_members(x, t, 10.0)
+    assert sur.active == 3
+    assert np.argmax(sur.member_errors) == 1
+    mean, spread = sur.predict(params[:8])
+    assert torch.isfinite(spread).all()
