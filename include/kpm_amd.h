@@ -47,7 +47,10 @@ typedef void* kpm_stream_t;
 /* A CSR matrix in device memory (borrowed).  indptr has n_rows+1 int32
  * entries, indices nnz int32 entries, values nnz complex entries of
  * value_type.  This is the csr_matrix that kwant/kpm.py:181 builds from the
- * int32-indexed COO of kwant/_system.pyx:118-119. */
+ * int32-indexed COO of kwant/_system.pyx:118-119.
+ * row_nnz_hint: a row length that nearly all rows stay within (for example
+ * the 99th percentile), 0 if unknown.  It only selects how many entries per
+ * row the recursion kernel requests ahead; results never depend on it. */
 typedef struct {
   int32_t n_rows;
   int32_t n_cols;
@@ -56,6 +59,7 @@ typedef struct {
   const int32_t* indices;
   const void* values;
   int32_t value_type;
+  int32_t row_nnz_hint;
 } kpm_csr;
 
 typedef struct kpm_plan kpm_plan;

@@ -52,7 +52,27 @@ class KpmCsr(ctypes.Structure):
         ("indices", ctypes.c_void_p),
         ("values", ctypes.c_void_p),
         ("value_type", ctypes.c_int32),
+        ("row_nnz_hint", ctypes.c_int32),
     ]
+
+
+def row_nnz_hint(row_lengths, quantile=0.99):
+    """A row length that ``quantile`` of the rows stay within (the recursion
+    kernel's prefetch depth follows it; results never depend on it).
+    ``row_lengths``: numpy array or torch tensor (on any device)."""
+    cap = 64
+    if isinstance(row_lengths, np.ndarray):
+        if row_lengths.size == 0:
+            return 0
+        counts = np.bincount(np.minimum(row_lengths, cap), minlength=cap + 1)
+    else:
+        import torch
+        if row_lengths.numel() == 0:
+            return 0
+        counts = torch.bincount(torch.clamp(row_lengths, max=cap).to(torch.int64),
+                                minlength=cap + 1).cpu().numpy()
+    cum = np.cumsum(counts)
+    return int(np.searchsorted(cum, quantile * cum[-1]))
 
 
 class MlpGemmDesc(ctypes.Structure):
@@ -215,7 +235,8 @@ class DeviceCSR:
         assert self.values.dtype == cdt
         self.desc = KpmCsr(int(csr.shape[0]), int(csr.shape[1]), self.nnz,
                            self.indptr.data_ptr(), self.indices.data_ptr(),
-                           self.values.data_ptr(), self.value_type)
+                           self.values.data_ptr(), self.value_type,
+                           row_nnz_hint(np.diff(csr.indptr)))
 
     @classmethod
     def from_device_arrays(cls, shape, indptr, indices, values, precision):
@@ -229,7 +250,8 @@ class DeviceCSR:
         self.precision = precision
         self.indptr, self.indices, self.values = indptr, indices, values
         self.desc = KpmCsr(int(shape[0]), int(shape[1]), self.nnz, indptr.data_ptr(),
-                           indices.data_ptr(), values.data_ptr(), self.value_type)
+                           indices.data_ptr(), values.data_ptr(), self.value_type,
+                           row_nnz_hint(indptr[1:] - indptr[:-1]))
         return self
 
     def spmm(self, x, y, width, n_slots=1, x_stride=0, y_stride=0):

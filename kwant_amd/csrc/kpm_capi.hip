@@ -96,7 +96,7 @@ extern "C" {
 
 const char* kpm_last_error(void) { return g_last_error.c_str(); }
 
-int kpm_abi_version(void) { return 100; }
+int kpm_abi_version(void) { return 101; }
 
 int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch) {
   hipDeviceProp_t prop;
@@ -124,8 +124,8 @@ int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count) {
 
 int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant, int32_t* lanes_per_row,
                      int32_t* vec, int32_t* waves, int32_t* rounds, int32_t* unroll) {
-  KernelShape ks[16];
-  const int n = variants_for(width, value_type, ks, 16);
+  KernelShape ks[kMaxVariants];
+  const int n = variants_for(width, value_type, ks, kMaxVariants);
   if (variant < 0 || variant >= n) return fail(KPM_ERR_ARG, "variant out of range");
   if (lanes_per_row) *lanes_per_row = ks[variant].lpr;
   if (vec) *vec = ks[variant].vec;
@@ -148,8 +148,8 @@ int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n
   for (int s = 0; s < n_segments; ++s)
     if (seg_row_begin[s + 1] <= seg_row_begin[s])
       return fail(KPM_ERR_ARG, "every segment needs at least one row");
-  KernelShape ks[16];
-  const int nv = variants_for(width, value_type, ks, 16);
+  KernelShape ks[kMaxVariants];
+  const int nv = variants_for(width, value_type, ks, kMaxVariants);
   if (nv == 0) return fail(KPM_ERR_UNSUPPORTED, "block width must be a power of two <= 128 "
                                                 "(<= 64 for complex128)");
   if (variant < 0) variant = 0;
@@ -308,7 +308,7 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
       p.prev_part = plan->part + ((n - 1) & 1) * half;
       p.prev_dots = d_dots + int64_t(n - 1 - out_base) * rec;
     }
-    rc = launch_cheb(plan->value_type, plan->shape, dots_mode, false, n, p,
+    rc = launch_cheb(plan->value_type, plan->shape, dots_mode, false, n, h->row_nnz_hint, p,
                      (hipStream_t)stream);
     if (rc) return rc;
     plan->step = n + 1;
@@ -337,7 +337,7 @@ int kpm_block_dots(kpm_plan* plan, const void* x, const void* w, double* d_out,
   p.w = w;
   p.dots = d_out;
   fill_params(plan, p);
-  int rc = launch_cheb(plan->value_type, plan->shape, dots_mode, true, 0, p,
+  int rc = launch_cheb(plan->value_type, plan->shape, dots_mode, true, 0, 0, p,
                        (hipStream_t)stream);
   if (!rc) rc = launch_reduce(dots_mode, plan->width, p, (hipStream_t)stream);
   return rc;

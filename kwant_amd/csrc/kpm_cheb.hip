@@ -272,6 +272,15 @@ __device__ inline void add_dots(double (&as)[NC][VEC], double (&ac)[NC][VEC],
   }
 }
 
+// Entries per chunk of the per-lane path: all index and value loads of a
+// chunk are issued before its gathers, so a row of up to CHV entries costs one
+// index-to-gather round trip.
+#ifndef KPM_VEC_CHUNK
+#define KPM_VEC_CHUNK 4
+#endif
+constexpr int CHV = KPM_VEC_CHUNK;
+
+
 // The same product for one row per lane group with per-lane (vector) loads of
 // the matrix entries (several rows per wavefront).  Also loads the row's own
 // a_n (xs) and a_{n-1} (xp), after the first chunk of matrix entries.
@@ -285,11 +294,11 @@ __device__ inline CLane<T, VEC> spmm_row_vector(const int32_t* __restrict__ indp
   using IO = LaneIO<T, VEC>;
   const int k0 = live ? indptr[row] : 0;
   const int k1 = live ? indptr[row + 1] : 0;
-  int j0[4];
-  T hr0[4], hi0[4];
+  int j0[CHV];
+  T hr0[CHV], hi0[CHV];
   if (k0 < k1) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < CHV; ++u) {
       const int kk = k0 + u < k1 ? k0 + u : k1 - 1;
       j0[u] = indices[kk];
       load_value<T>(values, kk, hr0[u], hi0[u]);
@@ -303,20 +312,20 @@ __device__ inline CLane<T, VEC> spmm_row_vector(const int32_t* __restrict__ indp
     zero(xp);
   CLane<T, VEC> acc;
   zero(acc);
-  for (int k = k0; k < k1; k += 4) {
-    T hr[4], hi[4];
-    int j[4];
-    CLane<T, VEC> x[4];
+  for (int k = k0; k < k1; k += CHV) {
+    T hr[CHV], hi[CHV];
+    int j[CHV];
+    CLane<T, VEC> x[CHV];
     if (k == k0) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < CHV; ++u) {
         j[u] = j0[u];
         hr[u] = hr0[u];
         hi[u] = hi0[u];
       }
     } else {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < CHV; ++u) {
         const int kk = k + u < k1 ? k + u : k1 - 1;
         j[u] = indices[kk];
         load_value<T>(values, kk, hr[u], hi[u]);
@@ -324,12 +333,12 @@ __device__ inline CLane<T, VEC> spmm_row_vector(const int32_t* __restrict__ indp
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < CHV; ++u) {
       x[u] = IO::load(cur + 2 * (int64_t(j[u]) * W + col));
       if (k + u >= k1) zero(x[u]);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < CHV; ++u)
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {
         acc.re[v] = fma(hr[u], x[u].re[v], acc.re[v]);
@@ -350,7 +359,10 @@ __device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
 // stores can alias them, so for one or two rows per wavefront (LPR >= 32) the
 // compiler reads them with scalar loads, which complete on their own counter
 // instead of queueing behind the vector gathers.
-template <typename T, int VEC, int LPR, int WAVES, int MODE, int U>
+//
+// PF (pipelined lane-group shapes only): entries per row requested one
+// iteration ahead, chosen at launch from the matrix's row-length hint.
+template <typename T, int VEC, int LPR, int WAVES, int MODE, int U, int PF>
 __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
                                                           const int32_t* __restrict__ indptr,
                                                           const int32_t* __restrict__ indices,
@@ -449,6 +461,146 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         }
         add_dots<T, VEC, MODE, NC>(as, ac, xs[u], y[u], ww[u]);
       }
+    }
+  } else if constexpr (U == 2) {
+    // Software-pipelined lane groups.  While iteration it gathers its
+    // neighbours, the first PF entries of iteration it + 1's row and the
+    // extent of iteration it + 2's row are already requested, so an iteration
+    // waits on one round of memory latency (the gathers) instead of three
+    // (extent, entries, gathers).  A group loads its row's entries
+    // cooperatively (lane s of the group takes entries s, s + LPR, ...) and
+    // shuffles them to the group; entries past PF (rows of higher degree) are
+    // handled chunk by chunk after the prefetched ones, in the same order, so
+    // the sum is bitwise the other shapes' sum.
+    static_assert(PF > 0, "pipelined lane groups need a prefetch depth");
+    constexpr int NCH = (PF + LPR - 1) / LPR;
+    const int sub = lane % LPR;
+    auto extent = [&](int it_, int& k0, int& k1) {
+      const int r = rr.x + (it_ * WAVES + wave) * RPW + grp;
+      k0 = 0;
+      k1 = 0;
+      if (!only_dots && it_ < iters && r < rr.y) {
+        k0 = indptr[r];
+        k1 = indptr[r + 1];
+      }
+    };
+    auto head = [&](int k0, int k1, int* j, T* hr, T* hi) {
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int e = c * LPR + sub;
+        j[c] = 0;
+        hr[c] = T(0);
+        hi[c] = T(0);
+        if (e < PF && k0 + e < k1) {
+          j[c] = indices[k0 + e];
+          load_value<T>(values, k0 + e, hr[c], hi[c]);
+        }
+      }
+    };
+    int kA0, kA1, kB0, kB1;
+    int jA[NCH];
+    T hrA[NCH], hiA[NCH];
+    extent(0, kA0, kA1);
+    head(kA0, kA1, jA, hrA, hiA);
+    extent(1, kB0, kB1);
+    for (int it = 0; it < iters; ++it) {
+      const int row0 = __builtin_amdgcn_readfirstlane(rr.x + (it * WAVES + wave) * RPW);
+      if (row0 >= rr.y) break;
+      const int row = row0 + grp;
+      const bool live = row < rr.y;
+      const int64_t roff = 2 * (int64_t(live ? row : row0) * W + col);
+      int jB[NCH];
+      T hrB[NCH], hiB[NCH];
+      int kC0, kC1;
+      head(kB0, kB1, jB, hrB, hiB);
+      extent(it + 2, kC0, kC1);
+      CLane<T, VEC> y, xs, ww;
+      zero(y);
+      xs = IO::load(cur + roff);
+      if (!only_dots) {
+        CLane<T, VEC> xp, acc;
+        if (n > 0)
+          xp = IO::load_nt(prv + roff);
+        else
+          zero(xp);
+        zero(acc);
+        const int deg = kA1 - kA0;
+        CLane<T, VEC> x[PF];
+        T hr[PF], hi[PF];
+#pragma unroll
+        for (int e = 0; e < PF; ++e) {
+          const int j = __shfl(jA[e / LPR], e % LPR, LPR);
+          hr[e] = __shfl(hrA[e / LPR], e % LPR, LPR);
+          hi[e] = __shfl(hiA[e / LPR], e % LPR, LPR);
+          if (e < deg)
+            x[e] = IO::load(cur + 2 * (int64_t(j) * W + col));
+          else
+            zero(x[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < PF; ++e) {
+          if (e < deg) {
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+              acc.re[v] = fma(hr[e], x[e].re[v], acc.re[v]);
+              acc.re[v] = fma(-hi[e], x[e].im[v], acc.re[v]);
+              acc.im[v] = fma(hr[e], x[e].im[v], acc.im[v]);
+              acc.im[v] = fma(hi[e], x[e].re[v], acc.im[v]);
+            }
+          }
+        }
+        for (int kt = kA0 + PF; kt < kA1; kt += LPR) {
+          // entries past the prefetched ones: the whole group takes this loop
+          // together (one row per group), so the shuffles stay in the group
+          int jt = 0;
+          T htr = T(0), hti = T(0);
+          if (kt + sub < kA1) {
+            jt = indices[kt + sub];
+            load_value<T>(values, kt + sub, htr, hti);
+          }
+#pragma unroll
+          for (int s = 0; s < LPR; ++s) {
+            const int j = __shfl(jt, s, LPR);
+            const T er = __shfl(htr, s, LPR);
+            const T ei = __shfl(hti, s, LPR);
+            if (kt + s < kA1) {
+              const CLane<T, VEC> xe = IO::load(cur + 2 * (int64_t(j) * W + col));
+#pragma unroll
+              for (int v = 0; v < VEC; ++v) {
+                acc.re[v] = fma(er, xe.re[v], acc.re[v]);
+                acc.re[v] = fma(-ei, xe.im[v], acc.re[v]);
+                acc.im[v] = fma(er, xe.im[v], acc.im[v]);
+                acc.im[v] = fma(ei, xe.re[v], acc.im[v]);
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          y.re[v] = fma(acc.re[v] - bsh * xs.re[v], mul, -xp.re[v]);
+          y.im[v] = fma(acc.im[v] - bsh * xs.im[v], mul, -xp.im[v]);
+        }
+        if (live) IO::store_nt(nxt + roff, y);
+      }
+      if (MODE == KPM_DOTS_PROJECT)
+        ww = IO::load(wv + roff);
+      else
+        zero(ww);
+      if (!live) {
+        zero(xs);
+        zero(y);
+      }
+      add_dots<T, VEC, MODE, NC>(as, ac, xs, y, ww);
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        jA[c] = jB[c];
+        hrA[c] = hrB[c];
+        hiA[c] = hiB[c];
+      }
+      kA0 = kB0;
+      kA1 = kB1;
+      kB0 = kC0;
+      kB1 = kC1;
     }
   } else {
     for (int it = 0; it < iters; ++it) {
@@ -639,6 +791,13 @@ namespace {
 #endif
 constexpr int kWaves = KPM_WAVES;
 
+// Prefetch depths of the pipelined lane-group loop: the smallest one that
+// covers the matrix's typical row length is launched (padding a row's
+// prefetch past its length costs shuffles and registers, and the registers
+// cost occupancy).
+constexpr int kPrefetchDepths[] = {4, 6, 8, 10, 12, 16};
+constexpr int kDefaultPrefetch = 12;
+
 template <int MODE>
 int launch_reduce_mode(const StepParams& p, int width, hipStream_t stream) {
   constexpr int NC = NComp<MODE>::value;
@@ -648,22 +807,22 @@ int launch_reduce_mode(const StepParams& p, int width, hipStream_t stream) {
   return check_hip(hipGetLastError(), "cheb_reduce launch");
 }
 
-template <typename T, int VEC, int LPR, int U>
+template <typename T, int VEC, int LPR, int U, int PF>
 int launch_u(int mode, bool only_dots, int iters, int n, const StepParams& p,
              hipStream_t stream) {
   dim3 grid(p.nblocks_launch), block(kWaves * 64);
   switch (mode) {
     case KPM_DOTS_NONE:
-      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_NONE, U>), grid, block, 0,
+      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_NONE, U, PF>), grid, block, 0,
                          stream, p, p.indptr, p.indices, p.values, n, iters, int(only_dots));
       break;
     case KPM_DOTS_DOUBLING:
-      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_DOUBLING, U>), grid, block,
+      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_DOUBLING, U, PF>), grid, block,
                          0, stream, p, p.indptr, p.indices, p.values, n, iters,
                          int(only_dots));
       break;
     case KPM_DOTS_PROJECT:
-      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_PROJECT, U>), grid, block,
+      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, kWaves, KPM_DOTS_PROJECT, U, PF>), grid, block,
                          0, stream, p, p.indptr, p.indices, p.values, n, iters,
                          int(only_dots));
       break;
@@ -673,14 +832,28 @@ int launch_u(int mode, bool only_dots, int iters, int n, const StepParams& p,
   return check_hip(hipGetLastError(), "cheb_kernel launch");
 }
 
-// Row-slot unrolling applies to the scalar-metadata shapes (LPR >= 32).
+// unroll 2: row-slot unrolling for the scalar-metadata shapes (LPR >= 32),
+// the pipelined loop for lane groups of 4 to 16 lanes with prefetch depth pf
+// (one of kPrefetchDepths).
 template <typename T, int VEC, int LPR>
-int launch_one(int mode, bool only_dots, int iters, int n, int unroll, const StepParams& p,
-               hipStream_t stream) {
+int launch_one(int mode, bool only_dots, int iters, int n, int unroll, int pf,
+               const StepParams& p, hipStream_t stream) {
   if constexpr (LPR >= 32) {
-    if (unroll == 2) return launch_u<T, VEC, LPR, 2>(mode, only_dots, iters, n, p, stream);
+    if (unroll == 2) return launch_u<T, VEC, LPR, 2, 0>(mode, only_dots, iters, n, p, stream);
+  } else if constexpr (LPR >= 4) {
+    if (unroll == 2) {
+      switch (pf) {
+        case 4: return launch_u<T, VEC, LPR, 2, 4>(mode, only_dots, iters, n, p, stream);
+        case 6: return launch_u<T, VEC, LPR, 2, 6>(mode, only_dots, iters, n, p, stream);
+        case 8: return launch_u<T, VEC, LPR, 2, 8>(mode, only_dots, iters, n, p, stream);
+        case 10: return launch_u<T, VEC, LPR, 2, 10>(mode, only_dots, iters, n, p, stream);
+        case 12: return launch_u<T, VEC, LPR, 2, 12>(mode, only_dots, iters, n, p, stream);
+        case 16: return launch_u<T, VEC, LPR, 2, 16>(mode, only_dots, iters, n, p, stream);
+        default: return fail(KPM_ERR_ARG, "unsupported prefetch depth");
+      }
+    }
   }
-  return launch_u<T, VEC, LPR, 1>(mode, only_dots, iters, n, p, stream);
+  return launch_u<T, VEC, LPR, 1, 0>(mode, only_dots, iters, n, p, stream);
 }
 
 template <typename T, int VEC, int LPR>
@@ -730,16 +903,24 @@ int variants_for(int width, int value_type, KernelShape* out, int max_out) {
   int n = 0;
   // A variant is a lane layout plus how many residency rounds the grid is
   // sized for (1: every workgroup resident at once, one contiguous row range
-  // each; 2: twice as many, half as long).  Rows per workgroup are set by the
-  // plan from the kernel's occupancy and the device's CU count.
+  // each; R: R times as many, R times shorter).  Rows per workgroup are set by
+  // the plan from the kernel's occupancy and the device's CU count.  Many
+  // rounds keep the resident workgroups of an XCD on one narrow window of
+  // rows, so neighbours far apart in the row order (the next plane of a 3D
+  // lattice) are read while still in L2; few rounds suit matrices whose
+  // neighbours are close (2D lattices).
   auto add = [&](int lpr, int vec) {
-    // order: the measured-best shape on the 2M-site graphene block first
-    const int max_unroll = lpr >= 32 ? 2 : 1;
-    for (int unroll = 1; unroll <= max_unroll; ++unroll)
-      for (int rounds = 2; rounds >= 1; --rounds) {
+    // unroll 2: two row slots per wave (LPR >= 32) or the pipelined
+    // lane-group loop (LPR 4 .. 16), which comes first for lane groups
+    const int n_unroll = lpr >= 4 ? 2 : 1;
+    const int first = lpr >= 4 && lpr < 32 ? 2 : 1;
+    for (int ui = 0; ui < n_unroll; ++ui) {
+      const int unroll = ui == 0 ? first : 3 - first;
+      for (int rounds : {2, 1, 8, 32, 128}) {
         if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, rounds, unroll, 0};
         ++n;
       }
+    }
   };
   if (value_type == KPM_C64) {
     if (width >= 2) add(width / 2, 2);
@@ -757,15 +938,18 @@ struct OccQuery {
   int unroll;
   template <typename T, int V, int LP>
   int operator()() {
-    if constexpr (LP >= 32) {
+    // the pipelined lane-group shapes are sized with the default depth
+    if constexpr (LP >= 4) {
       if (unroll == 2)
         return check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                             out, cheb_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING, 2>,
+                             out,
+                             cheb_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING, 2,
+                                         LP >= 32 ? 0 : kDefaultPrefetch>,
                              kWaves * 64, 0),
                          "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     }
     return check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         out, cheb_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING, 1>, kWaves * 64,
+                         out, cheb_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING, 1, 0>, kWaves * 64,
                          0),
                      "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   }
@@ -788,11 +972,12 @@ struct ChebLauncher {
   int iters;
   int n;
   int unroll;
+  int pf;
   const StepParams& p;
   hipStream_t s;
   template <typename T, int V, int LP>
   int operator()() {
-    return launch_one<T, V, LP>(mode, only_dots, iters, n, unroll, p, s);
+    return launch_one<T, V, LP>(mode, only_dots, iters, n, unroll, pf, p, s);
   }
 };
 
@@ -812,9 +997,17 @@ struct SpmmLauncher {
 
 }  // namespace
 
+int prefetch_depth(int32_t row_nnz_hint) {
+  if (row_nnz_hint <= 0) return kDefaultPrefetch;
+  for (int pf : kPrefetchDepths)
+    if (row_nnz_hint <= pf) return pf;
+  return kPrefetchDepths[sizeof(kPrefetchDepths) / sizeof(int) - 1];
+}
+
 int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots, int n,
-                const StepParams& p, hipStream_t stream) {
-  ChebLauncher l{mode, only_dots, ks.iters, n, ks.unroll, p, stream};
+                int row_nnz_hint, const StepParams& p, hipStream_t stream) {
+  ChebLauncher l{mode, only_dots, ks.iters, n, ks.unroll, prefetch_depth(row_nnz_hint), p,
+                 stream};
   if (value_type == KPM_C64) return dispatch_shape<float>(ks.vec, ks.lpr, l);
   if (value_type == KPM_C128) return dispatch_shape<double>(ks.vec, ks.lpr, l);
   return fail(KPM_ERR_ARG, "unknown value type");

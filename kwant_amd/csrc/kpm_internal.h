@@ -55,12 +55,15 @@ struct StepParams {
 // (kpm_block_dots).  A dots-mode launch writes workgroup partials to p.part
 // and, when p.prev_part is set, first reduces those of the previous step into
 // p.prev_dots; launch_reduce reduces p.part into p.dots on its own.
+// row_nnz_hint: the matrix's typical row length (kpm_csr), 0 if unknown.
 int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots, int n,
-                const StepParams& p, hipStream_t stream);
+                int row_nnz_hint, const StepParams& p, hipStream_t stream);
 int launch_reduce(int mode, int width, const StepParams& p, hipStream_t stream);
 int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x,
                 int64_t x_stride, void* y, int64_t y_stride, int n_slots,
                 hipStream_t stream);
+// At most kMaxVariants launch shapes per (width, value type).
+constexpr int kMaxVariants = 32;
 int variants_for(int width, int value_type, KernelShape* out, int max_out);
 int blocks_per_cu(int value_type, const KernelShape& ks, int* out);
 
