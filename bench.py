@@ -554,8 +554,8 @@ def main():
     steps_s = 1000.0 / ms_step
     bstep = bytes_per_step(n, nnz, width, 8, 8)
     achieved = bstep / (ms_step * 1e-3) / 1e9
-    lpr, vec, waves, rounds, unroll = eng.variant_info(width, eng.KPM_C64,
-                                                       args.variant)
+    lpr, vec, waves, rounds, unroll, chunk = eng.variant_info(width, eng.KPM_C64,
+                                                              args.variant)
     log("%.4f ms/step, %.1f steps/s, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
         % (ms_step, steps_s, achieved, 100 * achieved / HBM_PEAK_GBS))
 
@@ -592,7 +592,8 @@ def main():
                                % (args.cells, args.cells, n, nnz, width),
                    "N": n, "nnz": nnz, "block_width": width,
                    "kernel": {"lanes_per_row": lpr, "complex_per_lane": vec, "waves": waves,
-                              "grid_rounds": rounds, "row_slots": unroll}},
+                              "grid_rounds": rounds, "row_slots": unroll,
+                              "interleave_chunk": chunk, "variant": args.variant}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_step": bstep},

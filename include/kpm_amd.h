@@ -39,7 +39,7 @@ extern "C" {
 
 /* dots_mode of kpm_cheb_steps / kpm_block_dots */
 #define KPM_DOTS_NONE 0     /* recursion only (Correlator expansions)         */
-#define KPM_DOTS_DOUBLING 1 /* <a_n|a_n>, <a_{n+1}|a_n> per column            */
+#define KPM_DOTS_DOUBLING 1 /* <a_n|a_n>, Re <a_{n+1}|a_n> per column         */
 #define KPM_DOTS_PROJECT 2  /* <w|a_n>, <w|a_{n+1}> per column (operator path) */
 
 typedef void* kpm_stream_t;
@@ -76,12 +76,16 @@ int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch);
 /* Number of recursion kernel variants for a (width, value_type) pair, and
  * the launch shape of each: lanes per matrix row, complex values per lane,
  * waves per workgroup, the residency rounds the grid is sized for (rows per
- * workgroup = rows / (CUs x resident workgroups per CU x rounds)) and the row
- * slots each wavefront processes per iteration.  Used by the host autotuner. */
+ * workgroup = rows / (CUs x resident workgroups per CU x rounds)), the row
+ * slots each wavefront processes per iteration (2: the pipelined loop for
+ * 4 to 16 lanes per row) and the interleave chunk (0: each workgroup owns
+ * contiguous rows; c: the workgroups of an XCD deal a range out in chunks of
+ * c wavefront iterations).  Used by the host autotuner. */
 int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count);
 int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant,
                      int32_t* lanes_per_row, int32_t* vec,
-                     int32_t* waves, int32_t* rounds, int32_t* unroll);
+                     int32_t* waves, int32_t* rounds, int32_t* unroll,
+                     int32_t* chunk);
 
 /* Create a recursion plan for an N x width vector block whose rows are split
  * into n_segments independent systems (segment s owns rows
@@ -121,9 +125,12 @@ int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream);
  *
  * dots_mode KPM_DOTS_DOUBLING writes, per step, segment and column c,
  *   d_dots[((n - out_base) * n_segments + s) * width + c] = 4 doubles
- *   { <a_n|a_n>, 0, <a_{n+1}|a_n> (re, im) }
+ *   { <a_n|a_n>, 0, Re <a_{n+1}|a_n>, 0 }
  * (the two vdot()s of kwant/kpm.py:444-445, 470-473; at n == 0 the second
- * is <a_1|a_0>, the conjugate of mu_1).  KPM_DOTS_PROJECT writes
+ * is <a_1|a_0>, the conjugate of mu_1).  H must be Hermitian, as the
+ * expansion assumes: then Im <a_{n+1}|a_n> is exactly 0 (it equals
+ * -Im <a_{n-1}|a_n>, down to Im <a_1|a_0> = 0) and is written as 0 rather
+ * than summed from rounding noise.  KPM_DOTS_PROJECT writes
  * { <w|a_n>, <w|a_{n+1}> } with w the device block `w` (the matrix-operator
  * moments vdot(a_0, O a_n) of kwant/kpm.py:193-195, 484 with w = O^H a_0).
  * Sums are FP64 and deterministic (fixed order), so repeated runs and

@@ -37,6 +37,7 @@ struct kpm_plan {
   int32_t nblocks = 0, nblocks_launch = 0;
   int32_t step = 0;  // the next kpm_cheb_steps launch computes a_{step+1}
   int2* blk_rows = nullptr;
+  int64_t* blk_stride = nullptr;
   int32_t* blk_seg = nullptr;
   int32_t* seg_blk = nullptr;
   double* scale = nullptr;
@@ -66,6 +67,7 @@ int upload(T* d, const std::vector<T>& h, const char* what) {
 void free_plan(kpm_plan* p) {
   if (!p) return;
   (void)hipFree(p->blk_rows);
+  (void)hipFree(p->blk_stride);
   (void)hipFree(p->blk_seg);
   (void)hipFree(p->seg_blk);
   (void)hipFree(p->scale);
@@ -75,6 +77,8 @@ void free_plan(kpm_plan* p) {
 
 int fill_params(const kpm_plan* plan, StepParams& p) {
   p.blk_rows = plan->blk_rows;
+  p.blk_stride = plan->blk_stride;
+  p.chunk_iters = plan->shape.chunk;
   p.blk_seg = plan->blk_seg;
   p.seg_blk = plan->seg_blk;
   p.scale = plan->scale;
@@ -96,7 +100,7 @@ extern "C" {
 
 const char* kpm_last_error(void) { return g_last_error.c_str(); }
 
-int kpm_abi_version(void) { return 101; }
+int kpm_abi_version(void) { return 102; }
 
 int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch) {
   hipDeviceProp_t prop;
@@ -123,7 +127,8 @@ int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count) {
 }
 
 int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant, int32_t* lanes_per_row,
-                     int32_t* vec, int32_t* waves, int32_t* rounds, int32_t* unroll) {
+                     int32_t* vec, int32_t* waves, int32_t* rounds, int32_t* unroll,
+                     int32_t* chunk) {
   KernelShape ks[kMaxVariants];
   const int n = variants_for(width, value_type, ks, kMaxVariants);
   if (variant < 0 || variant >= n) return fail(KPM_ERR_ARG, "variant out of range");
@@ -132,6 +137,7 @@ int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant, int32_t
   if (waves) *waves = ks[variant].waves;
   if (rounds) *rounds = ks[variant].rounds;
   if (unroll) *unroll = ks[variant].unroll;
+  if (chunk) *chunk = ks[variant].chunk;
   return KPM_OK;
 }
 
@@ -196,15 +202,44 @@ int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n
   p->nblocks = int32_t(blk_rows.size());
   p->nblocks_launch = (p->nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;
 
+  // contiguous workgroups: one chunk each
+  std::vector<int64_t> blk_stride(blk_rows.size(), int64_t(n_rows));
+  if (p->shape.chunk > 0) {
+    // Interleaved: the workgroups of one segment on one XCD (a run of
+    // consecutive logical blocks, see the kernel's XCD remap) share the run's
+    // rows in chunks of C rows, dealt round-robin, so the run sweeps its
+    // range as one front G*C rows wide.
+    const int64_t C = rows_per_iter * p->shape.chunk;
+    const int per = p->nblocks_launch / kNumXcd;
+    int64_t max_chunks = 1;
+    for (int b0 = 0; b0 < p->nblocks;) {
+      int b1 = b0 + 1;
+      while (b1 < p->nblocks && blk_seg[b1] == blk_seg[b0] && b1 / per == b0 / per) ++b1;
+      const int64_t r_begin = blk_rows[b0].x, r_end = blk_rows[b1 - 1].y;
+      const int64_t g = b1 - b0;
+      for (int64_t k = 0; k < g; ++k) {
+        const int64_t first = r_begin + k * C < r_end ? r_begin + k * C : r_end;
+        blk_rows[b0 + k] = make_int2(int(first), int(r_end));
+        blk_stride[b0 + k] = g * C;
+        const int64_t chunks = (r_end - first + g * C - 1) / (g * C);
+        if (chunks > max_chunks) max_chunks = chunks;
+      }
+      b0 = b1;
+    }
+    p->shape.iters = int(max_chunks * p->shape.chunk);
+  }
+
   const int maxnc = 4;
   int rc = KPM_OK;
   if (!rc) rc = dev_alloc(&p->blk_rows, blk_rows.size(), "blk_rows");
+  if (!rc) rc = dev_alloc(&p->blk_stride, blk_stride.size(), "blk_stride");
   if (!rc) rc = dev_alloc(&p->blk_seg, blk_seg.size(), "blk_seg");
   if (!rc) rc = dev_alloc(&p->seg_blk, seg_blk.size(), "seg_blk");
   if (!rc) rc = dev_alloc(&p->scale, size_t(2) * n_segments, "scale");
   p->part_half = int64_t(p->nblocks) * maxnc * width * 2;
   if (!rc) rc = dev_alloc(&p->part, size_t(2 * p->part_half), "partials");
   if (!rc) rc = upload(p->blk_rows, blk_rows, "upload blk_rows");
+  if (!rc) rc = upload(p->blk_stride, blk_stride, "upload blk_stride");
   if (!rc) rc = upload(p->blk_seg, blk_seg, "upload blk_seg");
   if (!rc) rc = upload(p->seg_blk, seg_blk, "upload seg_blk");
   if (!rc) rc = check_hip(hipDeviceSynchronize(), "plan setup");

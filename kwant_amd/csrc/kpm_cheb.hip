@@ -144,16 +144,19 @@ __device__ inline void acc2(double& s, double& c, double t) {
   c += e;
 }
 
-// Number of FP64 accumulators per column for a dots mode.
+// Number of FP64 accumulators per column for a dots mode.  The doubling
+// mode sums <a_n|a_n> and Re <a_{n+1}|a_n>: for a Hermitian H (which the
+// expansion assumes) Im <a_{n+1}|a_n> = -Im <a_{n-1}|a_n> = ... =
+// Im <a_1|a_0> = 0 exactly, so only rounding noise would be summed there.
 template <int MODE>
 struct NComp {
-  static constexpr int value = MODE == KPM_DOTS_PROJECT ? 4 : (MODE == KPM_DOTS_DOUBLING ? 3 : 1);
+  static constexpr int value = MODE == KPM_DOTS_PROJECT ? 4 : (MODE == KPM_DOTS_DOUBLING ? 2 : 1);
 };
 
 // Position of accumulator c in the 4-double output record of a column.
 template <int MODE>
 __device__ inline int out_pos(int c) {
-  if (MODE == KPM_DOTS_DOUBLING) return c == 0 ? 0 : c + 1;
+  if (MODE == KPM_DOTS_DOUBLING) return c == 0 ? 0 : 2;
   return c;
 }
 
@@ -256,7 +259,6 @@ __device__ inline void add_dots(double (&as)[NC][VEC], double (&ac)[NC][VEC],
       const double yr = y.re[v], yi = y.im[v];
       acc2(as[0][v], ac[0][v], fma(xr, xr, xi * xi));
       acc2(as[1][v], ac[1][v], fma(yr, xr, yi * xi));
-      acc2(as[2][v], ac[2][v], fma(yr, xi, -(yi * xr)));
     }
   } else if (MODE == KPM_DOTS_PROJECT) {
 #pragma unroll
@@ -350,6 +352,14 @@ __device__ inline CLane<T, VEC> spmm_row_vector(const int32_t* __restrict__ indp
   return acc;
 }
 
+// First row of wavefront iteration it of a workgroup whose chunks of ipc
+// iterations (rows_per_iter rows each) start at begin, begin + stride, ...
+__device__ inline int64_t iter_row0(int it, int64_t begin, int64_t stride, int ipc,
+                                    int rows_per_iter) {
+  const int m = it / ipc;
+  return begin + m * stride + int64_t(it - m * ipc) * rows_per_iter;
+}
+
 template <int NC, int MODE>
 __device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
                                    const int32_t* __restrict__ seg_blk, int nseg, int W,
@@ -391,6 +401,12 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
   }
   const int seg = p.blk_seg[lb];
   const int2 rr = p.blk_rows[lb];
+  // The workgroup's rows: chunks of ipc wavefront iterations, the first at
+  // rr.x, the next ones stride rows further on, up to rr.y.  A contiguous
+  // workgroup has one chunk of all its iterations; interleaved workgroups of
+  // one XCD take turns over a range so that together they sweep it in step.
+  const int64_t stride = p.blk_stride[lb];
+  const int ipc = p.chunk_iters > 0 ? p.chunk_iters : iters;
   const double sa = p.scale[2 * seg];
   const double sb = p.scale[2 * seg + 1];
   const T mul = T((n == 0 ? 1.0 : 2.0) / sa);
@@ -412,11 +428,14 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
     }
 
   if constexpr (RPW <= 2) {
-    // Each wave owns a contiguous range of iters * RPW rows and walks it U
-    // row slots at a time: all streaming loads and gathers of the U slots are
-    // issued before the first is consumed.
-    const int wbeg = rr.x + wave * iters * RPW;
-    const int wend = wbeg + iters * RPW < rr.y ? wbeg + iters * RPW : rr.y;
+    // In each chunk a wave owns a contiguous range of ipc * RPW rows and walks
+    // it U row slots at a time: all streaming loads and gathers of the U
+    // slots are issued before the first is consumed.
+    for (int64_t cb = rr.x; cb < rr.y; cb += stride) {
+    const int64_t wb64 = cb + int64_t(wave) * ipc * RPW;
+    if (wb64 >= rr.y) break;
+    const int wbeg = int(wb64);
+    const int wend = wb64 + int64_t(ipc) * RPW < rr.y ? int(wb64 + int64_t(ipc) * RPW) : rr.y;
     for (int r0 = __builtin_amdgcn_readfirstlane(wbeg); r0 < wend; r0 += U * RPW) {
       CLane<T, VEC> xs[U], xp[U], y[U], ww[U];
       int64_t roff[U];
@@ -462,6 +481,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         add_dots<T, VEC, MODE, NC>(as, ac, xs[u], y[u], ww[u]);
       }
     }
+    }
   } else if constexpr (U == 2) {
     // Software-pipelined lane groups.  While iteration it gathers its
     // neighbours, the first PF entries of iteration it + 1's row and the
@@ -476,7 +496,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
     constexpr int NCH = (PF + LPR - 1) / LPR;
     const int sub = lane % LPR;
     auto extent = [&](int it_, int& k0, int& k1) {
-      const int r = rr.x + (it_ * WAVES + wave) * RPW + grp;
+      const int64_t r = iter_row0(it_, rr.x, stride, ipc, WAVES * RPW) + wave * RPW + grp;
       k0 = 0;
       k1 = 0;
       if (!only_dots && it_ < iters && r < rr.y) {
@@ -504,8 +524,9 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
     head(kA0, kA1, jA, hrA, hiA);
     extent(1, kB0, kB1);
     for (int it = 0; it < iters; ++it) {
-      const int row0 = __builtin_amdgcn_readfirstlane(rr.x + (it * WAVES + wave) * RPW);
-      if (row0 >= rr.y) break;
+      const int64_t r64 = iter_row0(it, rr.x, stride, ipc, WAVES * RPW) + wave * RPW;
+      if (r64 >= rr.y) break;
+      const int row0 = __builtin_amdgcn_readfirstlane(int(r64));
       const int row = row0 + grp;
       const bool live = row < rr.y;
       const int64_t roff = 2 * (int64_t(live ? row : row0) * W + col);
@@ -605,8 +626,9 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
   } else {
     for (int it = 0; it < iters; ++it) {
       // rows row0 .. row0 + RPW - 1 belong to this wave; lane group grp owns one
-      const int row0 = __builtin_amdgcn_readfirstlane(rr.x + (it * WAVES + wave) * RPW);
-      if (row0 >= rr.y) break;
+      const int64_t r64 = iter_row0(it, rr.x, stride, ipc, WAVES * RPW) + wave * RPW;
+      if (r64 >= rr.y) break;
+      const int row0 = __builtin_amdgcn_readfirstlane(int(r64));
       const int row = row0 + grp;
       const bool live = row < rr.y;
       const int64_t roff = 2 * (int64_t(live ? row : row0) * W + col);
@@ -716,7 +738,10 @@ __device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
     if (lane == 0) {
       double* rec = rec_out + (int64_t(seg) * W + cc) * 4;
       rec[out_pos<MODE>(c)] = s + e;
-      if (MODE == KPM_DOTS_DOUBLING && c == 0) rec[1] = 0.0;
+      if (MODE == KPM_DOTS_DOUBLING && c == 0) {
+        rec[1] = 0.0;
+        rec[3] = 0.0;
+      }
     }
   }
 }
@@ -917,7 +942,11 @@ int variants_for(int width, int value_type, KernelShape* out, int max_out) {
     for (int ui = 0; ui < n_unroll; ++ui) {
       const int unroll = ui == 0 ? first : 3 - first;
       for (int rounds : {2, 1, 8, 32, 128}) {
-        if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, rounds, unroll, 0};
+        if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, rounds, unroll, 0, 0};
+        ++n;
+      }
+      for (int chunk : {4, 8, 32}) {
+        if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, 1, unroll, chunk, 0};
         ++n;
       }
     }

@@ -24,6 +24,8 @@ struct KernelShape {
   int waves;  // wavefronts per workgroup
   int rounds; // residency rounds the grid is sized for
   int unroll; // row slots per wave iteration (scalar-metadata shapes)
+  int chunk;  // 0: contiguous rows per workgroup; c > 0: the workgroups of an
+              // XCD interleave chunks of c wavefront iterations (rounds 1)
   int iters;  // row iterations per workgroup (set by the plan)
 };
 
@@ -37,7 +39,9 @@ struct StepParams {
   int32_t slots;
   const void* w;
   double* dots;            // this step's record: nseg x W x 4 doubles
-  const int2* blk_rows;    // logical block -> [row_begin, row_end)
+  const int2* blk_rows;    // logical block -> [first row, row_end)
+  const int64_t* blk_stride; // logical block -> rows from one chunk to the next
+  int32_t chunk_iters;     // wavefront iterations per chunk (0: one chunk)
   const int32_t* blk_seg;  // logical block -> segment
   const int32_t* seg_blk;  // segment -> first logical block (nseg + 1)
   const double* scale;     // segment -> (a, b)
@@ -63,7 +67,7 @@ int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x,
                 int64_t x_stride, void* y, int64_t y_stride, int n_slots,
                 hipStream_t stream);
 // At most kMaxVariants launch shapes per (width, value type).
-constexpr int kMaxVariants = 32;
+constexpr int kMaxVariants = 48;
 int variants_for(int width, int value_type, KernelShape* out, int max_out);
 int blocks_per_cu(int value_type, const KernelShape& ks, int* out);
 

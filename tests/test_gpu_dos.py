@@ -103,6 +103,75 @@ def test_all_variants_agree_bitwise():
                     assert np.array_equal(m, base), (prec, width, v)
 
 
+def ragged_hermitian(n, seed):
+    """Hermitian CSR with ragged rows: dense Hermitian blocks of 1 to 40
+    rows (a block of one row is an empty row) under a random symmetric
+    permutation, so the row lengths 0, 2, 3, 5, ..., 40 cross every prefetch
+    depth of the recursion kernel."""
+    import scipy.sparse as sp
+    rng = np.random.RandomState(seed)
+    blocks, total = [], 0
+    while total < n:
+        s = int(min(rng.choice([1, 1, 2, 3, 5, 7, 9, 11, 13, 17, 40]), n - total))
+        if s == 1:
+            blocks.append(np.zeros((1, 1), complex))
+        else:
+            a = rng.randn(s, s) + 1j * rng.randn(s, s)
+            blocks.append(a + a.conj().T)
+        total += s
+    h = sp.block_diag(blocks).tocsr()
+    h.eliminate_zeros()
+    perm = rng.permutation(n)
+    h = h[perm][:, perm].tocsr()
+    h.sort_indices()
+    return h
+
+
+def test_prefetch_depth_never_changes_results():
+    """Moments are bitwise the same whatever row-length hint (hence prefetch
+    depth of the pipelined lane groups) and launch variant run; the rows
+    include empty ones and rows longer than every depth."""
+    import torch
+    from kwant_amd import _engine as eng
+    h = ragged_hermitian(700, 11)
+    lengths = np.diff(h.indptr)
+    assert lengths.min() == 0 and lengths.max() > 16
+    r = float(np.max(np.abs(h).sum(axis=1)))
+    for prec in ("complex128", "complex64"):
+        cdt, vt = eng.complex_dtype(prec)
+        for width in (8, 16):
+            gen = torch.Generator(device="cuda")
+            gen.manual_seed(width)
+            ph = torch.rand((h.shape[0], width), generator=gen, device="cuda",
+                            dtype=torch.float64)
+            block = torch.polar(torch.ones_like(ph), 2 * np.pi * ph).to(cdt)
+            base = None
+            for v in range(eng.variant_count(width, vt)):
+                for hint in (0, 3, 4, 6, 8, 10, 12, 16, 64):
+                    mat = eng.DeviceCSR(h, prec)
+                    mat.desc.row_nnz_hint = hint
+                    rec = eng.BlockRecursion(mat, (1.05 * r, 0.1), block, width,
+                                             eng.DOTS_DOUBLING, variant=v)
+                    d = rec.step(24)
+                    if base is None:
+                        base = d
+                    else:
+                        assert np.array_equal(d, base), (prec, width, v, hint)
+
+
+def test_ragged_rows_match_oracle():
+    """Empty rows and rows longer than every prefetch depth: complex128
+    moments within 1e-11 of the oracle for widths served by each lane
+    layout (1, 8 and 16 vectors per block)."""
+    h = ragged_hermitian(500, 12)
+    bounds = gershgorin(h)
+    for nv in (1, 8, 16):
+        kw = dict(num_moments=60, num_vectors=nv, bounds=bounds, rng=4)
+        g = np.array(kpm().SpectralDensity(h, max_block_width=nv, **kw)._moments_list)
+        o = np.array(ko.SpectralDensity(h, **kw)._moments_list)
+        assert rel(g, o, 500) < 1e-11, nv
+
+
 # --- mirrors of kwant/tests/test_kpm.py on the GPU path ---------------------
 
 def test_increase_num_moments_bitwise():
