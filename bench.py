@@ -126,7 +126,7 @@ def cpu_baseline_pool(h, a, b, steps, width, max_procs=16):
                       % (procs, steps, 1e3 * t_vs, width)}
 
 
-def c1_bench(log_fn):
+def small_square_bench(log_fn):
     """The reference's own CPU case in full: square 100 x 100, U[-0.5, 0.5]
     disorder, 10 random-phase vectors, 100 moments, Jackson kernel -- the
     GPU engine and the oracle on the same seeded input."""
@@ -149,7 +149,7 @@ def c1_bench(log_fn):
     e_g, rho_g = g()
     e_o, rho_o = o()
     err = float(np.max(np.abs(rho_g - rho_o)) / np.max(np.abs(rho_o)))
-    log_fn("C1 (square 100x100, 10 vectors, 100 moments): GPU %.3f s, CPU oracle %.3f s "
+    log_fn("small square case (100x100, 10 vectors, 100 moments): GPU %.3f s, CPU oracle %.3f s "
            "(wall, bounds by Lanczos / ARPACK); densities with explicit bounds agree to %.1e"
            % (t_gpu, t_cpu, err))
     return {"workload": "square 100x100, U[-0.5,0.5], 10 vectors, 100 moments, Jackson, "
@@ -497,7 +497,7 @@ def main():
         cpu_all = cpu_baseline_pool(h, a, b, args.cpu_steps, width)
         log("CPU, %d processes: %.4f block steps/s (%s)"
             % (cpu_all["cores"], cpu_all["value"], cpu_all["cpu_model"]))
-    c1 = c1_bench(log)
+    small = small_square_bench(log)
 
     dev = eng.torch_device()
     block = block.to(dev)
@@ -554,8 +554,8 @@ def main():
     steps_s = 1000.0 / ms_step
     bstep = bytes_per_step(n, nnz, width, 8, 8)
     achieved = bstep / (ms_step * 1e-3) / 1e9
-    lpr, vec, waves, rounds, unroll, chunk = eng.variant_info(width, eng.KPM_C64,
-                                                              args.variant)
+    lpr, vec, waves, rounds, unroll, chunk, spare = eng.variant_info(width, eng.KPM_C64,
+                                                                     args.variant)
     log("%.4f ms/step, %.1f steps/s, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
         % (ms_step, steps_s, achieved, 100 * achieved / HBM_PEAK_GBS))
 
@@ -593,13 +593,14 @@ def main():
                    "N": n, "nnz": nnz, "block_width": width,
                    "kernel": {"lanes_per_row": lpr, "complex_per_lane": vec, "waves": waves,
                               "grid_rounds": rounds, "row_slots": unroll,
-                              "interleave_chunk": chunk, "variant": args.variant}},
+                              "interleave_chunk": chunk, "spare_per_cu": spare,
+                              "variant": args.variant}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_step": bstep},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
-        "c1": c1,
+        "small_square": small,
         "correlator": correlator,
         "ensemble": ensemble,
         "surrogate": surrogate,

@@ -78,14 +78,15 @@ int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch);
  * waves per workgroup, the residency rounds the grid is sized for (rows per
  * workgroup = rows / (CUs x resident workgroups per CU x rounds)), the row
  * slots each wavefront processes per iteration (2: the pipelined loop for
- * 4 to 16 lanes per row) and the interleave chunk (0: each workgroup owns
+ * 4 to 16 lanes per row), the interleave chunk (0: each workgroup owns
  * contiguous rows; c: the workgroups of an XCD deal a range out in chunks of
- * c wavefront iterations).  Used by the host autotuner. */
+ * c wavefront iterations) and the resident workgroups per CU the grid leaves
+ * spare.  Used by the host autotuner. */
 int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count);
 int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant,
                      int32_t* lanes_per_row, int32_t* vec,
                      int32_t* waves, int32_t* rounds, int32_t* unroll,
-                     int32_t* chunk);
+                     int32_t* chunk, int32_t* spare);
 
 /* Create a recursion plan for an N x width vector block whose rows are split
  * into n_segments independent systems (segment s owns rows

@@ -112,7 +112,7 @@ def load_library(path=LIB_PATH):
         "kpm_abi_version": (ctypes.c_int, []),
         "kpm_device_info": (ctypes.c_int, [i32, pi32, pi32]),
         "kpm_variant_count": (ctypes.c_int, [i32, i32, pi32]),
-        "kpm_variant_info": (ctypes.c_int, [i32, i32, i32, pi32, pi32, pi32, pi32, pi32, pi32]),
+        "kpm_variant_info": (ctypes.c_int, [i32, i32, i32] + [pi32] * 7),
         "kpm_plan_create": (ctypes.c_int, [i32, i32, i32, i32, pi32, i32,
                                            ctypes.POINTER(vp)]),
         "kpm_plan_destroy": (ctypes.c_int, [vp]),
@@ -193,8 +193,8 @@ def variant_count(width, value_type):
 
 def variant_info(width, value_type, variant):
     """(lanes per row, complex values per lane, waves, residency rounds,
-    row slots per wave iteration, interleave chunk)."""
-    vals = [ctypes.c_int32() for _ in range(6)]
+    row slots per wave iteration, interleave chunk, spare workgroups per CU)."""
+    vals = [ctypes.c_int32() for _ in range(7)]
     check(load_library().kpm_variant_info(width, value_type, variant,
                                           *[ctypes.byref(v) for v in vals]),
           "kpm_variant_info")

@@ -100,7 +100,7 @@ extern "C" {
 
 const char* kpm_last_error(void) { return g_last_error.c_str(); }
 
-int kpm_abi_version(void) { return 102; }
+int kpm_abi_version(void) { return 103; }
 
 int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch) {
   hipDeviceProp_t prop;
@@ -128,7 +128,7 @@ int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count) {
 
 int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant, int32_t* lanes_per_row,
                      int32_t* vec, int32_t* waves, int32_t* rounds, int32_t* unroll,
-                     int32_t* chunk) {
+                     int32_t* chunk, int32_t* spare) {
   KernelShape ks[kMaxVariants];
   const int n = variants_for(width, value_type, ks, kMaxVariants);
   if (variant < 0 || variant >= n) return fail(KPM_ERR_ARG, "variant out of range");
@@ -138,6 +138,7 @@ int kpm_variant_info(int32_t width, int32_t value_type, int32_t variant, int32_t
   if (rounds) *rounds = ks[variant].rounds;
   if (unroll) *unroll = ks[variant].unroll;
   if (chunk) *chunk = ks[variant].chunk;
+  if (spare) *spare = ks[variant].spare;
   return KPM_OK;
 }
 
@@ -177,6 +178,7 @@ int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n
     delete p;
     return rc0;
   }
+  occ -= p->shape.spare;
   if (occ < 1) occ = 1;
   const int64_t rows_per_iter = int64_t(p->shape.waves) * 64 / p->shape.lpr;
   const int64_t target = int64_t(cus) * occ * p->shape.rounds;

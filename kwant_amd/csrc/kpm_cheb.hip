@@ -942,13 +942,19 @@ int variants_for(int width, int value_type, KernelShape* out, int max_out) {
     for (int ui = 0; ui < n_unroll; ++ui) {
       const int unroll = ui == 0 ? first : 3 - first;
       for (int rounds : {2, 1, 8, 32, 128}) {
-        if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, rounds, unroll, 0, 0};
+        if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, rounds, unroll, 0, 0, 0};
         ++n;
       }
-      for (int chunk : {4, 8, 32}) {
-        if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, 1, unroll, chunk, 0};
-        ++n;
-      }
+      // The interleaved sweep needs every workgroup resident at once; the
+      // occupancy query can promise one workgroup per CU more than the
+      // dispatcher places (measured on the 2M-site graphene block with 64
+      // vectors: 0.87 ms per step at the reported 7 workgroups per CU, 0.68 ms
+      // at 6), so each chunk also comes with one spare slot per CU.
+      for (int chunk : {8, 32})
+        for (int spare = 0; spare <= 1; ++spare) {
+          if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, 1, unroll, chunk, spare, 0};
+          ++n;
+        }
     }
   };
   if (value_type == KPM_C64) {

@@ -26,6 +26,7 @@ struct KernelShape {
   int unroll; // row slots per wave iteration (scalar-metadata shapes)
   int chunk;  // 0: contiguous rows per workgroup; c > 0: the workgroups of an
               // XCD interleave chunks of c wavefront iterations (rounds 1)
+  int spare;  // resident workgroups per CU left unused by the grid
   int iters;  // row iterations per workgroup (set by the plan)
 };
 
