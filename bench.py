@@ -45,12 +45,14 @@ def bytes_per_step(n, nnz, width, s_v, s_m):
     return 3 * width * n * s_v + nnz * (s_m + 4) + 4 * (n + 1)
 
 
-def read_traffic():
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), if committed."""
+def read_traffic(variant):
+    """Bytes per launch fetched past L2 plus written, measured with
+    rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration) for this launch variant, if committed; else None."""
+    key = "traffic_bytes_per_launch variant %d:" % variant
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -572,7 +574,7 @@ def main():
     if not args.no_surrogate:
         surrogate = surrogate_bench(args.sur_epochs, args.sur_tol, args.sur_points, 32, 32, log)
 
-    traffic = read_traffic()
+    traffic = read_traffic(args.variant)
     line = {
         "metric": "KPM Chebyshev steps/s (% of HBM peak); surrogate speedup at 1e-3 DOS error",
         "value": steps_s,
