@@ -45,6 +45,34 @@ def bytes_per_step(n, nnz, width, s_v, s_m):
     return 3 * width * n * s_v + nnz * (s_m + 4) + 4 * (n + 1)
 
 
+def stream_probe(log_fn):
+    """Achievable HBM rates on this GPU from torch's own kernels on 1 GiB
+    complex64 tensors: a copy (1 read : 1 write) and an add c = a + b
+    (2 reads : 1 write, the recursion's mix), HIP-event timed."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.complex64, device="cuda")
+    b = torch.ones(n, dtype=torch.complex64, device="cuda")
+    c = torch.empty_like(a)
+    out = {}
+    for name, fn, nbytes in (("copy", lambda: c.copy_(a), 2 * 8 * n),
+                             ("add", lambda: torch.add(a, b, out=c), 3 * 8 * n)):
+        fn()
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(10):
+            fn()
+        end.record()
+        torch.cuda.synchronize()
+        out[name] = nbytes * 10 / (start.elapsed_time(end) * 1e-3) / 1e9
+    log_fn("stream probe (torch, 1 GiB): copy %.0f GB/s, add %.0f GB/s" % (out["copy"], out["add"]))
+    del a, b, c
+    torch.cuda.empty_cache()
+    return out
+
+
 def read_traffic(variant):
     """Bytes per launch fetched past L2 plus written, measured with
     rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
@@ -241,7 +269,7 @@ def surrogate_bench(epochs, tol, points, judge_points, judge_sets, log_fn):
     return out
 
 
-def ensemble_bench(count, moments, width, log_fn):
+def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
     """The batched disorder ensemble: ``count`` cubic 4-orbital systems with
     L ~ U{20..60}, generated on the GPU, ``moments`` moments with ``width``
     vectors each, every step one launch over all systems, the step sequence
@@ -251,12 +279,15 @@ def ensemble_bench(count, moments, width, log_fn):
     sizes = ensemble_sizes(count, 20, 60, seed=0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ens = CubicEnsemble(sizes, seed=0, num_moments=moments, num_vectors=width)
+    ens = CubicEnsemble(sizes, seed=0, num_moments=moments, num_vectors=width, variant=variant)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     t_eager = ens.run(graph=False)
-    ens.run(graph=True)                      # capture + first replay
-    t_graph = min(ens.run(graph=True) for _ in range(2))
+    if graph:
+        ens.run(graph=True)                  # capture + first replay
+        t_graph = min(ens.run(graph=True) for _ in range(2))
+    else:
+        t_graph = min(t_eager, ens.run(graph=False))
     bstep = ens.bytes_per_step()
     ms_step = 1e3 * t_graph / ens.steps
     achieved = bstep / (ms_step * 1e-3) / 1e9
@@ -424,6 +455,10 @@ def main():
     ap.add_argument("--cells", type=int, default=1000, help="unit cells per side")
     ap.add_argument("--width", type=int, default=64, help="vectors per block")
     ap.add_argument("--variant", type=int, default=-1, help="kernel launch shape")
+    ap.add_argument("--ens-variant", type=int, default=-1,
+                    help="kernel launch shape of the ensemble (default: autotuned)")
+    ap.add_argument("--ens-eager", action="store_true",
+                    help="time the ensemble with eager launches only (no HIP graph)")
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -435,7 +470,7 @@ def main():
                     help="run only one of the secondary measurements (for profiling)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the moment-surrogate sweep measurement")
-    ap.add_argument("--sur-epochs", type=int, default=6000)
+    ap.add_argument("--sur-epochs", type=int, default=12000)
     ap.add_argument("--sur-tol", type=float, default=1e-3)
     ap.add_argument("--sur-points", type=int, default=4096)
     ap.add_argument("--calibrate", action="store_true",
@@ -448,7 +483,8 @@ def main():
     if args.only:
         fn = {"correlator": lambda: correlator_bench(512, 1024, 32, 2, log, full=False),
               "correlator-full": lambda: correlator_bench(512, 1024, 32, 2, log),
-              "ensemble": lambda: ensemble_bench(256, 512, 16, log),
+              "ensemble": lambda: ensemble_bench(256, 512, 16, log, args.ens_variant,
+                                                 not args.ens_eager),
               "surrogate": lambda: surrogate_bench(args.sur_epochs, args.sur_tol,
                                                    args.sur_points, 32, 32, log)}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
@@ -569,12 +605,13 @@ def main():
         correlator = correlator_bench(512, 1024, 32, 2, log)
     ensemble = None
     if not args.no_ensemble:
-        ensemble = ensemble_bench(256, 512, 16, log)
+        ensemble = ensemble_bench(256, 512, 16, log, args.ens_variant, not args.ens_eager)
     surrogate = None
     if not args.no_surrogate:
         surrogate = surrogate_bench(args.sur_epochs, args.sur_tol, args.sur_points, 32, 32, log)
 
     traffic = read_traffic(args.variant)
+    stream = stream_probe(log)
     line = {
         "metric": "KPM Chebyshev steps/s (% of HBM peak); surrogate speedup at 1e-3 DOS error",
         "value": steps_s,
@@ -599,7 +636,9 @@ def main():
                               "variant": args.variant}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_step": bstep},
+                     "algorithmic_bytes_per_step": bstep,
+                     "measured_stream_GBs": stream,
+                     "frac_of_measured_add": achieved / stream["add"]},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "small_square": small,

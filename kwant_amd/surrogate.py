@@ -362,7 +362,7 @@ class MomentSurrogate:
 
     def fit(self, params, targets, epochs=400, batch=64, lr=3e-3, lr_min=3e-6, seed=0,
             log=None, graph=True, b1=0.9, b2=0.999, ema_decay=None, ema_start=0.5,
-            warmup=0.02, health=10.0):
+            warmup=0.02, health=10.0, collapse=0.5):
         """Train every member on (params, targets) with Adam and a cosine
         learning-rate schedule; members draw their own mini-batch order.
 
@@ -378,7 +378,12 @@ class MomentSurrogate:
         linearly over the first ``warmup`` fraction of the iterations.  After
         training, a member whose training error exceeds ``health`` times the
         members' median is left out of the ensemble statistics (a member
-        stuck in a poor minimum would otherwise set the spread everywhere)."""
+        stuck in a poor minimum would otherwise set the spread everywhere),
+        and so is a member whose error exceeds ``collapse`` times that of the
+        best constant prediction (a collapsed network: when every member
+        collapses to the same constant their spread is zero, so the spread
+        gate alone would accept it).  With fewer than two members left every
+        point falls back to plain KPM."""
         import torch
         dev = self.mlp.params.device
         x = self._inputs(params)
@@ -440,18 +445,26 @@ class MomentSurrogate:
                 log("epoch %d: mse %.3e (last mini-batch)" % (ep, mse))
         if ema is not None:
             self.mlp.params.copy_(ema)
-        self._select_members(x, t, health, log)
+        self._select_members(x, t, health, log, collapse)
 
-    def _select_members(self, x, t, health, log=None):
+    def _select_members(self, x, t, health, log=None, collapse=0.5):
         """Order the members by training error and keep those within
-        ``health`` times the median (at least one)."""
+        ``health`` times the median and within ``collapse`` times the error
+        of the best constant prediction (the targets' variance; not checked
+        when the targets are constant)."""
         import torch
         count = min(len(x), 1024)
         y = self.mlp.forward(x[None, :count])
         err = ((y - t[None, :count]) ** 2).mean(dim=(1, 2)).double().cpu().numpy()
+        tc = t[:count].double()
+        base = float(((tc - tc.mean(dim=0)) ** 2).mean())
         med = float(np.median(err))
         order = np.argsort(err)
-        keep = int(max(1, np.sum(err <= health * med)))
+        healthy = err <= health * med
+        if base > 1e-12:
+            healthy &= err <= collapse * base
+        keep = int(np.sum(healthy))
+        self.constant_error = base
         self.member_errors = err
         self.active = keep
         if keep < self.mlp.members or np.any(order != np.arange(len(order))):
@@ -461,8 +474,8 @@ class MomentSurrogate:
                     for view in (self.mlp.weight(li, buf), self.mlp.bias(li, buf)):
                         view.copy_(view[idx])
         if log is not None:
-            log("members' training MSE %s; %d of %d kept"
-                % (np.array2string(err, precision=2), keep, len(err)))
+            log("members' training MSE %s (constant prediction %.2e); %d of %d kept"
+                % (np.array2string(err, precision=2), base, keep, len(err)))
 
     def predict(self, params, chunk=4096, members=False):
         """Ensemble mean of the damped wide-window moments (P, M) and the
@@ -492,12 +505,18 @@ class MomentSurrogate:
                              out_scale=self._out_scale)
             if per is not None:
                 per[:, s:e].copy_(sh)
+            if self.active == 0:
+                continue
             act = sh[:self.active]
             eng.mlp_ensemble(act, mean=mean[s:e])
+            if self.active < 2:
+                continue
             nd = nodes[:self.active, :e - s]
             eng.mlp_gemm(act, self._l1[None].expand(self.active, k, m), nd,
                          flags=eng.MLP_BF16X3)
             eng.mlp_ensemble(nd, dev=spread[s:e])
+        if self.active == 0:
+            mean.fill_(float("nan"))        # no healthy member: nothing to predict
         if self.active < 2:
             spread.fill_(float("inf"))      # no spread without two healthy members
         if members:

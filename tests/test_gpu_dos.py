@@ -139,7 +139,7 @@ def test_prefetch_depth_never_changes_results():
     r = float(np.max(np.abs(h).sum(axis=1)))
     for prec in ("complex128", "complex64"):
         cdt, vt = eng.complex_dtype(prec)
-        for width in (8, 16):
+        for width in (8, 16, 32, 64):
             gen = torch.Generator(device="cuda")
             gen.manual_seed(width)
             ph = torch.rand((h.shape[0], width), generator=gen, device="cuda",
@@ -147,7 +147,8 @@ def test_prefetch_depth_never_changes_results():
             block = torch.polar(torch.ones_like(ph), 2 * np.pi * ph).to(cdt)
             base = None
             for v in range(eng.variant_count(width, vt)):
-                for hint in (0, 3, 4, 6, 8, 10, 12, 16, 64):
+                hints = (0, 3, 4, 6, 8, 10, 12, 16, 64) if width <= 16 else (0, 4, 10, 64)
+                for hint in hints:
                     mat = eng.DeviceCSR(h, prec)
                     mat.desc.row_nnz_hint = hint
                     rec = eng.BlockRecursion(mat, (1.05 * r, 0.1), block, width,

@@ -371,3 +371,43 @@ def test_failed_member_is_left_out():
     assert np.argmax(sur.member_errors) == 1
     mean, spread = sur.predict(params[:8])
     assert torch.isfinite(spread).all()
+
+
+def test_collapsed_ensemble_falls_back_everywhere():
+    """Members that all collapse to the same constant output agree perfectly
+    (zero spread), so the spread gate alone would accept them; the health
+    check compares each member with the best constant prediction, leaves
+    them all out, and every point then falls back (infinite spread)."""
+    import torch
+    from kwant_amd.surrogate import MomentSurrogate
+
+    class Fam:
+        num_moments = 32
+        a, b = 7.2, 0.0
+        g = np.linspace(1.0, 0.05, 32)
+
+    rng = np.random.RandomState(4)
+    params = rng.uniform([0, 0, -1], [4, 0.02, 1], (256, 3))
+    prof = np.linspace(0.5, 0.0, 32)
+    targets = (params[:, :1] * 0.05 + 0.1) * prof[None, :] + 0.02 * np.sin(
+        params[:, 1:2] * 300.0 + np.arange(32)[None, :])
+    tight = Fam()
+    tight.a = 6.1
+    sur = MomentSurrogate(Fam(), tight, members=4, layers=(2, 32, 32), seed=5)
+    sur.fit(params, targets, epochs=200, batch=32, lr=1e-2)
+    assert sur.active >= 2
+    assert np.all(sur.member_errors[:sur.active] < 0.5 * sur.constant_error)
+    mean, spread = sur.predict(params[:16])
+    assert torch.isfinite(spread).all()
+    # collapse every member onto the mean target: zero weights, output bias = mean
+    for li in range(len(sur.mlp.shapes)):
+        sur.mlp.weight(li).zero_()
+        sur.mlp.bias(li).zero_()
+    last = len(sur.mlp.shapes) - 1
+    sur.mlp.bias(last).copy_(torch.from_numpy(targets.mean(axis=0).astype(np.float32)).cuda()[None])
+    x = sur._inputs(params)
+    t = torch.from_numpy(targets.astype(np.float32)).cuda()
+    sur._select_members(x, t, 10.0)
+    assert sur.active == 0
+    mean, spread = sur.predict(params[:16])
+    assert torch.isinf(spread).all()
