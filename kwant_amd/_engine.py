@@ -287,7 +287,11 @@ class Plan:
             self.handle = None
 
     def set_scale(self, ab):
+        """(a, b) per segment, flattened: 2 * n_segments values."""
         ab = np.ascontiguousarray(np.asarray(ab, dtype=np.float64).reshape(-1))
+        if ab.size != 2 * self.n_segments:
+            raise ValueError("need (a, b) for each of the plan's %d segments, got %d values"
+                             % (self.n_segments, ab.size))
         check(load_library().kpm_plan_set_scale(
             self.handle, ab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
             "kpm_plan_set_scale")
@@ -343,7 +347,10 @@ class BlockRecursion:
         self.mat, self.width, self.mode, self.n_rows = mat, width, mode, n
         self.cdt, self.vt = complex_dtype(mat.precision)
         self.plan = plan if plan is not None else Plan(n, width, self.vt, variant=variant)
-        self.plan.set_scale([ab[0], ab[1]])
+        ab = np.asarray(ab, dtype=np.float64).reshape(-1)
+        if ab.size == 2 and self.plan.n_segments > 1:
+            ab = np.tile(ab, self.plan.n_segments)   # one (a, b) for every segment
+        self.plan.set_scale(ab)
         self.slots = 2 if store_slots is None else int(store_slots)
         if self.slots < 2:
             raise ValueError("a ring needs at least two slots")
