@@ -292,7 +292,8 @@ int kpm_mlp_mse(const float* y, const float* t, float* dy, int64_t n, float scal
  *   dev[p] = (1/width) sum_n (sample standard deviation over members)
  *                                                             (dev may be NULL)
  * With y = densities at Chebyshev nodes, dev is the L1 norm of the ensemble's
- * pointwise standard deviation: the surrogate's spread (fallback gate). */
+ * pointwise standard deviation: the surrogate's spread (fallback gate).  dev
+ * needs members >= 2 (KPM_ERR_ARG otherwise). */
 int kpm_mlp_ensemble(const float* y, int64_t z_stride, int64_t row_stride,
                      int32_t members, int64_t rows, int32_t width, float* mean,
                      int64_t mean_stride, float* dev, kpm_stream_t stream);

@@ -353,6 +353,8 @@ int kpm_mlp_ensemble(const float* y, int64_t z_stride, int64_t row_stride, int32
   if (!y || members <= 0 || rows <= 0 || width <= 0 || row_stride < width ||
       (mean && mean_stride < width) || (!mean && !dev))
     return fail(KPM_ERR_ARG, "bad ensemble arguments");
+  if (dev && members < 2)
+    return fail(KPM_ERR_ARG, "the spread needs at least two members");
   if ((rows + 3) / 4 > (int64_t(1) << 31) - 1) return fail(KPM_ERR_ARG, "too many rows");
   hipLaunchKernelGGL(ensemble_kernel, dim3(unsigned((rows + 3) / 4)), dim3(256), 0,
                      (hipStream_t)stream, y, z_stride, row_stride, members, rows, width, mean,
