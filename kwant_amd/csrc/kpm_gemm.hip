@@ -285,24 +285,35 @@ __global__ __launch_bounds__(256) void corr_gemm_mfma(const void* Av, int64_t ld
 
   const int lr = tid >> 2;        // staged row
   const int lq = (tid & 3) * 4;   // first staged complex k of this thread
-  int64_t since_fold = 0;
-  for (int64_t k0 = kb; k0 < ke; k0 += MKC) {
+  // Register prefetch: the next stage's operands are requested from global
+  // memory before the current stage is multiplied out, so their latency
+  // overlaps the matrix-core work.
+  T pa[4][2], pb[4][2];
+  auto fetch = [&](int64_t k0) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int64_t kc = k0 + lq + e;
-      T ar = T(0), ai = T(0), br = T(0), bi = T(0);
+      pa[e][0] = pa[e][1] = pb[e][0] = pb[e][1] = T(0);
       if (kc < ke) {
         if (m0 + lr < m_rows) {
           const T* p = A + 2 * (int64_t(m0 + lr) * lda + kc);
-          ar = p[0];
-          ai = p[1];
+          pa[e][0] = p[0];
+          pa[e][1] = p[1];
         }
         if (n0 + lr < n_rows) {
           const T* p = B + 2 * (int64_t(n0 + lr) * ldb + kc);
-          br = p[0];
-          bi = p[1];
+          pb[e][0] = p[0];
+          pb[e][1] = p[1];
         }
       }
+    }
+  };
+  int64_t since_fold = 0;
+  if (kb < ke) fetch(kb);
+  for (int64_t k0 = kb; k0 < ke; k0 += MKC) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const T ar = pa[e][0], ai = pa[e][1], br = pb[e][0], bi = pb[e][1];
       const int o = lr * MLD + 2 * (lq + e);
       as_[o] = F(ar);
       as_[o + 1] = F(ai);
@@ -316,6 +327,7 @@ __global__ __launch_bounds__(256) void corr_gemm_mfma(const void* Av, int64_t ld
       }
     }
     __syncthreads();
+    if (k0 + MKC < ke) fetch(k0 + MKC);
 #pragma unroll
     for (int kk = 0; kk < MKR / 4; ++kk) {
       const int kr = kk * 4 + (lane >> 4);
