@@ -33,6 +33,7 @@ namespace {
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4a __attribute__((ext_vector_type(4)));
 typedef short s8v __attribute__((ext_vector_type(8)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 
 constexpr int SB = 64;           // output tile (rows and cols)
 constexpr int SK = 64;           // real k' per LDS stage
@@ -41,23 +42,51 @@ constexpr int FOLD = 256;        // real k' per exact FP32 interval
 constexpr int kMaxSlices = 4;
 
 // e[r] such that max |x| over the panel's 2 kn components of row r is < 2^e
-// (0 for an all-zero row).  One wavefront per row.
+// (0 for an all-zero row), in three launches: clear, a grid of row spans
+// that each fold their maximum's exponent in with an integer atomicMax
+// (order-independent), and the decode of the biased exponent.  A row holds
+// up to 2^21 components, so one workgroup per row would leave most CUs idle.
+constexpr int kExpBias = 1100;     // > -(smallest double exponent): biased values > 0
+constexpr int kExpSpan = 16384;    // components per workgroup
+
+__global__ __launch_bounds__(256) void exponent_clear(int* __restrict__ e, int rows) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) e[r] = 0;
+}
+
 __global__ __launch_bounds__(256) void row_exponent_kernel(const double* __restrict__ X,
                                                            int64_t ld, int rows, int64_t k0,
                                                            int64_t kn, int* __restrict__ e) {
-  const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int r = blockIdx.y;
   if (r >= rows) return;
   const double* x = X + 2 * (int64_t(r) * ld + k0);
+  const int64_t n2 = 2 * kn;
+  const int64_t begin = int64_t(blockIdx.x) * kExpSpan;
+  const int64_t end = begin + kExpSpan < n2 ? begin + kExpSpan : n2;
   double m = 0.0;
-  for (int64_t i = lane; i < 2 * kn; i += 64) m = fmax(m, fabs(x[i]));
+  // (re, im) pairs: 16-byte aligned, n2 and begin are even
+  for (int64_t i = begin + 2 * threadIdx.x; i < end; i += 2 * blockDim.x) {
+    const d2v v = *reinterpret_cast<const d2v*>(x + i);
+    m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-  if (lane == 0) {
-    int ex = 0;
-    if (m > 0.0) frexp(m, &ex);
-    e[r] = ex;
+  __shared__ double wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    if (m > 0.0) {
+      int ex = 0;
+      frexp(m, &ex);
+      atomicMax(e + r, ex + kExpBias);
+    }
   }
+}
+
+__global__ __launch_bounds__(256) void exponent_decode(int* __restrict__ e, int rows) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows && e[r] != 0) e[r] -= kExpBias;
 }
 
 // S slices of every component of the panel, rows padded to kp2 with zeros:
@@ -288,6 +317,8 @@ int kpm_corr_gemm_sliced_work(int32_t m_rows, int32_t n_rows, int64_t k, int32_t
                               int64_t* bytes) {
   if (!bytes || m_rows <= 0 || n_rows <= 0 || k <= 0 || slices < 1 || slices > kMaxSlices)
     return fail(KPM_ERR_ARG, "bad sliced GEMM shape");
+  if (m_rows > 65535 || n_rows > 65535)
+    return fail(KPM_ERR_ARG, "at most 65535 rows per operand");
   const SlicedPlan p = sliced_plan(m_rows, n_rows, k, slices);
   *bytes = p.slice_bytes + p.exp_bytes + p.part_bytes + 256;
   return KPM_OK;
@@ -298,6 +329,8 @@ int kpm_corr_gemm_sliced(const void* A, int64_t lda, const void* B, int64_t ldb,
                          int32_t slices, void* work, int64_t work_bytes, kpm_stream_t stream) {
   if (!A || !B || !mu || !work) return fail(KPM_ERR_ARG, "NULL buffer");
   if (m_rows <= 0 || n_rows <= 0 || k <= 0) return fail(KPM_ERR_ARG, "empty GEMM");
+  if (m_rows > 65535 || n_rows > 65535)
+    return fail(KPM_ERR_ARG, "at most 65535 rows per operand");
   if (slices < 1 || slices > kMaxSlices) return fail(KPM_ERR_ARG, "slices must be 1..4");
   if (lda < k || ldb < k || ldc < n_rows) return fail(KPM_ERR_ARG, "leading dimension too small");
   const SlicedPlan p = sliced_plan(m_rows, n_rows, k, slices);
@@ -315,10 +348,15 @@ int kpm_corr_gemm_sliced(const void* A, int64_t lda, const void* B, int64_t ldb,
   int panel_no = 0;
   for (int64_t k0 = 0; k0 < k; k0 += p.panel, ++panel_no) {
     const int64_t kn = k - k0 < p.panel ? k - k0 : p.panel;
-    hipLaunchKernelGGL(row_exponent_kernel, dim3((m_rows + 3) / 4), dim3(256), 0, s, a, lda,
+    const unsigned spans = unsigned((2 * kn + kExpSpan - 1) / kExpSpan);
+    hipLaunchKernelGGL(exponent_clear, dim3((m_rows + 255) / 256), dim3(256), 0, s, ea, m_rows);
+    hipLaunchKernelGGL(exponent_clear, dim3((n_rows + 255) / 256), dim3(256), 0, s, eb, n_rows);
+    hipLaunchKernelGGL(row_exponent_kernel, dim3(spans, m_rows), dim3(256), 0, s, a, lda,
                        m_rows, k0, kn, ea);
-    hipLaunchKernelGGL(row_exponent_kernel, dim3((n_rows + 3) / 4), dim3(256), 0, s, b, ldb,
+    hipLaunchKernelGGL(row_exponent_kernel, dim3(spans, n_rows), dim3(256), 0, s, b, ldb,
                        n_rows, k0, kn, eb);
+    hipLaunchKernelGGL(exponent_decode, dim3((m_rows + 255) / 256), dim3(256), 0, s, ea, m_rows);
+    hipLaunchKernelGGL(exponent_decode, dim3((n_rows + 255) / 256), dim3(256), 0, s, eb, n_rows);
     const dim3 sg(unsigned((p.kp2 + 255) / 256));
     hipLaunchKernelGGL(slice_kernel, dim3(sg.x, m_rows), dim3(256), 0, s, a, lda, m_rows, k0,
                        kn, ea, int(slices), As, p.kp2);
