@@ -212,12 +212,23 @@ class DeviceCSR:
     """A CSR matrix resident on the GPU (int32 indices, complex values).
 
     Keeps kwant's CSR exactly as given (entry order within rows included),
-    so the SpMM sums each row in the order scipy's csr_matvec does."""
+    so the SpMM sums each row in the order scipy's csr_matvec does.  With
+    ``order`` (kwant_amd.ordering) the rows and columns are stored permuted,
+    P H P^T with device row i = caller row order[i]; vectors cross the
+    boundary through rows_in / rows_out, and every row sum stays the same
+    sequence of operations."""
 
-    def __init__(self, matrix, precision="c128"):
+    def __init__(self, matrix, precision="c128", order=None):
         import torch
+        from .ordering import permute_csr
         dev = torch_device()
         csr = scipy.sparse.csr_matrix(matrix)
+        self.order = self.inverse = None
+        if order is not None:
+            csr = permute_csr(csr, order)
+            self.order = np.asarray(order, dtype=np.int64)
+            self.inverse = np.empty_like(self.order)
+            self.inverse[self.order] = np.arange(len(self.order))
         if csr.nnz >= 2 ** 31 or csr.shape[0] >= 2 ** 31:
             raise ValueError("matrix too large for int32 CSR indices")
         self.shape = csr.shape
@@ -238,10 +249,19 @@ class DeviceCSR:
                            self.values.data_ptr(), self.value_type,
                            row_nnz_hint(np.diff(csr.indptr)))
 
+    def rows_in(self, host):
+        """Caller-order rows (axis 0) of a host array, in device order."""
+        return host if self.order is None else np.asarray(host)[self.order]
+
+    def rows_out(self, host):
+        """Device-order rows (axis 0) of a host array, in caller order."""
+        return host if self.order is None else np.asarray(host)[self.inverse]
+
     @classmethod
     def from_device_arrays(cls, shape, indptr, indices, values, precision):
         """Wrap CSR arrays that already live on the GPU (generated there)."""
         self = cls.__new__(cls)
+        self.order = self.inverse = None
         cdt, self.value_type = complex_dtype(precision)
         if values.dtype != cdt:
             raise ValueError("values dtype does not match precision")
@@ -395,8 +415,8 @@ class BlockRecursion:
 
     def last_two(self, col):
         """(a_{n-1}, a_n) of column ``col`` as host complex128 arrays."""
-        return (self.slot(self.n - 1)[:, col].cpu().numpy().astype(np.complex128),
-                self.slot(self.n)[:, col].cpu().numpy().astype(np.complex128))
+        return tuple(self.mat.rows_out(self.slot(k)[:, col].cpu().numpy().astype(np.complex128))
+                     for k in (self.n - 1, self.n))
 
 
 def corr_gemm(a, b, mu, precision, k_chunk=256, work=None):

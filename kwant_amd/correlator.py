@@ -23,6 +23,7 @@ from scipy.integrate import simpson
 from scipy.sparse import coo_matrix, csr_matrix
 
 from . import _engine as eng
+from . import ordering
 from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _is_local_operator,
                   _is_system, _rescale, _scale_from_bounds, ensure_rng, fermi_distribution,
                   jackson_kernel, SAMPLING, _chebyshev_nodes)
@@ -54,7 +55,7 @@ class _Expansion:
     def __init__(self, hamiltonian, params=None, num_vectors=10, num_moments=None,
                  energy_resolution=None, vector_factory=None, bounds=None, eps=0.05,
                  rng=None, kernel=None, mean=True, accumulate_vectors=True,
-                 precision='complex128'):
+                 precision='complex128', reorder='auto'):
         if num_moments and energy_resolution:
             raise TypeError("either 'num_moments' or 'energy_resolution' must be provided.")
         self.eps = eps
@@ -64,7 +65,9 @@ class _Expansion:
         self._v0 = np.exp(2j * np.pi * rng.random_sample(ham.shape[0]))
         if eps <= 0:
             raise ValueError("'eps' must be positive")
-        self.hamiltonian, (self._a, self._b) = _rescale(ham, eps, self._v0, bounds, precision)
+        order = ordering.choose_order(ham, reorder)
+        self.hamiltonian, (self._a, self._b) = _rescale(ham, eps, self._v0, bounds, precision,
+                                                        order)
         self.dmat = self.hamiltonian._dmat
         if energy_resolution:
             num_moments = math.ceil((1.6 * self._a) / energy_resolution)
@@ -189,7 +192,8 @@ class Correlator:
         if kind != 'matrix':
             return None
         if key not in self._dev_ops:
-            self._dev_ops[key] = eng.DeviceCSR(payload, self._spectrum_R.dmat.precision)
+            dmat = self._spectrum_R.dmat
+            self._dev_ops[key] = eng.DeviceCSR(payload, dmat.precision, order=dmat.order)
         return self._dev_ops[key]
 
     def _apply(self, key, op, src, dst, width, n_slots):
@@ -204,11 +208,15 @@ class Correlator:
                 src, dst, width, n_slots=n_slots, x_stride=src[0].numel(),
                 y_stride=dst[0].numel())
         else:
+            dmat = self._spectrum_R.dmat
             host = src.cpu().numpy().astype(np.complex128)
             out = np.empty_like(host)
             for s in range(n_slots):
+                caller = dmat.rows_out(host[s])
+                res = np.empty_like(caller)
                 for c in range(width):
-                    out[s, :, c] = payload(host[s, :, c])
+                    res[:, c] = payload(caller[:, c])
+                out[s] = dmat.rows_in(res)
             dst.copy_(torch.from_numpy(out).to(dst.device).to(dst.dtype))
 
     def _add_vectors(self, count):
@@ -227,7 +235,7 @@ class Correlator:
             vecs = [factory[r] for r in range(start, start + width)]
             start += width
             host = np.stack([np.asarray(v) for v in vecs], axis=1).astype(np.complex128)
-            block = torch.from_numpy(np.ascontiguousarray(host)).to(dev).to(cdt)
+            block = torch.from_numpy(np.ascontiguousarray(dmat.rows_in(host))).to(dev).to(cdt)
             # right expansion: all T_n(H~)|r>
             rec_r = eng.BlockRecursion(dmat, (self._a, self._b), block, width,
                                        eng.DOTS_NONE, store_slots=m_count)

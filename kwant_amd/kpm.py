@@ -20,8 +20,10 @@ and exceptions.  What differs is where the work runs:
 
 Extra keyword arguments (all optional): ``precision`` ('complex128' |
 'complex64'), ``max_block_width`` (power of two <= 64), ``variant`` (kernel
-launch shape, see kwant_amd.autotune), ``gemm_precision`` for the
-Correlator ('fp64' | 'fp32' | 'fp32x3').
+launch shape, see kwant_amd.autotune), ``reorder`` (True | False | 'auto' |
+a permutation: the device row order, see kwant_amd.ordering; the moments do
+not depend on it), ``gemm_precision`` for the Correlator ('fp64' | 'fp32' |
+'fp32x3' | 'sliced3' | 'sliced4').
 """
 import math
 import numbers
@@ -37,6 +39,7 @@ from scipy.sparse.linalg import LinearOperator
 
 from . import _engine as eng
 from . import autotune
+from . import ordering
 from .lanczos import lanczos_bounds
 
 __all__ = ['SpectralDensity', 'Correlator', 'conductivity',
@@ -270,24 +273,25 @@ class _RescaledOperator(LinearOperator):
         import torch
         dev = eng.torch_device()
         cdt, _ = eng.complex_dtype(self._dmat.precision)
-        x = torch.from_numpy(np.ascontiguousarray(np.asarray(v).reshape(-1),
-                                                  dtype=np.complex128)).to(dev).to(cdt)
+        v = self._dmat.rows_in(np.asarray(v).reshape(-1))
+        x = torch.from_numpy(np.ascontiguousarray(v, dtype=np.complex128)).to(dev).to(cdt)
         y = torch.empty_like(x)
         self._dmat.spmm(x, y, 1)
         y = (y - self._b * x) / self._a
-        return y.cpu().numpy().astype(np.complex128)
+        return self._dmat.rows_out(y.cpu().numpy().astype(np.complex128))
 
 
-def _rescale(hamiltonian, eps, v0, bounds, precision='complex128'):
-    """Upload H and return (GPU rescaled operator, (a, b)) as
-    kwant/kpm.py:1145-1187; bounds=None runs FP64 Lanczos on the GPU."""
-    dmat = eng.DeviceCSR(hamiltonian, precision)
+def _rescale(hamiltonian, eps, v0, bounds, precision='complex128', order=None):
+    """Upload H (rows in ``order``, see kwant_amd.ordering) and return (GPU
+    rescaled operator, (a, b)) as kwant/kpm.py:1145-1187; bounds=None runs
+    FP64 Lanczos on the GPU."""
+    dmat = eng.DeviceCSR(hamiltonian, precision, order=order)
     if bounds:
         lmin, lmax = bounds
     else:
         dmat64 = dmat if dmat.precision in ('complex128', 'c128') else \
-            eng.DeviceCSR(hamiltonian, 'complex128')
-        lmin, lmax = lanczos_bounds(dmat64, v0, tol=eps / 2)
+            eng.DeviceCSR(hamiltonian, 'complex128', order=order)
+        lmin, lmax = lanczos_bounds(dmat64, dmat64.rows_in(v0), tol=eps / 2)
     a, b = _scale_from_bounds(lmin, lmax, eps)
     return _RescaledOperator(dmat, a, b), (a, b)
 
@@ -303,7 +307,7 @@ class _DosBlock:
         dev = eng.torch_device()
         cdt, _ = eng.complex_dtype(dmat.precision)
         host = np.stack([np.asarray(v) for v in vectors], axis=1).astype(np.complex128)
-        block = torch.from_numpy(np.ascontiguousarray(host)).to(dev).to(cdt)
+        block = torch.from_numpy(np.ascontiguousarray(dmat.rows_in(host))).to(dev).to(cdt)
         self.count = len(vectors)
         self.rec = eng.BlockRecursion(dmat, ab, block, width, mode, w=w, variant=variant)
         self.moments = None  # (count, M) complex128
@@ -330,7 +334,7 @@ class SpectralDensity:
                  num_vectors=10, num_moments=None, energy_resolution=None,
                  vector_factory=None, bounds=None, eps=0.05, rng=None,
                  kernel=None, mean=True, accumulate_vectors=True, *,
-                 precision='complex128', max_block_width=64, variant=-1):
+                 precision='complex128', max_block_width=64, variant=-1, reorder='auto'):
         if num_moments and energy_resolution:
             raise TypeError("either 'num_moments' or 'energy_resolution' "
                             "must be provided.")
@@ -363,8 +367,9 @@ class SpectralDensity:
         self._precision = precision
         self._max_width = max_block_width
         self._variant = variant
+        order = ordering.choose_order(hamiltonian, reorder)
         self.hamiltonian, (self._a, self._b) = _rescale(
-            hamiltonian, self.eps, self._v0, bounds, precision)
+            hamiltonian, self.eps, self._v0, bounds, precision, order)
         self._dmat = self.hamiltonian._dmat
         self.bounds = (self._b - self._a, self._b + self._a)
 
@@ -537,7 +542,8 @@ class SpectralDensity:
             blk = _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_PROJECT,
                             variant=self._launch_shape(width, eng.DOTS_PROJECT))
             # w = O^H alpha_0, so <w|alpha_n> = vdot(alpha_0, O alpha_n)
-            op_h = eng.DeviceCSR(self._op_matrix.conj().T.tocsr(), self._dmat.precision)
+            op_h = eng.DeviceCSR(self._op_matrix.conj().T.tocsr(), self._dmat.precision,
+                                 order=self._dmat.order)
             w = torch.empty_like(blk.rec.ring[0])
             op_h.spmm(blk.rec.ring[0], w, width)
             blk.rec.w = w
@@ -595,19 +601,19 @@ class SpectralDensity:
         recursion runs on the GPU; each new alpha_n is handed to the callable
         on the host, one moment per step."""
         rec = blk.rec
-        alpha0 = rec.slot(0).cpu().numpy().astype(np.complex128)
+        alpha0 = self._dmat.rows_out(rec.slot(0).cpu().numpy().astype(np.complex128))
         moms = [list(old[c]) if old is not None else [] for c in range(blk.count)]
         if m_old == 0:
             alpha1 = None
             rec.step(1, want_dots=False)
-            alpha1 = rec.slot(1).cpu().numpy().astype(np.complex128)
+            alpha1 = self._dmat.rows_out(rec.slot(1).cpu().numpy().astype(np.complex128))
             for c in range(blk.count):
                 moms[c].append(self.operator(alpha0[:, c], alpha0[:, c]))
                 moms[c].append(self.operator(alpha0[:, c], alpha1[:, c]))
             m_old = 2
         for _ in range(m_old, m_new):
             rec.step(1, want_dots=False)
-            cur = rec.slot(rec.n).cpu().numpy().astype(np.complex128)
+            cur = self._dmat.rows_out(rec.slot(rec.n).cpu().numpy().astype(np.complex128))
             for c in range(blk.count):
                 moms[c].append(self.operator(alpha0[:, c], cur[:, c]))
         return moms

@@ -177,3 +177,20 @@ def test_sliced_gemm_panels_and_accumulate():
     eng.corr_gemm_sliced(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mu, slices=4)
     want = mu0 + _cgemm_ref(a, b)
     assert np.max(np.abs(mu.cpu().numpy() - want)) < 1e-8 * np.sqrt(k) * 16
+
+
+def test_conductivity_reordered_equals_caller_order():
+    """A shuffled square flake: conductivity with the rows stored in RCM
+    order (velocity operators permuted alike) equals the caller-order run."""
+    from kwant_amd import ordering
+    from kwant_amd.lattices import square
+    h0, pos0 = square(10, flux=0.05, disorder=0.5, seed=1)
+    order = np.random.RandomState(7).permutation(h0.shape[0])
+    h = ordering.permute_csr(h0, order)
+    pos = pos0[order]
+    kw = dict(num_moments=30, num_vectors=4, bounds=(-4.5, 4.5), rng=0)
+    a = kpm().conductivity(h, 'x', 'y', positions=pos, reorder=False, **kw)
+    b = kpm().conductivity(h, 'x', 'y', positions=pos, reorder=True, **kw)
+    assert b._spectrum_R.dmat.order is not None
+    for mu in (-1.0, 0.0, 1.5):
+        assert abs(a(mu, 0.05) - b(mu, 0.05)) < 1e-10 * (1 + abs(a(mu, 0.05)))

@@ -405,3 +405,33 @@ def test_ensemble_segments_equal_separate_runs():
                 got += [2 * d[n, s, c, 0] - mu0, 2 * (d[n, s, c, 2] + 1j * d[n, s, c, 3]) - mu1]
             assert rel(got, mom, x.shape[0]) < 1e-11, (s, c)
 
+
+
+def test_reorder_leaves_moments_unchanged():
+    """A shuffled lattice expanded in the caller's order and in RCM order
+    (kwant_amd.ordering): the device stores P H P^T and the vectors permuted
+    alike, so the moments are bitwise the same; so are the callable-operator
+    moments and the rescaled operator's products, which cross the boundary
+    in the caller's order."""
+    from kwant_amd import ordering
+    from kwant_amd.lattices import honeycomb
+    h0, _ = honeycomb(40, disorder=0.5, seed=2)
+    n = h0.shape[0]
+    h = ordering.permute_csr(h0, np.random.RandomState(5).permutation(n))
+    bounds = gershgorin(h)
+    k = kpm()
+    for prec in ('complex128', 'complex64'):
+        kw = dict(num_moments=80, num_vectors=8, bounds=bounds, rng=2, precision=prec)
+        plain = k.SpectralDensity(h, reorder=False, **kw)
+        rcm = k.SpectralDensity(h, reorder=True, **kw)
+        assert rcm._dmat.order is not None and plain._dmat.order is None
+        assert np.array_equal(np.array(plain._moments_list), np.array(rcm._moments_list))
+        v = np.random.RandomState(3).randn(n) + 0j
+        assert np.allclose(plain.hamiltonian @ v, rcm.hamiltonian @ v, rtol=0, atol=1e-6)
+    # callable operator: vectors handed to it in the caller's order
+    weights = np.random.RandomState(4).rand(n)
+    op = lambda bra, ket: np.vdot(bra, weights * ket)
+    kw = dict(num_moments=20, num_vectors=2, bounds=bounds, rng=2, operator=op)
+    a = np.array(k.SpectralDensity(h, reorder=False, **kw)._moments_list)
+    b = np.array(k.SpectralDensity(h, reorder=True, **kw)._moments_list)
+    assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
