@@ -279,6 +279,40 @@ def surrogate_bench(epochs, tol, points, judge_points, judge_sets, log_fn):
     return out
 
 
+def density_bench(mat, ab, block, width, variant, dos_ms, steps, log_fn):
+    """Site-resolved density (kwant.operator.Density, sum=False, mean=True)
+    on the same block: each step also reads w = M^H a_0 (here M = 1, so w =
+    a_0) and writes one complex128 per row, the sum over the block's
+    vectors of conj(w) a_{n+1}.  Kernel time only; the per-site sums and
+    the copy to the host are outside the timed region."""
+    import torch
+    from kwant_amd import _engine as eng
+    n, nnz = mat.shape[0], mat.nnz
+    w = block.clone()
+    rec = eng.BlockRecursion(mat, ab, block, width, eng.DOTS_LOCAL, w=w, variant=variant)
+    chunk = 64
+    rec.local_steps(chunk)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps // chunk):
+        rec.local_steps(chunk)
+    end.record()
+    torch.cuda.synchronize()
+    ms_step = start.elapsed_time(end) / (chunk * (steps // chunk))
+    bstep = bytes_per_step(n, nnz, width, 8, 8) + n * width * 8 + n * 16
+    gbs = bstep / (ms_step * 1e-3) / 1e9
+    log_fn("site-resolved density: %.4f ms/step (DOS step %.4f ms), %.0f GB/s algorithmic "
+           "(%.1f%% of 8 TB/s)" % (ms_step, dos_ms, gbs, 100 * gbs / HBM_PEAK_GBS))
+    del rec, w
+    torch.cuda.empty_cache()
+    return {"mode": "KPM_DOTS_LOCAL (per-row sum over the 64 vectors of conj(w) a_n+1)",
+            "ms_per_step": ms_step, "dos_ms_per_step": dos_ms,
+            "algorithmic_bytes_per_step": bstep, "achieved_GBs": gbs,
+            "frac_of_hbm_peak": gbs / HBM_PEAK_GBS, "steps": chunk * (steps // chunk),
+            "variant": variant}
+
+
 def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
     """The batched disorder ensemble: ``count`` cubic 4-orbital systems with
     L ~ U{20..60}, generated on the GPU, ``moments`` moments with ``width``
@@ -478,6 +512,8 @@ def main():
                     help="skip the Kubo-Bastin Gamma_nm measurement")
     ap.add_argument("--only", choices=["correlator", "correlator-full", "ensemble", "surrogate"],
                     help="run only one of the secondary measurements (for profiling)")
+    ap.add_argument("--no-density", action="store_true",
+                    help="skip the site-resolved density step measurement")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the moment-surrogate sweep measurement")
     ap.add_argument("--sur-epochs", type=int, default=9600)
@@ -583,9 +619,10 @@ def main():
         log("c64 vs c128 moments: max |d mu| / mu0 = %.3e; c128 %.0f steps/s"
             % (err, full / t128))
 
-    rec = eng.BlockRecursion(m64, (a, b), block.to(torch.complex64), width,
-                             eng.DOTS_DOUBLING, variant=args.variant)
+    block64 = block.to(torch.complex64)
     del block
+    rec = eng.BlockRecursion(m64, (a, b), block64, width, eng.DOTS_DOUBLING,
+                             variant=args.variant)
     plan = rec.plan
     dots = torch.empty((args.warmup + m_steps, 1, width, 4), dtype=torch.float64, device=dev)
     stride = n * width
@@ -607,8 +644,13 @@ def main():
     log("%.4f ms/step, %.1f steps/s, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
         % (ms_step, steps_s, achieved, 100 * achieved / HBM_PEAK_GBS))
 
-
-    del rec, plan, dots, m64
+    del rec, plan, dots
+    torch.cuda.empty_cache()
+    site_density = None
+    if not args.no_density:
+        site_density = density_bench(m64, (a, b), block64, width, args.variant, ms_step, 256,
+                                     log)
+    del m64, block64
     torch.cuda.empty_cache()
     correlator = None
     if not args.no_correlator:
@@ -654,6 +696,7 @@ def main():
         "small_square": small,
         "correlator": correlator,
         "ensemble": ensemble,
+        "site_density": site_density,
         "surrogate": surrogate,
     }
     line.update(check)

@@ -41,6 +41,8 @@ extern "C" {
 #define KPM_DOTS_NONE 0     /* recursion only (Correlator expansions)         */
 #define KPM_DOTS_DOUBLING 1 /* <a_n|a_n>, Re <a_{n+1}|a_n> per column         */
 #define KPM_DOTS_PROJECT 2  /* <w|a_n>, <w|a_{n+1}> per column (operator path) */
+#define KPM_DOTS_LOCAL 3    /* per row: sum_c conj(w[r,c]) a_{n+1}[r,c]       */
+#define KPM_DOTS_LOCAL_COLS 4 /* per row and column: conj(w[r,c]) a_{n+1}[r,c] */
 
 typedef void* kpm_stream_t;
 
@@ -135,9 +137,23 @@ int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream);
  * { <w|a_n>, <w|a_{n+1}> } with w the device block `w` (the matrix-operator
  * moments vdot(a_0, O a_n) of kwant/kpm.py:193-195, 484 with w = O^H a_0).
  * Sums are FP64 and deterministic (fixed order), so repeated runs and
- * resumed runs are bitwise identical.  The call fails with KPM_ERR_ARG
- * (nothing launched) when a step's slot (n - out_base) would fall outside
- * [0, out_cap). */
+ * resumed runs are bitwise identical.
+ *
+ * The site-resolved operator path (kwant.operator.Density with sum=False,
+ * kwant/operator.pyx:771-801, evaluated as op(a_0, a_{n+1}) by
+ * kwant/kpm.py:484) writes per matrix row instead of per column, with
+ * w = M^H a_0 for the block-diagonal onsite matrix M, as complex128
+ * (re, im) pairs and without workgroup partials:
+ *   KPM_DOTS_LOCAL:      d_dots[((n - out_base) * n_rows + r) * 2 + {0, 1}]
+ *                        = sum_c conj(w[r,c]) a_{n+1}[r,c]  (the block's
+ *                        vectors summed, for mean=True; a TwoSum sum
+ *                        rounded once)
+ *   KPM_DOTS_LOCAL_COLS: d_dots[(((n - out_base) * n_rows + r) * width + c) * 2
+ *                        + {0, 1}] = conj(w[r,c]) a_{n+1}[r,c]  (mean=False)
+ * The caller sums the rows of each site.
+ *
+ * The call fails with KPM_ERR_ARG (nothing launched) when a step's slot
+ * (n - out_base) would fall outside [0, out_cap). */
 int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring,
                    int64_t slot_stride, int32_t slots, const void* w,
                    double* d_dots, int32_t out_base, int32_t out_cap,
@@ -146,7 +162,9 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring,
 /* Dots of one block without a matrix product: per segment and column,
  * DOUBLING: { <x|x>, 0, 0, 0 }; PROJECT: { <w|x>, 0, 0 }.  Written to
  * d_out[s * width + c] (4 doubles).  This is the odd-moment tail
- * vdot(alpha_next, alpha_next) of kwant/kpm.py:474-477. */
+ * vdot(alpha_next, alpha_next) of kwant/kpm.py:474-477.  LOCAL and
+ * LOCAL_COLS write the per-row record of kpm_cheb_steps for x in place of
+ * a_{n+1} (the first site-resolved moment op(a_0, a_0), kwant/kpm.py:447). */
 int kpm_block_dots(kpm_plan* plan, const void* x, const void* w,
                    double* d_out, int32_t dots_mode, kpm_stream_t stream);
 

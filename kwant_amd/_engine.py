@@ -21,6 +21,8 @@ KPM_C128 = 2
 DOTS_NONE = 0
 DOTS_DOUBLING = 1
 DOTS_PROJECT = 2
+DOTS_LOCAL = 3
+DOTS_LOCAL_COLS = 4
 GEMM_FP64 = 0
 GEMM_FP32 = 1
 GEMM_FP32X3 = 2
@@ -402,6 +404,34 @@ class BlockRecursion:
         if dots is None:
             return None
         return dots[:, 0].cpu().numpy()
+
+    def _local_shape(self):
+        if self.mode not in (DOTS_LOCAL, DOTS_LOCAL_COLS):
+            raise ValueError("not a site-resolved recursion")
+        return (self.n_rows, self.width) if self.mode == DOTS_LOCAL_COLS else (self.n_rows,)
+
+    def local_steps(self, n_steps):
+        """Advance ``n_steps`` in a site-resolved mode; returns the device
+        records <w|a_{n+1}> per row (device row order) of every step:
+        (n_steps, N) complex128 for DOTS_LOCAL (summed over the block's
+        columns), (n_steps, N, W) for DOTS_LOCAL_COLS."""
+        import torch
+        shape = self._local_shape()
+        if self.n + n_steps >= self.slots and self.slots > 2:
+            raise ValueError("ring too small for the requested moments")
+        out = torch.empty((n_steps,) + shape, dtype=torch.complex128, device=self.ring.device)
+        if n_steps > 0:
+            self.plan.steps(self.mat, self.ring, self.slot_stride, self.slots, n_steps,
+                            self.mode, dots=out, out_base=self.n, w=self.w)
+            self.n += n_steps
+        return out
+
+    def local_now(self):
+        """The site-resolved record of the current a_n alone (no step)."""
+        import torch
+        out = torch.empty(self._local_shape(), dtype=torch.complex128, device=self.ring.device)
+        self.plan.block_dots(self.slot(self.n), out, self.mode, w=self.w)
+        return out
 
     def slot(self, k):
         return self.ring[k % self.slots]

@@ -100,7 +100,7 @@ extern "C" {
 
 const char* kpm_last_error(void) { return g_last_error.c_str(); }
 
-int kpm_abi_version(void) { return 103; }
+int kpm_abi_version(void) { return 104; }
 
 int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch) {
   hipDeviceProp_t prop;
@@ -312,11 +312,13 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
   if (slots < 2) return fail(KPM_ERR_ARG, "ring needs at least two slots");
   if (slot_stride < int64_t(plan->n_rows) * plan->width)
     return fail(KPM_ERR_ARG, "slot_stride smaller than n_rows * width");
-  if (dots_mode < KPM_DOTS_NONE || dots_mode > KPM_DOTS_PROJECT)
+  if (dots_mode < KPM_DOTS_NONE || dots_mode > KPM_DOTS_LOCAL_COLS)
     return fail(KPM_ERR_ARG, "unknown dots_mode");
   if (dots_mode != KPM_DOTS_NONE && (!d_dots || out_cap <= 0))
     return fail(KPM_ERR_ARG, "dots output buffer missing");
-  if (dots_mode == KPM_DOTS_PROJECT && !w) return fail(KPM_ERR_ARG, "projection vector missing");
+  const bool local = dots_mode == KPM_DOTS_LOCAL || dots_mode == KPM_DOTS_LOCAL_COLS;
+  if ((dots_mode == KPM_DOTS_PROJECT || local) && !w)
+    return fail(KPM_ERR_ARG, "projection vector missing");
   if (n_steps < 0) return fail(KPM_ERR_ARG, "n_steps must be non-negative");
   if (dots_mode != KPM_DOTS_NONE &&
       (plan->step - out_base < 0 || plan->step + n_steps - out_base > out_cap))
@@ -336,12 +338,17 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
   // standalone launch.
   const int64_t rec = int64_t(plan->nseg) * plan->width * 4;
   const int64_t half = plan->part_half;
+  // site-resolved records: one complex128 per row (or per row and column)
+  const int64_t local_rec =
+      int64_t(plan->n_rows) * 2 * (dots_mode == KPM_DOTS_LOCAL_COLS ? plan->width : 1);
+  p.local_cols = dots_mode == KPM_DOTS_LOCAL_COLS;
   for (int i = 0; i < n_steps; ++i) {
     const int n = plan->step;
     p.part = plan->part + (n & 1) * half;
     p.prev_part = nullptr;
     p.prev_dots = nullptr;
-    if (dots_mode != KPM_DOTS_NONE && i > 0) {
+    p.local = local ? d_dots + int64_t(n - out_base) * local_rec : nullptr;
+    if (dots_mode != KPM_DOTS_NONE && !local && i > 0) {
       p.prev_part = plan->part + ((n - 1) & 1) * half;
       p.prev_dots = d_dots + int64_t(n - 1 - out_base) * rec;
     }
@@ -350,7 +357,7 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
     if (rc) return rc;
     plan->step = n + 1;
   }
-  if (dots_mode != KPM_DOTS_NONE && n_steps > 0) {
+  if (dots_mode != KPM_DOTS_NONE && !local && n_steps > 0) {
     const int last = plan->step - 1;
     p.part = plan->part + (last & 1) * half;
     p.dots = d_dots + int64_t(last - out_base) * rec;
@@ -363,9 +370,11 @@ int kpm_block_dots(kpm_plan* plan, const void* x, const void* w, double* d_out,
                    int32_t dots_mode, kpm_stream_t stream) {
   if (!plan) return fail(KPM_ERR_ARG, "plan is NULL");
   if (!x || !d_out) return fail(KPM_ERR_ARG, "NULL buffer");
-  if (dots_mode != KPM_DOTS_DOUBLING && dots_mode != KPM_DOTS_PROJECT)
-    return fail(KPM_ERR_ARG, "dots_mode must be DOUBLING or PROJECT");
-  if (dots_mode == KPM_DOTS_PROJECT && !w) return fail(KPM_ERR_ARG, "projection vector missing");
+  if (dots_mode < KPM_DOTS_DOUBLING || dots_mode > KPM_DOTS_LOCAL_COLS)
+    return fail(KPM_ERR_ARG, "dots_mode must be DOUBLING, PROJECT, LOCAL or LOCAL_COLS");
+  const bool local = dots_mode == KPM_DOTS_LOCAL || dots_mode == KPM_DOTS_LOCAL_COLS;
+  if ((dots_mode == KPM_DOTS_PROJECT || local) && !w)
+    return fail(KPM_ERR_ARG, "projection vector missing");
   if (!plan->scale_set) return fail(KPM_ERR_ARG, "scale not set (kpm_plan_set_scale)");
   StepParams p{};
   p.ring = reinterpret_cast<char*>(const_cast<void*>(x));
@@ -373,10 +382,12 @@ int kpm_block_dots(kpm_plan* plan, const void* x, const void* w, double* d_out,
   p.slots = 1;
   p.w = w;
   p.dots = d_out;
+  p.local = local ? d_out : nullptr;
+  p.local_cols = dots_mode == KPM_DOTS_LOCAL_COLS;
   fill_params(plan, p);
   int rc = launch_cheb(plan->value_type, plan->shape, dots_mode, true, 0, 0, p,
                        (hipStream_t)stream);
-  if (!rc) rc = launch_reduce(dots_mode, plan->width, p, (hipStream_t)stream);
+  if (!rc && !local) rc = launch_reduce(dots_mode, plan->width, p, (hipStream_t)stream);
   return rc;
 }
 

@@ -274,6 +274,51 @@ __device__ inline void add_dots(double (&as)[NC][VEC], double (&ac)[NC][VEC],
   }
 }
 
+// Site-resolved operator record (KPM_DOTS_LOCAL) of one matrix row: the
+// FP64 products conj(w[row, c]) x[row, c] of the lane's columns, either
+// stored per column (p.local_cols) or summed over the block's columns -- a
+// TwoSum over the lane's columns, then a butterfly over the row's LPR lanes
+// carrying the (sum, error) pairs, rounded once -- and stored by the row's
+// first lane.  Every lane of the wavefront calls this (the butterfly
+// shuffles); rows past the end pass live = false and store nothing.
+template <typename T, int VEC, int LPR>
+__device__ inline void write_local(const StepParams& p, int row, bool live, int col,
+                                   const CLane<T, VEC>& x, const CLane<T, VEC>& ww) {
+  constexpr int W = LPR * VEC;
+  double re[VEC], im[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+    const double wr = ww.re[v], wi = ww.im[v];
+    const double xr = x.re[v], xi = x.im[v];
+    re[v] = fma(wr, xr, wi * xi);
+    im[v] = fma(wr, xi, -(wi * xr));
+  }
+  d2v* out = reinterpret_cast<d2v*>(p.local);
+  if (p.local_cols) {
+    if (live) {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) out[int64_t(row) * W + col + v] = d2v{re[v], im[v]};
+    }
+    return;
+  }
+  double sr = 0.0, cr = 0.0, si = 0.0, ci = 0.0;
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+    acc2(sr, cr, re[v]);
+    acc2(si, ci, im[v]);
+  }
+#pragma unroll
+  for (int off = 1; off < LPR; off <<= 1) {
+    const double osr = __shfl_xor(sr, off), ocr = __shfl_xor(cr, off);
+    const double osi = __shfl_xor(si, off), oci = __shfl_xor(ci, off);
+    acc2(sr, cr, osr);
+    cr += ocr;
+    acc2(si, ci, osi);
+    ci += oci;
+  }
+  if (live && col == 0) out[row] = d2v{sr + cr, si + ci};
+}
+
 // Entries per chunk of the per-lane path: all index and value loads of a
 // chunk are issued before its gathers, so a row of up to CHV entries costs one
 // index-to-gather round trip.
@@ -394,7 +439,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
   const int grp = lane / LPR;
   const int col = (lane % LPR) * VEC;
 
-  if (MODE != KPM_DOTS_NONE && p.prev_part != nullptr) {
+  if (MODE != KPM_DOTS_NONE && MODE != KPM_DOTS_LOCAL && p.prev_part != nullptr) {
     // the previous step's dots: its partials are complete (kernel boundary)
     reduce_sums<NC, MODE>(p.prev_part, p.nblocks, p.seg_blk, p.nseg, W, p.prev_dots,
                           lb * WAVES + wave, p.nblocks * WAVES);
@@ -452,7 +497,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
           xp[u] = IO::load_nt(prv + roff[u]);
         else
           zero(xp[u]);
-        if (MODE == KPM_DOTS_PROJECT)
+        if (MODE == KPM_DOTS_PROJECT || MODE == KPM_DOTS_LOCAL)
           ww[u] = IO::load(wv + roff[u]);
         else
           zero(ww[u]);
@@ -479,6 +524,9 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
           zero(y[u]);
         }
         add_dots<T, VEC, MODE, NC>(as, ac, xs[u], y[u], ww[u]);
+        if constexpr (MODE == KPM_DOTS_LOCAL)
+          write_local<T, VEC, LPR>(p, r0 + u * RPW + grp, live[u], col, only_dots ? xs[u] : y[u],
+                                   ww[u]);
       }
     }
     }
@@ -603,7 +651,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         }
         if (live) IO::store_nt(nxt + roff, y);
       }
-      if (MODE == KPM_DOTS_PROJECT)
+      if (MODE == KPM_DOTS_PROJECT || MODE == KPM_DOTS_LOCAL)
         ww = IO::load(wv + roff);
       else
         zero(ww);
@@ -612,6 +660,8 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         zero(y);
       }
       add_dots<T, VEC, MODE, NC>(as, ac, xs, y, ww);
+      if constexpr (MODE == KPM_DOTS_LOCAL)
+        write_local<T, VEC, LPR>(p, row, live, col, only_dots ? xs : y, ww);
 #pragma unroll
       for (int c = 0; c < NCH; ++c) {
         jA[c] = jB[c];
@@ -650,7 +700,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
       } else {
         xs = IO::load(cur + roff);
       }
-      if (MODE == KPM_DOTS_PROJECT)
+      if (MODE == KPM_DOTS_PROJECT || MODE == KPM_DOTS_LOCAL)
         ww = IO::load(wv + roff);
       else
         zero(ww);
@@ -659,10 +709,12 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         zero(y);
       }
       add_dots<T, VEC, MODE, NC>(as, ac, xs, y, ww);
+      if constexpr (MODE == KPM_DOTS_LOCAL)
+        write_local<T, VEC, LPR>(p, row, live, col, only_dots ? xs : y, ww);
     }
   }
 
-  if (MODE == KPM_DOTS_NONE) return;
+  if (MODE == KPM_DOTS_NONE || MODE == KPM_DOTS_LOCAL) return;
 
   // ---- workgroup reduction: row groups of a wave, then waves via LDS ----
 #pragma unroll
@@ -855,6 +907,11 @@ int launch_u(int mode, bool only_dots, int iters, int n, const StepParams& p,
       hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, WV, KPM_DOTS_PROJECT, U, PF>), grid, block,
                          0, stream, p, p.indptr, p.indices, p.values, n, iters,
                          int(only_dots));
+      break;
+    case KPM_DOTS_LOCAL:
+    case KPM_DOTS_LOCAL_COLS:  // the same kernel; p.local_cols selects the record
+      hipLaunchKernelGGL((cheb_kernel<T, VEC, LPR, WV, KPM_DOTS_LOCAL, U, PF>), grid, block, 0,
+                         stream, p, p.indptr, p.indices, p.values, n, iters, int(only_dots));
       break;
     default:
       return fail(KPM_ERR_ARG, "unknown dots mode");

@@ -41,6 +41,7 @@ from . import _engine as eng
 from . import autotune
 from . import ordering
 from .lanczos import lanczos_bounds
+from .operator import density_layout
 
 __all__ = ['SpectralDensity', 'Correlator', 'conductivity',
            'RandomVectors', 'LocalVectors', 'jackson_kernel', 'lorentz_kernel',
@@ -342,10 +343,19 @@ class SpectralDensity:
         hamiltonian = _as_csr(hamiltonian, params)
 
         self._op_matrix = None
+        self._density = None
         if operator is None:
             self.operator = None
         elif _is_local_operator(operator):
             self.operator = operator.bind(params=params)
+            # a Density is evaluated on the GPU: sum=True as the matrix
+            # operator sum_i M_i, sum=False per site inside the recursion
+            self._density = density_layout(self.operator)
+            if self._density is not None and \
+                    self._density.matrix.shape[0] != hamiltonian.shape[0]:
+                raise ValueError('vector is incorrect shape')
+            if self._density is not None and self._density.sum:
+                self._op_matrix = self._density.matrix
         elif callable(operator):
             self.operator = operator
         elif hasattr(operator, 'dot'):
@@ -473,6 +483,15 @@ class SpectralDensity:
     def _moments(self):
         """real_if_close, vectors to axis 1, mean, / a, kernel
         (kwant/kpm.py:378-388)."""
+        if self._local_mean:
+            # the site-resolved blocks hold their vectors' sum: the mean is
+            # the total over blocks / R (real_if_close taken on the total)
+            total = self._blocks[0].moments.copy()
+            for blk in self._blocks[1:]:
+                total += blk.moments
+            moments = np.real_if_close(total) / self.num_vectors
+            moments /= self._a
+            return self.kernel(moments)
         moments = np.real_if_close(self._moments_list)
         moments = np.swapaxes(moments, 0, 1)
         if self.mean:
@@ -517,14 +536,14 @@ class SpectralDensity:
                 start += width
                 blk = self._new_block(chunk, width)
                 self._expand_new(blk, n_moments)
-                self._moments_list.extend(list(blk.moments))
+                self._moments_list.extend(self._entries(blk))
                 if not self._vector_factory.accumulate:
                     blk.rec = None  # no resume without accumulated vectors
                 self._blocks.append(blk)
         else:
             for blk in self._blocks:
                 self._expand_more(blk, self.num_moments, n_moments)
-            self._moments_list = [m for blk in self._blocks for m in blk.moments]
+            self._moments_list = [m for blk in self._blocks for m in self._entries(blk)]
 
     def _launch_shape(self, width, mode):
         """The requested kernel variant, or the autotuned one (variant=-1)."""
@@ -532,8 +551,32 @@ class SpectralDensity:
             return self._variant
         return autotune.pick_variant(self._dmat, (self._a, self._b), width, mode)
 
+    @property
+    def _local_mean(self):
+        return self._density is not None and not self._density.sum and self.mean
+
+    def _entries(self, blk):
+        """The block's per-vector entries of _moments_list (placeholders
+        where the block keeps only its vectors' sum)."""
+        if self._local_mean:
+            return [None] * blk.count
+        return list(blk.moments)
+
     def _new_block(self, vectors, width):
         ab = (self._a, self._b)
+        if self._density is not None and not self._density.sum:
+            import torch
+            mode = eng.DOTS_LOCAL if self.mean else eng.DOTS_LOCAL_COLS
+            blk = _DosBlock(self._dmat, ab, vectors, width, mode,
+                            variant=self._launch_shape(width, eng.DOTS_PROJECT))
+            # w = M^H alpha_0: <w|alpha_n> per row is the row's part of
+            # vdot(alpha_0, M alpha_n) (kwant/operator.pyx:791-797)
+            op_h = eng.DeviceCSR(self._density.matrix.conj().T.tocsr(), self._dmat.precision,
+                                 order=self._dmat.order)
+            w = torch.empty_like(blk.rec.ring[0])
+            op_h.spmm(blk.rec.ring[0], w, width)
+            blk.rec.w = w
+            return blk
         if self.operator is None:
             return _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_DOUBLING,
                              variant=self._launch_shape(width, eng.DOTS_DOUBLING))
@@ -569,8 +612,12 @@ class SpectralDensity:
             mom = np.zeros((blk.count, n_moments), dtype=np.complex128)
             dots = rec.step(n_moments - 1)
             mom[:, 0] = dots[0, :blk.count, 0] + 1j * dots[0, :blk.count, 1]
+            if self._density is not None and self._density.real_mu0:
+                mom[:, 0] = mom[:, 0].real   # op(a_0, a_0) of a Hermitian density
             mom[:, 1:] = (dots[:, :blk.count, 2] + 1j * dots[:, :blk.count, 3]).T
             blk.moments = mom
+        elif rec.mode in (eng.DOTS_LOCAL, eng.DOTS_LOCAL_COLS):
+            blk.moments = self._local_moments(blk, 0, n_moments, None)
         else:
             blk.moments = self._callable_moments(blk, 0, n_moments, None)
 
@@ -593,8 +640,64 @@ class SpectralDensity:
             dots = rec.step(m_new - m_old)
             mom[:, m_old:] = (dots[:, :blk.count, 2] + 1j * dots[:, :blk.count, 3]).T
             blk.moments = mom
+        elif rec.mode in (eng.DOTS_LOCAL, eng.DOTS_LOCAL_COLS):
+            blk.moments = self._local_moments(blk, m_old, m_new, blk.moments)
         else:
             blk.moments = self._callable_moments(blk, m_old, m_new, blk.moments)
+
+    # device records per step are kept under this many bytes at a time
+    _LOCAL_CHUNK_BYTES = 1 << 30
+
+    def _local_rows(self):
+        """Device rows of the density's orbitals (None: all rows in order)."""
+        import torch
+        lay = self._density
+        if getattr(self, '_local_idx', None) is None:
+            n = self._dmat.shape[0]
+            rows = lay.orbs if self._dmat.inverse is None else self._dmat.inverse[lay.orbs]
+            if len(rows) == n and np.array_equal(rows, np.arange(n)):
+                self._local_idx = (None,)
+            else:
+                self._local_idx = (torch.from_numpy(rows).to(eng.torch_device()),)
+        return self._local_idx[0]
+
+    def _local_moments(self, blk, m_old, m_new, old):
+        """Site-resolved Density moments op(alpha_0, alpha_n) of
+        kwant/kpm.py:446-448, 484 for n in [m_old, m_new): the recursion
+        kernel writes each row's <w|alpha_n>, the rows of each ``where`` site
+        are summed on the host.  mean=True: (M, n_where), the block's vectors
+        summed; mean=False: (count, M, n_where)."""
+        rec, lay = blk.rec, self._density
+        idx = self._local_rows()
+        cols = rec.mode == eng.DOTS_LOCAL_COLS
+
+        def sites(records):
+            if idx is not None:
+                records = records.index_select(1, idx)
+            host = records.cpu().numpy()
+            if cols:
+                host = host[:, :, :blk.count]
+            return lay.site_sums(host)       # (k, n_where) or (k, n_where, count)
+
+        parts = []
+        n = m_old
+        if n == 0:
+            first = sites(rec.local_now()[None])
+            if lay.real_mu0:
+                first = first.real + 0j      # op(a_0, a_0) of a Hermitian density
+            parts.append(first)
+            n = 1
+        per_step = rec.n_rows * 16 * (rec.width if cols else 1)
+        chunk = max(1, self._LOCAL_CHUNK_BYTES // per_step)
+        while n < m_new:
+            k = min(chunk, m_new - n)
+            parts.append(sites(rec.local_steps(k)))
+            n += k
+        new = np.concatenate(parts, axis=0) if parts else None
+        if cols:
+            new = np.moveaxis(new, 2, 0)     # (count, k, n_where)
+            return new if old is None else np.concatenate([old, new], axis=1)
+        return new if old is None else np.concatenate([old, new], axis=0)
 
     def _callable_moments(self, blk, m_old, m_new, old):
         """Operator path with a Python callable (kwant/kpm.py:478-484): the
