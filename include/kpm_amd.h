@@ -168,6 +168,21 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring,
 int kpm_block_dots(kpm_plan* plan, const void* x, const void* w,
                    double* d_out, int32_t dots_mode, kpm_stream_t stream);
 
+/* Bond records of a current-like operator (kwant.operator.Current,
+ * kwant/operator.pyx:940-1001, evaluated as op(a_0, a_n) by
+ * kwant/kpm.py:484): for every entry k of the square CSR `bonds` (row r,
+ * column s, value h -- the Hamiltonian entries of the hoppings asked for)
+ * and every column c of the n_rows x width blocks,
+ *   t_k(c) = i (conj(h) conj(bra[s,c]) v[r,c] - conj(w[r,c]) h ket[s,c])
+ * with v = M ket and w = M^H bra for the block-diagonal onsite matrix M
+ * (v = ket, w = bra when M = 1).  per_column == 0: d_out[k] = sum_c t_k(c)
+ * (complex128, the vectors of the block summed for mean=True); otherwise
+ * d_out[k * width + c] = t_k(c).  The caller sums the entries of each
+ * hopping.  All blocks have the bond matrix's value type. */
+int kpm_bond_records(const kpm_csr* bonds, const void* bra, const void* ket,
+                     const void* v, const void* w, int32_t width,
+                     int32_t per_column, double* d_out, kpm_stream_t stream);
+
 /* y = scale * op x for an n_rows x width block (op: n_rows x n_cols CSR of the
  * same value type; x: n_cols x width; y: n_rows x width), for n_slots
  * consecutive slots (x + k*x_stride, y + k*y_stride).  This is the operator

@@ -38,7 +38,7 @@ EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
     "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
-    "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
+    "kpm_bond_records", "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
     "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
     "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
     "kpm_mlp_ensemble", "kpm_mlp_adam",
@@ -124,6 +124,8 @@ def load_library(path=LIB_PATH):
         "kpm_cheb_steps": (ctypes.c_int, [vp, ctypes.POINTER(KpmCsr), vp, i64, i32, vp,
                                           vp, i32, i32, i32, i32, vp]),
         "kpm_block_dots": (ctypes.c_int, [vp, vp, vp, vp, i32, vp]),
+        "kpm_bond_records": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, vp, vp, vp, i32, i32,
+                                            vp, vp]),
         "kpm_spmm": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, i64, vp, i64, i32, i32, vp]),
         "kpm_corr_gemm": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
                                          i32, vp, i64, vp]),
@@ -275,6 +277,24 @@ class DeviceCSR:
                            indices.data_ptr(), values.data_ptr(), self.value_type,
                            row_nnz_hint(indptr[1:] - indptr[:-1]))
         return self
+
+    def bond_records(self, bra, ket, v, w, width, out, per_column=False):
+        """Bond records of the entries of this matrix (see kpm_bond_records):
+        out[k] (or out[k, c] with per_column) = i (conj(h) conj(bra_s) v_r -
+        conj(w_r) h ket_s) summed over the block's columns."""
+        n = self.shape[0]
+        for t in (bra, ket, v, w):
+            if tuple(t.shape) != (n, width) or not t.is_contiguous():
+                raise ValueError("bond operands must be contiguous (N, width) blocks")
+        want = (self.nnz, width) if per_column else (self.nnz,)
+        import torch
+        if out.dtype != torch.complex128 or tuple(out.shape) != want or not out.is_contiguous():
+            raise ValueError("bond records must be a contiguous complex128 %s tensor" % (want,))
+        if self.nnz == 0:
+            return
+        check(load_library().kpm_bond_records(
+            ctypes.byref(self.desc), bra.data_ptr(), ket.data_ptr(), v.data_ptr(), w.data_ptr(),
+            width, int(bool(per_column)), out.data_ptr(), stream_handle()), "kpm_bond_records")
 
     def spmm(self, x, y, width, n_slots=1, x_stride=0, y_stride=0):
         """y[slot] = self @ x[slot] for row-major blocks of ``width`` columns."""

@@ -24,6 +24,7 @@ from scipy.sparse import coo_matrix, csr_matrix
 
 from . import _engine as eng
 from . import ordering
+from .operator import operator_layout
 from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _is_local_operator,
                   _is_system, _rescale, _scale_from_bounds, ensure_rng, fermi_distribution,
                   jackson_kernel, SAMPLING, _chebyshev_nodes)
@@ -40,7 +41,12 @@ def _normalize_operator(op, params):
     if op is None:
         return 'id', None
     if _is_local_operator(op):
-        return 'callable', op.bind(params=params).act
+        bound = op.bind(params=params)
+        layout = operator_layout(bound)
+        if layout is not None:
+            # Density / Source / Current act as one sparse matrix on the GPU
+            return 'matrix', layout.act_matrix
+        return 'callable', bound.act
     if callable(op):
         return 'callable', op
     if hasattr(op, 'dot'):

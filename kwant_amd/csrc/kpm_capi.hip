@@ -405,6 +405,18 @@ int kpm_spmm(const kpm_csr* op, const void* x, int64_t x_stride, void* y, int64_
                      (hipStream_t)stream);
 }
 
+int kpm_bond_records(const kpm_csr* bonds, const void* bra, const void* ket, const void* v,
+                     const void* w, int32_t width, int32_t per_column, double* d_out,
+                     kpm_stream_t stream) {
+  if (!bonds || !bra || !ket || !v || !w || !d_out) return fail(KPM_ERR_ARG, "NULL argument");
+  if (bonds->n_rows <= 0 || bonds->n_cols != bonds->n_rows)
+    return fail(KPM_ERR_ARG, "bond matrix must be square and non-empty");
+  if (!bonds->indptr || (bonds->nnz > 0 && (!bonds->indices || !bonds->values)))
+    return fail(KPM_ERR_ARG, "bond matrix arrays are NULL");
+  if (bonds->nnz == 0) return KPM_OK;
+  return launch_bond(width, bonds, bra, ket, v, w, per_column != 0, d_out, (hipStream_t)stream);
+}
+
 int kpm_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb, double* mu,
                   int64_t ldc, int32_t m_rows, int32_t n_rows, int64_t k, int32_t precision,
                   int32_t k_chunk, double* work, int64_t work_elems, kpm_stream_t stream) {

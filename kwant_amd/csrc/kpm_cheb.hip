@@ -858,6 +858,84 @@ __global__ __launch_bounds__(WAVES * 64) void spmm_kernel(const int32_t* indptr,
   IO::store(ys + 2 * (int64_t(row) * W + col), acc);
 }
 
+// Bond records of a current-like operator (kwant.operator.Current,
+// kwant/operator.pyx:940-1001): for every entry k of the bond matrix B (row
+// r, column s = indices[k], value h = values[k]) and every block column c,
+//   t_k(c) = i (conj(h) conj(bra[s,c]) v[r,c] - conj(w[r,c]) h ket[s,c])
+// with v = M ket and w = M^H bra for the block-diagonal onsite matrix M.
+// FP64 products; summed over the block's columns like the site-resolved
+// records (TwoSum per lane, butterfly over the row's lanes, one rounding),
+// out[k] = complex128, or kept per column, out[k * W + c].  A lane group owns
+// one row; the groups of a wavefront run independent entry loops, and every
+// shuffle stays inside its group.
+template <typename T, int VEC, int LPR, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void bond_kernel(
+    const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const void* __restrict__ values, const T* __restrict__ bra, const T* __restrict__ ket,
+    const T* __restrict__ v, const T* __restrict__ w, double* __restrict__ out, int per_column,
+    int n_rows) {
+  constexpr int W = LPR * VEC;
+  constexpr int RPW = 64 / LPR;
+  using IO = LaneIO<T, VEC>;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int row = (blockIdx.x * WAVES + wave) * RPW + lane / LPR;
+  const int col = (lane % LPR) * VEC;
+  if (row >= n_rows) return;  // whole lane groups leave together
+  const int64_t roff = 2 * (int64_t(row) * W + col);
+  const CLane<T, VEC> vr = IO::load(v + roff);
+  const CLane<T, VEC> wr = IO::load(w + roff);
+  d2v* rec = reinterpret_cast<d2v*>(out);
+  const int k1 = indptr[row + 1];
+  for (int k = indptr[row]; k < k1; ++k) {
+    const int s = indices[k];
+    T hr, hi;
+    load_value<T>(values, k, hr, hi);
+    const CLane<T, VEC> bs = IO::load(bra + 2 * (int64_t(s) * W + col));
+    const CLane<T, VEC> ks = IO::load(ket + 2 * (int64_t(s) * W + col));
+    double tr[VEC], ti[VEC];
+#pragma unroll
+    for (int u = 0; u < VEC; ++u) {
+      const double h_r = hr, h_i = hi;
+      // p = h bra_s, t1 = conj(p) v_r
+      const double pr = fma(h_r, double(bs.re[u]), -(h_i * double(bs.im[u])));
+      const double pi = fma(h_r, double(bs.im[u]), h_i * double(bs.re[u]));
+      const double t1r = fma(pr, double(vr.re[u]), pi * double(vr.im[u]));
+      const double t1i = fma(pr, double(vr.im[u]), -(pi * double(vr.re[u])));
+      // q = h ket_s, t2 = conj(w_r) q
+      const double qr = fma(h_r, double(ks.re[u]), -(h_i * double(ks.im[u])));
+      const double qi = fma(h_r, double(ks.im[u]), h_i * double(ks.re[u]));
+      const double t2r = fma(double(wr.re[u]), qr, double(wr.im[u]) * qi);
+      const double t2i = fma(double(wr.re[u]), qi, -(double(wr.im[u]) * qr));
+      // i (t1 - t2)
+      tr[u] = t2i - t1i;
+      ti[u] = t1r - t2r;
+    }
+    if (per_column) {
+#pragma unroll
+      for (int u = 0; u < VEC; ++u) rec[int64_t(k) * W + col + u] = d2v{tr[u], ti[u]};
+      continue;
+    }
+    double sr = 0.0, cr = 0.0, si = 0.0, ci = 0.0;
+#pragma unroll
+    for (int u = 0; u < VEC; ++u) {
+      acc2(sr, cr, tr[u]);
+      acc2(si, ci, ti[u]);
+    }
+#pragma unroll
+    for (int off = 1; off < LPR; off <<= 1) {
+      const double osr = __shfl_xor(sr, off), ocr = __shfl_xor(cr, off);
+      const double osi = __shfl_xor(si, off), oci = __shfl_xor(ci, off);
+      acc2(sr, cr, osr);
+      cr += ocr;
+      acc2(si, ci, osi);
+      ci += oci;
+    }
+    if (col == 0) rec[k] = d2v{sr + cr, si + ci};
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Variant table and dispatch.
 
@@ -961,6 +1039,18 @@ int launch_spmm_one(const kpm_csr* op, const void* x, int64_t xs, void* y, int64
                      op->indices, op->values, reinterpret_cast<const T*>(x), xs,
                      reinterpret_cast<T*>(y), ys, op->n_rows);
   return check_hip(hipGetLastError(), "spmm_kernel launch");
+}
+
+template <typename T, int VEC, int LPR>
+int launch_bond_one(const kpm_csr* b, const void* bra, const void* ket, const void* v,
+                    const void* w, int per_column, double* out, hipStream_t stream) {
+  constexpr int RPB = kWaves * 64 / LPR;
+  dim3 grid((b->n_rows + RPB - 1) / RPB), block(kWaves * 64);
+  hipLaunchKernelGGL((bond_kernel<T, VEC, LPR, kWaves>), grid, block, 0, stream, b->indptr,
+                     b->indices, b->values, reinterpret_cast<const T*>(bra),
+                     reinterpret_cast<const T*>(ket), reinterpret_cast<const T*>(v),
+                     reinterpret_cast<const T*>(w), out, per_column, b->n_rows);
+  return check_hip(hipGetLastError(), "bond_kernel launch");
 }
 
 // (vec, lpr) dispatch for a value type; returns KPM_ERR_UNSUPPORTED if none.
@@ -1103,6 +1193,21 @@ struct ChebLauncher {
   }
 };
 
+struct BondLauncher {
+  const kpm_csr* b;
+  const void* bra;
+  const void* ket;
+  const void* v;
+  const void* w;
+  int per_column;
+  double* out;
+  hipStream_t s;
+  template <typename T, int V, int LP>
+  int operator()() {
+    return launch_bond_one<T, V, LP>(b, bra, ket, v, w, per_column, out, s);
+  }
+};
+
 struct SpmmLauncher {
   const kpm_csr* op;
   const void* x;
@@ -1154,6 +1259,19 @@ int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x, int
     if (width > 64) return fail(KPM_ERR_UNSUPPORTED, "complex128 spmm width must be <= 64");
     return dispatch_shape<double>(1, width, l);
   }
+  return fail(KPM_ERR_ARG, "unknown value type");
+}
+
+int launch_bond(int width, const kpm_csr* b, const void* bra, const void* ket, const void* v,
+                const void* w, int per_column, double* out, hipStream_t stream) {
+  if (width < 1 || width > 64 || (width & (width - 1)) != 0)
+    return fail(KPM_ERR_UNSUPPORTED, "bond width must be a power of two <= 64");
+  BondLauncher l{b, bra, ket, v, w, per_column, out, stream};
+  if (b->value_type == KPM_C64) {
+    if (width >= 2) return dispatch_shape<float>(2, width / 2, l);
+    return dispatch_shape<float>(1, 1, l);
+  }
+  if (b->value_type == KPM_C128) return dispatch_shape<double>(1, width, l);
   return fail(KPM_ERR_ARG, "unknown value type");
 }
 

@@ -66,6 +66,8 @@ struct StepParams {
 int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots, int n,
                 int row_nnz_hint, const StepParams& p, hipStream_t stream);
 int launch_reduce(int mode, int width, const StepParams& p, hipStream_t stream);
+int launch_bond(int width, const kpm_csr* b, const void* bra, const void* ket, const void* v,
+                const void* w, int per_column, double* out, hipStream_t stream);
 int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x,
                 int64_t x_stride, void* y, int64_t y_stride, int n_slots,
                 hipStream_t stream);

@@ -41,7 +41,7 @@ from . import _engine as eng
 from . import autotune
 from . import ordering
 from .lanczos import lanczos_bounds
-from .operator import density_layout
+from .operator import BondLayout, SiteLayout, operator_layout
 
 __all__ = ['SpectralDensity', 'Correlator', 'conductivity',
            'RandomVectors', 'LocalVectors', 'jackson_kernel', 'lorentz_kernel',
@@ -343,19 +343,20 @@ class SpectralDensity:
         hamiltonian = _as_csr(hamiltonian, params)
 
         self._op_matrix = None
-        self._density = None
+        self._layout = None
         if operator is None:
             self.operator = None
         elif _is_local_operator(operator):
             self.operator = operator.bind(params=params)
-            # a Density is evaluated on the GPU: sum=True as the matrix
-            # operator sum_i M_i, sum=False per site inside the recursion
-            self._density = density_layout(self.operator)
-            if self._density is not None and \
-                    self._density.matrix.shape[0] != hamiltonian.shape[0]:
+            # Density, Source and Current are evaluated on the GPU: sum=True
+            # as the matrix operator sum_i Q_i, sum=False per site (inside the
+            # recursion kernel) or per hopping (bond records)
+            self._layout = operator_layout(self.operator)
+            if self._layout is not None and \
+                    self._layout.act_matrix.shape[0] != hamiltonian.shape[0]:
                 raise ValueError('vector is incorrect shape')
-            if self._density is not None and self._density.sum:
-                self._op_matrix = self._density.matrix
+            if self._layout is not None and self._layout.sum:
+                self._op_matrix = self._layout.act_matrix
         elif callable(operator):
             self.operator = operator
         elif hasattr(operator, 'dot'):
@@ -553,7 +554,7 @@ class SpectralDensity:
 
     @property
     def _local_mean(self):
-        return self._density is not None and not self._density.sum and self.mean
+        return self._layout is not None and not self._layout.sum and self.mean
 
     def _entries(self, blk):
         """The block's per-vector entries of _moments_list (placeholders
@@ -564,14 +565,26 @@ class SpectralDensity:
 
     def _new_block(self, vectors, width):
         ab = (self._a, self._b)
-        if self._density is not None and not self._density.sum:
+        if isinstance(self._layout, BondLayout) and not self._layout.sum:
+            import torch
+            blk = _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_NONE,
+                            variant=self._launch_shape(width, eng.DOTS_NONE))
+            # a_0 is kept (the two-slot ring overwrites it); w = M^H a_0
+            blk.bra = blk.rec.ring[0].clone()
+            blk.w = blk.bra
+            onsite = self._bond_device()[2]
+            if onsite is not None:
+                blk.w = torch.empty_like(blk.bra)
+                onsite[1].spmm(blk.bra, blk.w, width)
+            return blk
+        if isinstance(self._layout, SiteLayout) and not self._layout.sum:
             import torch
             mode = eng.DOTS_LOCAL if self.mean else eng.DOTS_LOCAL_COLS
             blk = _DosBlock(self._dmat, ab, vectors, width, mode,
                             variant=self._launch_shape(width, eng.DOTS_PROJECT))
             # w = M^H alpha_0: <w|alpha_n> per row is the row's part of
             # vdot(alpha_0, M alpha_n) (kwant/operator.pyx:791-797)
-            op_h = eng.DeviceCSR(self._density.matrix.conj().T.tocsr(), self._dmat.precision,
+            op_h = eng.DeviceCSR(self._layout.matrix.conj().T.tocsr(), self._dmat.precision,
                                  order=self._dmat.order)
             w = torch.empty_like(blk.rec.ring[0])
             op_h.spmm(blk.rec.ring[0], w, width)
@@ -612,12 +625,14 @@ class SpectralDensity:
             mom = np.zeros((blk.count, n_moments), dtype=np.complex128)
             dots = rec.step(n_moments - 1)
             mom[:, 0] = dots[0, :blk.count, 0] + 1j * dots[0, :blk.count, 1]
-            if self._density is not None and self._density.real_mu0:
+            if self._layout is not None and self._layout.real_mu0:
                 mom[:, 0] = mom[:, 0].real   # op(a_0, a_0) of a Hermitian density
             mom[:, 1:] = (dots[:, :blk.count, 2] + 1j * dots[:, :blk.count, 3]).T
             blk.moments = mom
         elif rec.mode in (eng.DOTS_LOCAL, eng.DOTS_LOCAL_COLS):
             blk.moments = self._local_moments(blk, 0, n_moments, None)
+        elif hasattr(blk, 'bra'):
+            blk.moments = self._bond_moments(blk, 0, n_moments, None)
         else:
             blk.moments = self._callable_moments(blk, 0, n_moments, None)
 
@@ -642,6 +657,8 @@ class SpectralDensity:
             blk.moments = mom
         elif rec.mode in (eng.DOTS_LOCAL, eng.DOTS_LOCAL_COLS):
             blk.moments = self._local_moments(blk, m_old, m_new, blk.moments)
+        elif hasattr(blk, 'bra'):
+            blk.moments = self._bond_moments(blk, m_old, m_new, blk.moments)
         else:
             blk.moments = self._callable_moments(blk, m_old, m_new, blk.moments)
 
@@ -651,7 +668,7 @@ class SpectralDensity:
     def _local_rows(self):
         """Device rows of the density's orbitals (None: all rows in order)."""
         import torch
-        lay = self._density
+        lay = self._layout
         if getattr(self, '_local_idx', None) is None:
             n = self._dmat.shape[0]
             rows = lay.orbs if self._dmat.inverse is None else self._dmat.inverse[lay.orbs]
@@ -667,7 +684,7 @@ class SpectralDensity:
         kernel writes each row's <w|alpha_n>, the rows of each ``where`` site
         are summed on the host.  mean=True: (M, n_where), the block's vectors
         summed; mean=False: (count, M, n_where)."""
-        rec, lay = blk.rec, self._density
+        rec, lay = blk.rec, self._layout
         idx = self._local_rows()
         cols = rec.mode == eng.DOTS_LOCAL_COLS
 
@@ -696,6 +713,84 @@ class SpectralDensity:
         new = np.concatenate(parts, axis=0) if parts else None
         if cols:
             new = np.moveaxis(new, 2, 0)     # (count, k, n_where)
+            return new if old is None else np.concatenate([old, new], axis=1)
+        return new if old is None else np.concatenate([old, new], axis=0)
+
+    def _bond_device(self):
+        """The Current's entries as a device CSR in device row order, the
+        CSR position of every entry (entries are listed hopping by hopping),
+        and the onsite matrix M with M^H on the device (None when M = 1)."""
+        import torch
+        if getattr(self, '_bond', None) is None:
+            lay, n = self._layout, self._dmat.shape[0]
+            inv = self._dmat.inverse
+            r = lay.rows if inv is None else inv[lay.rows]
+            c = lay.cols if inv is None else inv[lay.cols]
+            perm = np.lexsort((c, r))
+            indptr = np.zeros(n + 1, dtype=np.int64)
+            np.cumsum(np.bincount(r, minlength=n), out=indptr[1:])
+            csr = csr_matrix((lay.values[perm], c[perm], indptr), shape=(n, n))
+            bond = eng.DeviceCSR(csr, self._dmat.precision)
+            pos = np.empty_like(perm)
+            pos[perm] = np.arange(len(perm))
+            onsite = None
+            if lay.onsite is not None:
+                onsite = (eng.DeviceCSR(lay.onsite, self._dmat.precision, order=self._dmat.order),
+                          eng.DeviceCSR(lay.onsite.conj().T.tocsr(), self._dmat.precision,
+                                        order=self._dmat.order))
+            self._bond = (bond, torch.from_numpy(pos).to(eng.torch_device()), onsite)
+        return self._bond
+
+    def _bond_moments(self, blk, m_old, m_new, old):
+        """Current moments op(alpha_0, alpha_n) of kwant/kpm.py:446-448, 484
+        for n in [m_old, m_new): after each recursion step the bond kernel
+        writes every hopping entry's record (kpm_bond_records), the entries
+        of each hopping are summed on the host.  mean=True: (M, n_where),
+        the block's vectors summed; mean=False: (count, M, n_where)."""
+        import torch
+        rec, lay = blk.rec, self._layout
+        bond, pos, onsite = self._bond_device()
+        cols = not self.mean
+        width = rec.width
+        tail = (bond.nnz, width) if cols else (bond.nnz,)
+        v = torch.empty_like(blk.bra) if onsite is not None else None
+        dev = blk.bra.device
+
+        def record(ket, out):
+            if onsite is not None:
+                onsite[0].spmm(ket, v, width)
+            bond.bond_records(blk.bra, ket, v if onsite is not None else ket, blk.w, width,
+                              out, per_column=cols)
+
+        def hops(buf):
+            host = buf.index_select(1, pos).cpu().numpy()
+            if cols:
+                host = host[:, :, :blk.count]
+            return lay.hop_sums(host)      # (k, n_where) or (k, n_where, count)
+
+        parts = []
+        n = m_old
+        if n == 0:
+            buf = torch.empty((1,) + tail, dtype=torch.complex128, device=dev)
+            record(rec.slot(0), buf[0])
+            first = hops(buf)
+            if lay.real_mu0:
+                first = first.real + 0j    # op(a_0, a_0) of a Hermitian current
+            parts.append(first)
+            n = 1
+        per_step = max(1, int(np.prod(tail)) * 16)
+        chunk = max(1, self._LOCAL_CHUNK_BYTES // per_step)
+        while n < m_new:
+            k = min(chunk, m_new - n)
+            buf = torch.empty((k,) + tail, dtype=torch.complex128, device=dev)
+            for i in range(k):
+                rec.step(1, want_dots=False)
+                record(rec.slot(rec.n), buf[i])
+            parts.append(hops(buf))
+            n += k
+        new = np.concatenate(parts, axis=0)
+        if cols:
+            new = np.moveaxis(new, 2, 0)   # (count, k, n_where)
             return new if old is None else np.concatenate([old, new], axis=1)
         return new if old is None else np.concatenate([old, new], axis=0)
 

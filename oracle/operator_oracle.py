@@ -16,11 +16,17 @@ check_hermiticity and bra and ket are one object after ``asarray``, and
 summed when ``sum``.  Onsite normalisation (kwant/operator.pyx:273-321): a
 1 x 1 onsite becomes value * identity(max norbs).
 
+``current(...)`` restates Current._operate (kwant/operator.pyx:940-1001),
+``source(...)`` Source._operate (kwant/operator.pyx:1068-1116), with the
+Hamiltonian blocks H_ab given per ``where`` entry.
+
 Pinning (tests/test_operator_oracle.py): closed forms (identity onsite:
-sum=True is vdot(bra, ket), one orbital per site gives conj(bra_i) ket_i),
-a dense block-diagonal Q_w brute force for multi-orbital matrix onsites,
-and a hand-computed case of the flattened read for a scalar onsite on sites
-with fewer orbitals than the largest.
+sum=True is vdot(bra, ket), one orbital per site gives conj(bra_i) ket_i;
+the chain current 2 Im(t conj(psi_a) psi_b); the source of sigma_z under
+an onsite sigma_x, 2 sigma_y), dense Q_w / J_w brute force, the stationary
+continuity sum_b J_ab(psi, psi) = 0 for an eigenstate psi, and a
+hand-computed case of the flattened read for a scalar onsite on sites with
+fewer orbitals than the largest.
 """
 import numpy as np
 
@@ -35,19 +41,28 @@ def _orbs(site_ranges, site):
     return offset + (site - first) * norbs, norbs
 
 
+def _onsite_mats(site_ranges, onsite, count):
+    if isinstance(onsite, list):
+        return [np.asarray(m, dtype=complex).reshape(-1) for m in onsite]
+    mat = np.atleast_2d(np.asarray(onsite, dtype=complex))
+    if mat.shape[0] == 1:
+        max_norbs = max(r[1] for r in site_ranges)
+        mat = mat[0, 0] * np.eye(max_norbs, dtype=complex)
+    return [mat.reshape(-1)] * count
+
+
+def _finish(result, check_hermiticity, same, sum):
+    if check_hermiticity and same:
+        result = result.real
+    return np.sum(result) if sum else result
+
+
 def density(site_ranges, onsite, where, check_hermiticity=True, sum=False):
     """op(bra, ket) of a Density.  ``onsite``: a scalar or square matrix used
     on every site, or a list with one matrix per ``where`` entry."""
     site_ranges = [tuple(int(v) for v in r) for r in site_ranges]
     where = [int(w) for w in where]
-    if isinstance(onsite, list):
-        mats = [np.asarray(m, dtype=complex).reshape(-1) for m in onsite]
-    else:
-        mat = np.atleast_2d(np.asarray(onsite, dtype=complex))
-        if mat.shape[0] == 1:
-            max_norbs = max(r[1] for r in site_ranges)
-            mat = mat[0, 0] * np.eye(max_norbs, dtype=complex)
-        mats = [mat.reshape(-1)] * len(where)
+    mats = _onsite_mats(site_ranges, onsite, len(where))
     tot = site_ranges[-1][2]
 
     def op(bra, ket=None):
@@ -64,8 +79,71 @@ def density(site_ranges, onsite, where, check_hermiticity=True, sum=False):
                 for j in range(norbs):
                     tmp += bra[s + i].conjugate() * m[i * norbs + j] * ket[s + j]
             result[w] = tmp
-        if check_hermiticity and bra is ket:
-            result = result.real
-        return np.sum(result) if sum else result
+        return _finish(result, check_hermiticity, bra is ket, sum)
+
+    return op
+
+
+def current(site_ranges, onsite, hoppings, hams, check_hermiticity=True, sum=False):
+    """op(bra, ket) of a Current over ``hoppings`` (a, b); ``hams``: H_ab of
+    every hopping (a_norbs x b_norbs)."""
+    site_ranges = [tuple(int(v) for v in r) for r in site_ranges]
+    hoppings = [(int(a), int(b)) for a, b in hoppings]
+    mats = _onsite_mats(site_ranges, onsite, len(hoppings))
+    hs = [np.asarray(h, dtype=complex).reshape(-1) for h in hams]
+    tot = site_ranges[-1][2]
+
+    def op(bra, ket=None):
+        bra = np.asarray(bra, dtype=complex)
+        ket = bra if ket is None else np.asarray(ket, dtype=complex)
+        if bra.shape != (tot,) or ket.shape != (tot,):
+            raise ValueError('vector is incorrect shape')
+        result = np.zeros(len(hoppings), dtype=complex)
+        for w, (a, b) in enumerate(hoppings):
+            a_s, a_n = _orbs(site_ranges, a)
+            b_s, b_n = _orbs(site_ranges, b)
+            m, h = mats[w], hs[w]
+            tmp = 0j
+            for i in range(b_n):
+                for j in range(a_n):
+                    for k in range(a_n):
+                        tmp += (bra[b_s + i].conjugate() * h[j * b_n + i].conjugate() *
+                                m[j * a_n + k] * ket[a_s + k]
+                                - bra[a_s + j].conjugate() * m[j * a_n + k] *
+                                h[k * b_n + i] * ket[b_s + i])
+            result[w] = 1j * tmp
+        return _finish(result, check_hermiticity, bra is ket, sum)
+
+    return op
+
+
+def source(site_ranges, onsite, where, hams, check_hermiticity=True, sum=False):
+    """op(bra, ket) of a Source over the sites ``where``; ``hams``: the
+    onsite Hamiltonian H_aa of every site."""
+    site_ranges = [tuple(int(v) for v in r) for r in site_ranges]
+    where = [int(w) for w in where]
+    mats = _onsite_mats(site_ranges, onsite, len(where))
+    hs = [np.asarray(h, dtype=complex).reshape(-1) for h in hams]
+    tot = site_ranges[-1][2]
+
+    def op(bra, ket=None):
+        bra = np.asarray(bra, dtype=complex)
+        ket = bra if ket is None else np.asarray(ket, dtype=complex)
+        if bra.shape != (tot,) or ket.shape != (tot,):
+            raise ValueError('vector is incorrect shape')
+        result = np.zeros(len(where), dtype=complex)
+        for w, site in enumerate(where):
+            a_s, a_n = _orbs(site_ranges, site)
+            m, h = mats[w], hs[w]
+            tmp2 = 0j
+            for i in range(a_n):
+                tmp = 0j
+                for j in range(a_n):
+                    for k in range(a_n):
+                        tmp += (h[j * a_n + i].conjugate() * m[j * a_n + k] * ket[a_s + k]
+                                - m[i * a_n + j] * h[j * a_n + k] * ket[a_s + k])
+                tmp2 += bra[a_s + i].conjugate() * tmp
+            result[w] = 1j * tmp2
+        return _finish(result, check_hermiticity, bra is ket, sum)
 
     return op

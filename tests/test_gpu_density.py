@@ -168,3 +168,139 @@ def test_local_rejects_missing_w():
     rec = eng.BlockRecursion(mat, (2.0, 0.0), block, 1, eng.DOTS_LOCAL)
     with pytest.raises(ValueError):
         rec.local_steps(1)
+
+
+# -- Current (bond records) and Source ----------------------------------------
+
+from oracle.operator_oracle import current as oracle_current  # noqa: E402
+from oracle.operator_oracle import source as oracle_source  # noqa: E402
+
+
+def _hops(syst):
+    hops = list(syst.graph)[::6]
+    return hops + hops[:2]                          # repeated hoppings
+
+
+def _current_pair(onsite, sum_=False, mean=True, num_vectors=10, num_moments=30,
+                  precision='complex128', **kw):
+    from kwant_amd import kpm
+    from kwant_amd.operator import Current
+    syst = FakeSystem(RUNS, seed=3)
+    hops = _hops(syst)
+    bounds = _bounds(syst.h)
+    sd = kpm.SpectralDensity(syst, operator=Current(syst, onsite, where=hops, sum=sum_),
+                             num_vectors=num_vectors, num_moments=num_moments, bounds=bounds,
+                             rng=9, mean=mean, precision=precision, **kw)
+    ref_onsite = [onsite(a) for a, _ in hops] if callable(onsite) else onsite
+    ref = ko.SpectralDensity(
+        syst.h, operator=oracle_current(syst.site_ranges, ref_onsite, hops,
+                                        [syst.hamiltonian(a, b) for a, b in hops], sum=sum_),
+        num_vectors=num_vectors, num_moments=num_moments, bounds=bounds, rng=9, mean=mean)
+    return sd, ref
+
+
+def _onsite_plain(site):
+    return MATS[site]
+
+
+@pytest.mark.parametrize("onsite", [1, 0.5, 'callable'])
+def test_current_mean_matches_oracle(onsite):
+    onsite = _onsite_plain if onsite == 'callable' else onsite
+    sd, ref = _current_pair(onsite)
+    assert sd._moments().shape == ref._moments().shape
+    assert _rel(sd._moments(), ref._moments()) < 1e-11
+    assert _rel(sd.densities, ref.densities) < 1e-11
+
+
+def test_current_per_vector_and_sum_match_oracle():
+    sd, ref = _current_pair(_onsite_plain, mean=False, num_vectors=3)
+    assert _rel(sd._moments(), ref._moments()) < 1e-11
+    sd, ref = _current_pair(_onsite_plain, sum_=True)
+    assert sd._moments().shape == (30,)
+    assert _rel(sd._moments(), ref._moments()) < 1e-11
+
+
+def test_current_resume_reorder_and_complex64():
+    once, _ = _current_pair(0.5, num_moments=40)
+    more, _ = _current_pair(0.5, num_moments=25)
+    more.add_moments(15)
+    assert np.array_equal(once._moments(), more._moments())
+    perm, _ = _current_pair(0.5, num_moments=40, reorder=True)
+    assert perm._dmat.order is not None
+    assert np.array_equal(once._moments(), perm._moments())
+    sd, ref = _current_pair(_onsite_plain, precision='complex64')
+    assert _rel(sd._moments(), ref._moments()) < 1e-5
+
+
+def test_source_matches_oracle():
+    from kwant_amd import kpm
+    from kwant_amd.operator import Source
+    syst = FakeSystem(RUNS, seed=3)
+    bounds = _bounds(syst.h)
+    for mean in (True, False):
+        sd = kpm.SpectralDensity(syst, operator=Source(syst, _onsite_plain, where=WHERE),
+                                 num_vectors=4, num_moments=30, bounds=bounds, rng=11, mean=mean)
+        ref = ko.SpectralDensity(
+            syst.h, operator=oracle_source(syst.site_ranges, [MATS[s] for s in WHERE], WHERE,
+                                           [syst.hamiltonian(s, s) for s in WHERE]),
+            num_vectors=4, num_moments=30, bounds=bounds, rng=11, mean=mean)
+        assert _rel(sd._moments(), ref._moments()) < 1e-11
+
+
+def test_bond_records_against_numpy():
+    import torch
+    from kwant_amd import _engine as eng
+    n = 3000
+    syst = FakeSystem([(n, 1)], seed=4, density=0.002)
+    dev = eng.torch_device()
+    for prec in ('complex128', 'complex64'):
+        cdt, _ = eng.complex_dtype(prec)
+        npdt = np.complex64 if prec == 'complex64' else np.complex128
+        bond = eng.DeviceCSR(syst.h, prec)
+        tol = 1e-13 if prec == 'complex128' else 1e-6
+        for width in (1, 8, 32):
+            gen = torch.Generator(device=dev)
+            gen.manual_seed(width)
+            blocks = [torch.polar(torch.ones((n, width), dtype=torch.float64, device=dev),
+                                  torch.rand((n, width), generator=gen, device=dev,
+                                             dtype=torch.float64) * 6.283).to(cdt)
+                      for _ in range(4)]
+            bra, ket, v, w = blocks
+            hb, hk, hv, hw = (b.cpu().numpy().astype(np.complex128) for b in blocks)
+            csr = syst.h.tocsr()
+            r = np.repeat(np.arange(n), np.diff(csr.indptr))
+            c, h = csr.indices, csr.data.astype(npdt).astype(np.complex128)
+            want = 1j * (np.conj(h)[:, None] * np.conj(hb[c]) * hv[r]
+                         - np.conj(hw[r]) * h[:, None] * hk[c])
+            for cols in (False, True):
+                out = torch.empty((csr.nnz, width) if cols else (csr.nnz,),
+                                  dtype=torch.complex128, device=dev)
+                bond.bond_records(bra, ket, v, w, width, out, per_column=cols)
+                got = out.cpu().numpy()
+                ref = want if cols else want.sum(axis=1)
+                assert np.max(np.abs(got - ref)) < tol * np.max(np.abs(ref)), (prec, width, cols)
+
+
+def test_correlator_with_kwant_style_operators():
+    """Density / Current as Correlator operators run as sparse matrices on
+    the GPU; the result equals the oracle Correlator given the same
+    operators as matrices built independently (dense blocks)."""
+    from kwant_amd import kpm
+    from kwant_amd.operator import Current, Density
+    syst = FakeSystem([(40, 1)], seed=12, density=0.1)
+    hops = list(syst.graph)
+    bounds = _bounds(syst.h)
+    dens = np.diag(np.linspace(0.5, 1.5, 40)).astype(complex)
+    jmat = np.zeros((40, 40), dtype=complex)
+    for a, b in hops:
+        hab = syst.h[a, b]
+        jmat[b, a] += 1j * np.conj(hab)
+        jmat[a, b] += -1j * hab
+    ours = kpm.Correlator(syst, operator1=Density(syst, lambda s: dens[s, s]),
+                          operator2=Current(syst), num_moments=40, num_vectors=4, rng=13,
+                          bounds=bounds)
+    ref = ko.Correlator(syst.h, operator1=scipy.sparse.csr_matrix(dens),
+                        operator2=scipy.sparse.csr_matrix(jmat), num_moments=40,
+                        num_vectors=4, rng=13, bounds=bounds)
+    for mu, temp in ((0.1, 0.0), (-0.3, 0.05)):
+        assert abs(ours(mu, temp) - ref(mu, temp)) < 1e-10 * max(1.0, abs(ref(mu, temp)))

@@ -151,7 +151,7 @@ def test_layout_decomposition_matches_oracle():
         got = lay.site_sums(rows[None])[0]
         assert np.allclose(got, ref(bra, ket), rtol=1e-13, atol=1e-13)
         total = density_layout(Density(syst, onsite_arg, where=where, sum=True).bind())
-        assert abs(np.vdot(bra, total.matrix @ ket) - np.sum(ref(bra, ket))) < 1e-11
+        assert abs(np.vdot(bra, total.act_matrix @ ket) - np.sum(ref(bra, ket))) < 1e-11
 
 
 def test_plain_matrix_system():
@@ -159,3 +159,112 @@ def test_plain_matrix_system():
     op = Density(h, where=[3, 7])
     bra, ket = _vec(50, 15), _vec(50, 16)
     assert np.allclose(op(bra, ket), np.conj(bra[[3, 7]]) * ket[[3, 7]])
+
+
+# -- Current and Source --------------------------------------------------------
+
+from kwant_amd.operator import Current, Source, operator_layout  # noqa: E402
+from oracle.operator_oracle import current as oracle_current  # noqa: E402
+from oracle.operator_oracle import source as oracle_source  # noqa: E402
+
+
+def _blocks_of(syst, pairs):
+    return [syst.hamiltonian(a, b) for a, b in pairs]
+
+
+def test_oracle_chain_current_closed_form():
+    n, t = 12, 0.7 - 0.2j
+    h = scipy.sparse.diags([np.full(n - 1, t), np.full(n - 1, np.conj(t))], [1, -1]).tocsr()
+    ranges = [(0, 1, 0), (n, 0, n)]
+    psi = _vec(n, 20)
+    hops = [(a, a + 1) for a in range(n - 1)]
+    got = oracle_current(ranges, 1, hops, [h[a, b] for a, b in hops])(psi)
+    want = [2 * np.imag(t * np.conj(psi[a]) * psi[a + 1]) for a, _ in hops]
+    assert np.isrealobj(got)
+    assert np.allclose(got, want, rtol=0, atol=1e-13)
+
+
+def test_oracle_current_brute_force_and_continuity():
+    syst = FakeSystem([(12, 2)], seed=21, density=0.3)
+    hops = list(syst.graph)
+    m = _herm(2, 22)
+    bra, ket = _vec(24, 23), _vec(24, 24)
+    got = oracle_current(syst.site_ranges, m, hops, _blocks_of(syst, hops))(bra, ket)
+    for w, (a, b) in enumerate(hops):
+        j = np.zeros((24, 24), dtype=complex)         # dense J_w
+        hab = syst.h[2 * a:2 * a + 2, 2 * b:2 * b + 2].toarray()
+        j[2 * b:2 * b + 2, 2 * a:2 * a + 2] = 1j * hab.conj().T @ m
+        j[2 * a:2 * a + 2, 2 * b:2 * b + 2] = -1j * m @ hab
+        assert abs(got[w] - np.vdot(bra, j @ ket)) < 1e-12
+    # stationary continuity: no net current out of any site in an eigenstate
+    psi = np.linalg.eigh(syst.h.toarray())[1][:, 3]
+    cur = oracle_current(syst.site_ranges, 1, hops, _blocks_of(syst, hops))(psi)
+    out = np.zeros(12)
+    for (a, b), c in zip(hops, cur):
+        out[a] += c
+    assert np.max(np.abs(out)) < 1e-12
+
+
+def test_oracle_source_closed_form():
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sy = np.array([[0, -1j], [1j, 0]])
+    sz = np.diag([1.0, -1.0]).astype(complex)
+    ranges = [(0, 2, 0), (1, 0, 2)]
+    bra, ket = _vec(2, 25), _vec(2, 26)
+    got = oracle_source(ranges, sz, [0], [sx])(bra, ket)
+    assert abs(got[0] - np.vdot(bra, 2 * sy @ ket)) < 1e-13
+    assert abs(oracle_source(ranges, 1, [0], [sx])(bra, ket)[0]) < 1e-14
+
+
+@pytest.mark.parametrize("sum_", [False, True])
+def test_current_and_source_match_oracle(sum_):
+    syst = FakeSystem(RUNS, seed=27, density=0.05)
+    n = syst.site_ranges[-1][2]
+    bra, ket = _vec(n, 28), _vec(n, 29)
+    hops = list(syst.graph)[::7]
+    hops = hops + hops[:2]                          # repeated hoppings
+    for onsite in (1, 0.5):
+        cur = Current(syst, onsite, where=hops, sum=sum_)
+        ref = oracle_current(syst.site_ranges, onsite, hops, _blocks_of(syst, hops), sum=sum_)
+        assert np.allclose(cur(bra, ket), ref(bra, ket), rtol=1e-12, atol=1e-12)
+        assert np.allclose(cur(bra), ref(bra), rtol=1e-12, atol=1e-12)
+    sites = [3, 40, 40, 55]
+    mats = {s: _herm(syst.hamiltonian(s, s).shape[0], s) for s in set(sites)}
+    src = Source(syst, lambda s: mats[s], where=sites, sum=sum_)
+    ref = oracle_source(syst.site_ranges, [mats[s] for s in sites], sites,
+                        _blocks_of(syst, [(s, s) for s in sites]), sum=sum_)
+    assert np.allclose(src(bra, ket), ref(bra, ket), rtol=1e-12, atol=1e-12)
+
+
+def test_current_defaults_and_errors():
+    syst = FakeSystem([(10, 1)], seed=30, density=0.3)
+    cur = Current(syst)
+    assert cur.where.shape == (len(list(syst.graph)), 2)
+    with pytest.raises(ValueError):
+        Current(syst, where=[(0, 0)])               # not a hopping
+    plain = Current(syst.h)                           # a matrix: its off-diagonal entries
+    assert plain.where.shape[0] == syst.h.nnz - np.count_nonzero(syst.h.diagonal())
+    ket = _vec(10, 31)
+    assert np.allclose(cur.act(ket), cur.tocoo().tocsr() @ ket)
+
+
+def test_bond_layout_decomposition_matches_oracle():
+    """The bond kernel's algebra on the host: per entry (r, s, h)
+    i (conj(h) conj(bra_s) v_r - conj(w_r) h ket_s), v = M ket, w = M^H bra,
+    summed over each hopping's entries."""
+    syst = FakeSystem(RUNS, seed=32, density=0.05)
+    n = syst.site_ranges[-1][2]
+    bra, ket = _vec(n, 33), _vec(n, 34)
+    hops = list(syst.graph)[::5]
+    hops = hops + hops[:1]
+    for onsite in (1, 0.3):
+        lay = operator_layout(Current(syst, onsite, where=hops).bind())
+        ref = oracle_current(syst.site_ranges, onsite, hops, _blocks_of(syst, hops))
+        m = scipy.sparse.identity(n, format='csr') if lay.onsite is None else lay.onsite
+        v, w = m @ ket, m.conj().T @ bra
+        r, c, h = lay.rows, lay.cols, lay.values
+        rec = 1j * (np.conj(h) * np.conj(bra[c]) * v[r] - np.conj(w[r]) * h * ket[c])
+        got = lay.hop_sums(rec[None])[0]
+        assert np.allclose(got, ref(bra, ket), rtol=1e-12, atol=1e-12)
+        total = operator_layout(Current(syst, onsite, where=hops, sum=True).bind())
+        assert abs(np.vdot(bra, total.act_matrix @ ket) - np.sum(ref(bra, ket))) < 1e-11
