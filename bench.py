@@ -279,7 +279,7 @@ def surrogate_bench(epochs, tol, points, judge_points, judge_sets, log_fn):
     return out
 
 
-def density_bench(mat, ab, block, width, variant, dos_ms, steps, log_fn):
+def density_bench(mat, host_h, ab, block, width, variant, dos_ms, steps, log_fn):
     """Site-resolved density (kwant.operator.Density, sum=False, mean=True)
     on the same block: each step also reads w = M^H a_0 (here M = 1, so w =
     a_0) and writes one complex128 per row, the sum over the block's
@@ -304,13 +304,50 @@ def density_bench(mat, ab, block, width, variant, dos_ms, steps, log_fn):
     gbs = bstep / (ms_step * 1e-3) / 1e9
     log_fn("site-resolved density: %.4f ms/step (DOS step %.4f ms), %.0f GB/s algorithmic "
            "(%.1f%% of 8 TB/s)" % (ms_step, dos_ms, gbs, 100 * gbs / HBM_PEAK_GBS))
-    del rec, w
+    ket = rec.slot(rec.n).clone()
+    del rec
+    torch.cuda.empty_cache()
+    # Current over every hopping (onsite 1: v = ket, w = bra = a_0): one
+    # bond-record launch per moment, after the recursion step
+    import numpy as np
+    import scipy.sparse
+    csr = scipy.sparse.csr_matrix(host_h)
+    rows = np.repeat(np.arange(n), np.diff(csr.indptr))
+    keep = rows != csr.indices
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(rows[keep], minlength=n), out=indptr[1:])
+    hop = scipy.sparse.csr_matrix((csr.data[keep], csr.indices[keep], indptr), shape=csr.shape)
+    bond = eng.DeviceCSR(hop, "complex64")
+    out = torch.empty(bond.nnz, dtype=torch.complex128, device=block.device)
+    reps = 32
+    bond.bond_records(w, ket, ket, w, width, out)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        bond.bond_records(w, ket, ket, w, width, out)
+    end.record()
+    torch.cuda.synchronize()
+    ms_bond = start.elapsed_time(end) / reps
+    # distinct bytes: bra (= w) and ket (= v) once each, the entries, the records
+    bbond = 2 * n * width * 8 + bond.nnz * (8 + 4) + 4 * (n + 1) + bond.nnz * 16
+    gbond = bbond / (ms_bond * 1e-3) / 1e9
+    log_fn("current (bond records, %d hopping entries): %.4f ms per moment, %.0f GB/s "
+           "algorithmic (%.1f%% of 8 TB/s)" % (bond.nnz, ms_bond, gbond,
+                                               100 * gbond / HBM_PEAK_GBS))
+    del w, ket, out, bond
     torch.cuda.empty_cache()
     return {"mode": "KPM_DOTS_LOCAL (per-row sum over the 64 vectors of conj(w) a_n+1)",
             "ms_per_step": ms_step, "dos_ms_per_step": dos_ms,
             "algorithmic_bytes_per_step": bstep, "achieved_GBs": gbs,
             "frac_of_hbm_peak": gbs / HBM_PEAK_GBS, "steps": chunk * (steps // chunk),
-            "variant": variant}
+            "variant": variant,
+            "current": {"kernel": "bond_kernel (kpm_bond_records, onsite 1, every hopping)",
+                        "hopping_entries": int(hop.nnz),
+                        "ms_per_moment": ms_bond, "algorithmic_bytes": bbond,
+                        "achieved_GBs": gbond, "frac_of_hbm_peak": gbond / HBM_PEAK_GBS,
+                        "note": "per moment: this launch plus one DOS-shaped recursion step "
+                                "without dots"}}
+
 
 
 def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
@@ -648,8 +685,8 @@ def main():
     torch.cuda.empty_cache()
     site_density = None
     if not args.no_density:
-        site_density = density_bench(m64, (a, b), block64, width, args.variant, ms_step, 256,
-                                     log)
+        site_density = density_bench(m64, h, (a, b), block64, width, args.variant, ms_step,
+                                     256, log)
     del m64, block64
     torch.cuda.empty_cache()
     correlator = None

@@ -863,76 +863,105 @@ __global__ __launch_bounds__(WAVES * 64) void spmm_kernel(const int32_t* indptr,
 // r, column s = indices[k], value h = values[k]) and every block column c,
 //   t_k(c) = i (conj(h) conj(bra[s,c]) v[r,c] - conj(w[r,c]) h ket[s,c])
 // with v = M ket and w = M^H bra for the block-diagonal onsite matrix M.
-// FP64 products; summed over the block's columns like the site-resolved
-// records (TwoSum per lane, butterfly over the row's lanes, one rounding),
-// out[k] = complex128, or kept per column, out[k * W + c].  A lane group owns
-// one row; the groups of a wavefront run independent entry loops, and every
-// shuffle stays inside its group.
-template <typename T, int VEC, int LPR, int WAVES>
+// Summed over the block's columns the entry's h factors out:
+//   sum_c t_k(c) = i (conj(h) A - h B),  A = sum_c conj(bra[s,c]) v[r,c],
+//                                        B = sum_c conj(w[r,c]) ket[s,c],
+// so a column costs two FP64 complex products.  A lane group of LPR lanes
+// owns a row; lane `sub` holds G column groups of VEC, group g at columns
+// (g * LPR + sub) * VEC (each load instruction reads LPR * VEC contiguous
+// columns).  Column sums are plain FP64 additions in a fixed order (lane
+// sums, then a butterfly over the group), so a (width, value type) always
+// gives the same bits.  out[k] = complex128 record, or per column
+// out[k * W + c] = t_k(c) (per_column).
+template <typename T, int VEC, int LPR, int G, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void bond_kernel(
     const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
     const void* __restrict__ values, const T* __restrict__ bra, const T* __restrict__ ket,
     const T* __restrict__ v, const T* __restrict__ w, double* __restrict__ out, int per_column,
     int n_rows) {
-  constexpr int W = LPR * VEC;
+  constexpr int W = LPR * VEC * G;
   constexpr int RPW = 64 / LPR;
   using IO = LaneIO<T, VEC>;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const int row = (blockIdx.x * WAVES + wave) * RPW + lane / LPR;
-  const int col = (lane % LPR) * VEC;
+  // XCD-aware order: hardware block hb runs on XCD hb % kNumXcd; each XCD
+  // takes one contiguous range of row blocks, so the rows that a row's
+  // entries gather from stay in that XCD's L2
+  const int per = gridDim.x / kNumXcd;
+  const int lb = (blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
+  const int row = (lb * WAVES + wave) * RPW + lane / LPR;
+  const int sub = lane % LPR;
   if (row >= n_rows) return;  // whole lane groups leave together
-  const int64_t roff = 2 * (int64_t(row) * W + col);
-  const CLane<T, VEC> vr = IO::load(v + roff);
-  const CLane<T, VEC> wr = IO::load(w + roff);
+  auto colof = [&](int g) { return (g * LPR + sub) * VEC; };
+  // the row's v and w stay in their storage type (registers decide the
+  // occupancy, and occupancy the number of gathers in flight)
+  CLane<T, VEC> vv[G], ww[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    vv[g] = IO::load(v + 2 * (int64_t(row) * W + colof(g)));
+    ww[g] = IO::load(w + 2 * (int64_t(row) * W + colof(g)));
+  }
   d2v* rec = reinterpret_cast<d2v*>(out);
   const int k1 = indptr[row + 1];
   for (int k = indptr[row]; k < k1; ++k) {
     const int s = indices[k];
-    T hr, hi;
-    load_value<T>(values, k, hr, hi);
-    const CLane<T, VEC> bs = IO::load(bra + 2 * (int64_t(s) * W + col));
-    const CLane<T, VEC> ks = IO::load(ket + 2 * (int64_t(s) * W + col));
-    double tr[VEC], ti[VEC];
+    T hr_, hi_;
+    load_value<T>(values, k, hr_, hi_);
+    const double hr = hr_, hi = hi_;
+    CLane<T, VEC> bs[G], ks[G];
 #pragma unroll
-    for (int u = 0; u < VEC; ++u) {
-      const double h_r = hr, h_i = hi;
-      // p = h bra_s, t1 = conj(p) v_r
-      const double pr = fma(h_r, double(bs.re[u]), -(h_i * double(bs.im[u])));
-      const double pi = fma(h_r, double(bs.im[u]), h_i * double(bs.re[u]));
-      const double t1r = fma(pr, double(vr.re[u]), pi * double(vr.im[u]));
-      const double t1i = fma(pr, double(vr.im[u]), -(pi * double(vr.re[u])));
-      // q = h ket_s, t2 = conj(w_r) q
-      const double qr = fma(h_r, double(ks.re[u]), -(h_i * double(ks.im[u])));
-      const double qi = fma(h_r, double(ks.im[u]), h_i * double(ks.re[u]));
-      const double t2r = fma(double(wr.re[u]), qr, double(wr.im[u]) * qi);
-      const double t2i = fma(double(wr.re[u]), qi, -(double(wr.im[u]) * qr));
-      // i (t1 - t2)
-      tr[u] = t2i - t1i;
-      ti[u] = t1r - t2r;
+    for (int g = 0; g < G; ++g) {
+      bs[g] = IO::load(bra + 2 * (int64_t(s) * W + colof(g)));
+      ks[g] = IO::load(ket + 2 * (int64_t(s) * W + colof(g)));
     }
     if (per_column) {
 #pragma unroll
-      for (int u = 0; u < VEC; ++u) rec[int64_t(k) * W + col + u] = d2v{tr[u], ti[u]};
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) {
+          const double br = bs[g].re[u], bi = bs[g].im[u];
+          const double kr = ks[g].re[u], ki = ks[g].im[u];
+          const double vr = vv[g].re[u], vi = vv[g].im[u];
+          const double wr = ww[g].re[u], wi = ww[g].im[u];
+          // a = conj(bra) v, b = conj(w) ket, t = i (conj(h) a - h b)
+          const double ar = fma(br, vr, bi * vi);
+          const double ai = fma(br, vi, -(bi * vr));
+          const double bre = fma(wr, kr, wi * ki);
+          const double bim = fma(wr, ki, -(wi * kr));
+          const double dr = fma(hr, ar, hi * ai) - fma(hr, bre, -(hi * bim));
+          const double di = fma(hr, ai, -(hi * ar)) - fma(hr, bim, hi * bre);
+          rec[int64_t(k) * W + colof(g) + u] = d2v{-di, dr};
+        }
       continue;
     }
-    double sr = 0.0, cr = 0.0, si = 0.0, ci = 0.0;
+    double ar = 0.0, ai = 0.0, bre = 0.0, bim = 0.0;
 #pragma unroll
-    for (int u = 0; u < VEC; ++u) {
-      acc2(sr, cr, tr[u]);
-      acc2(si, ci, ti[u]);
-    }
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int u = 0; u < VEC; ++u) {
+        const double br = bs[g].re[u], bi = bs[g].im[u];
+        const double kr = ks[g].re[u], ki = ks[g].im[u];
+        const double vr = vv[g].re[u], vi = vv[g].im[u];
+        const double wr = ww[g].re[u], wi = ww[g].im[u];
+        ar = fma(br, vr, fma(bi, vi, ar));
+        ai = fma(br, vi, fma(-bi, vr, ai));
+        bre = fma(wr, kr, fma(wi, ki, bre));
+        bim = fma(wr, ki, fma(-wi, kr, bim));
+      }
 #pragma unroll
     for (int off = 1; off < LPR; off <<= 1) {
-      const double osr = __shfl_xor(sr, off), ocr = __shfl_xor(cr, off);
-      const double osi = __shfl_xor(si, off), oci = __shfl_xor(ci, off);
-      acc2(sr, cr, osr);
-      cr += ocr;
-      acc2(si, ci, osi);
-      ci += oci;
+      ar += __shfl_xor(ar, off);
+      ai += __shfl_xor(ai, off);
+      bre += __shfl_xor(bre, off);
+      bim += __shfl_xor(bim, off);
     }
-    if (col == 0) rec[k] = d2v{sr + cr, si + ci};
+    if (sub == 0) {
+      // i (conj(h) A - h B)
+      const double dr = fma(hr, ar, hi * ai) - fma(hr, bre, -(hi * bim));
+      const double di = fma(hr, ai, -(hi * ar)) - fma(hr, bim, hi * bre);
+      rec[k] = d2v{-di, dr};
+    }
   }
 }
 
@@ -1041,12 +1070,14 @@ int launch_spmm_one(const kpm_csr* op, const void* x, int64_t xs, void* y, int64
   return check_hip(hipGetLastError(), "spmm_kernel launch");
 }
 
-template <typename T, int VEC, int LPR>
+template <typename T, int VEC, int LPR, int G>
 int launch_bond_one(const kpm_csr* b, const void* bra, const void* ket, const void* v,
                     const void* w, int per_column, double* out, hipStream_t stream) {
   constexpr int RPB = kWaves * 64 / LPR;
-  dim3 grid((b->n_rows + RPB - 1) / RPB), block(kWaves * 64);
-  hipLaunchKernelGGL((bond_kernel<T, VEC, LPR, kWaves>), grid, block, 0, stream, b->indptr,
+  // a multiple of kNumXcd blocks (the kernel's XCD-aware order needs it)
+  const int blocks = (b->n_rows + RPB - 1) / RPB;
+  dim3 grid((blocks + kNumXcd - 1) / kNumXcd * kNumXcd), block(kWaves * 64);
+  hipLaunchKernelGGL((bond_kernel<T, VEC, LPR, G, kWaves>), grid, block, 0, stream, b->indptr,
                      b->indices, b->values, reinterpret_cast<const T*>(bra),
                      reinterpret_cast<const T*>(ket), reinterpret_cast<const T*>(v),
                      reinterpret_cast<const T*>(w), out, per_column, b->n_rows);
@@ -1193,21 +1224,6 @@ struct ChebLauncher {
   }
 };
 
-struct BondLauncher {
-  const kpm_csr* b;
-  const void* bra;
-  const void* ket;
-  const void* v;
-  const void* w;
-  int per_column;
-  double* out;
-  hipStream_t s;
-  template <typename T, int V, int LP>
-  int operator()() {
-    return launch_bond_one<T, V, LP>(b, bra, ket, v, w, per_column, out, s);
-  }
-};
-
 struct SpmmLauncher {
   const kpm_csr* op;
   const void* x;
@@ -1264,15 +1280,37 @@ int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x, int
 
 int launch_bond(int width, const kpm_csr* b, const void* bra, const void* ket, const void* v,
                 const void* w, int per_column, double* out, hipStream_t stream) {
-  if (width < 1 || width > 64 || (width & (width - 1)) != 0)
-    return fail(KPM_ERR_UNSUPPORTED, "bond width must be a power of two <= 64");
-  BondLauncher l{b, bra, ket, v, w, per_column, out, stream};
+  // lanes per row at most 8 (three butterfly levels), the rest of the row in
+  // G column groups per lane
+#define KPM_BOND(T, VEC, LPR, G) \
+  return launch_bond_one<T, VEC, LPR, G>(b, bra, ket, v, w, per_column, out, stream)
   if (b->value_type == KPM_C64) {
-    if (width >= 2) return dispatch_shape<float>(2, width / 2, l);
-    return dispatch_shape<float>(1, 1, l);
+    switch (width) {
+      case 1: KPM_BOND(float, 1, 1, 1);
+      case 2: KPM_BOND(float, 2, 1, 1);
+      case 4: KPM_BOND(float, 2, 2, 1);
+      case 8: KPM_BOND(float, 2, 4, 1);
+      case 16: KPM_BOND(float, 2, 8, 1);
+      case 32: KPM_BOND(float, 2, 8, 2);
+      case 64: KPM_BOND(float, 2, 8, 4);
+      default: break;
+    }
+  } else if (b->value_type == KPM_C128) {
+    switch (width) {
+      case 1: KPM_BOND(double, 1, 1, 1);
+      case 2: KPM_BOND(double, 1, 2, 1);
+      case 4: KPM_BOND(double, 1, 4, 1);
+      case 8: KPM_BOND(double, 1, 8, 1);
+      case 16: KPM_BOND(double, 1, 8, 2);
+      case 32: KPM_BOND(double, 1, 8, 4);
+      case 64: KPM_BOND(double, 1, 8, 8);
+      default: break;
+    }
+  } else {
+    return fail(KPM_ERR_ARG, "unknown value type");
   }
-  if (b->value_type == KPM_C128) return dispatch_shape<double>(1, width, l);
-  return fail(KPM_ERR_ARG, "unknown value type");
+#undef KPM_BOND
+  return fail(KPM_ERR_UNSUPPORTED, "bond width must be a power of two <= 64");
 }
 
 }  // namespace kpm
