@@ -350,6 +350,37 @@ def density_bench(mat, host_h, ab, block, width, variant, dos_ms, steps, log_fn)
 
 
 
+def transform_bench(moments, channels, log_fn):
+    """Moment-to-energy transform of a site-resolved expansion: the DCT-III
+    of (moments x channels) complex moments to 2 x moments nodes, on the GPU
+    (FP64 matrix-core GEMM, host copies included) and with scipy on one host
+    core (the reference's transform)."""
+    import numpy as np
+    import torch
+    from kwant_amd import kpm
+    rng = np.random.default_rng(0)
+    mom = rng.normal(size=(moments, channels)) + 1j * rng.normal(size=(moments, channels))
+    kpm._calc_fft_moments(mom[:, :64].copy())         # warm-up (library, GEMM)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rho_gpu, _ = kpm._calc_fft_moments(mom)
+    t_gpu = time.perf_counter() - t0
+    sub = max(1, channels // 16)
+    t0 = time.perf_counter()
+    saved = kpm.GPU_TRANSFORM_MIN
+    kpm.GPU_TRANSFORM_MIN = 1 << 62
+    rho_cpu, _ = kpm._calc_fft_moments(mom[:, :sub])
+    kpm.GPU_TRANSFORM_MIN = saved
+    t_cpu = (time.perf_counter() - t0) * channels / sub
+    err = float(np.max(np.abs(rho_gpu[:, :sub] - rho_cpu)) / np.max(np.abs(rho_cpu)))
+    log_fn("transform (DCT-III, %d moments x %d channels): GPU %.3f s (host copies "
+           "included), scipy %.2f s (1 core, from %d channels); max rel diff %.1e"
+           % (moments, channels, t_gpu, t_cpu, sub, err))
+    return {"moments": moments, "channels": channels, "gpu_s": t_gpu, "scipy_s": t_cpu,
+            "scipy_sample_channels": sub, "max_rel_diff": err,
+            "gemm_flops": 8.0 * 2 * moments * moments * channels}
+
+
 def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
     """The batched disorder ensemble: ``count`` cubic 4-orbital systems with
     L ~ U{20..60}, generated on the GPU, ``moments`` moments with ``width``
@@ -695,6 +726,9 @@ def main():
     ensemble = None
     if not args.no_ensemble:
         ensemble = ensemble_bench(256, 512, 16, log, args.ens_variant, not args.ens_eager)
+    transform = None
+    if not args.no_density:
+        transform = transform_bench(1024, 65536, log)
     surrogate = None
     if not args.no_surrogate:
         surrogate = surrogate_bench(args.sur_epochs, args.sur_tol, args.sur_points, 32, 32, log)
@@ -734,6 +768,7 @@ def main():
         "correlator": correlator,
         "ensemble": ensemble,
         "site_density": site_density,
+        "transform": transform,
         "surrogate": surrogate,
     }
     line.update(check)

@@ -237,10 +237,69 @@ def _chebyshev_nodes(n_sampling):
     return np.cos(raw + step / 2)
 
 
+# Moment arrays with at least this many (moment x channel) entries are
+# transformed on the GPU (channels: the trailing axes, e.g. sites x vectors of
+# a site-resolved density with mean=False); smaller ones on the host.
+GPU_TRANSFORM_MIN = 1 << 22
+
+
+def _transform_on_gpu(moments):
+    return moments.ndim > 1 and moments.size >= GPU_TRANSFORM_MIN
+
+
+def _gemm_fp64(a, channels):
+    """a (rows x M, host) @ channels (M x C, host) as one FP64 matrix-core
+    GEMM on the GPU (kpm_corr_gemm: mu += conj(a) @ b^T, a real here)."""
+    import torch
+    dev = eng.torch_device()
+    ta = torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).to(dev)
+    tb = torch.from_numpy(np.ascontiguousarray(channels.T, dtype=np.complex128)).to(dev)
+    mu = torch.zeros((a.shape[0], channels.shape[1]), dtype=torch.complex128, device=dev)
+    eng.corr_gemm(ta, tb, mu, eng.GEMM_FP64)
+    out = mu.cpu().numpy()
+    return out if np.iscomplexobj(channels) else out.real.copy()
+
+
+def _dct3_gemm(moments):
+    """The unnormalised DCT-III of the zero-padded moments along axis 0,
+    y_k = x_0 + 2 sum_{m>=1} x_m cos(pi m (2k + 1) / 2n) for n = SAMPLING * M
+    nodes, as a GEMM on the GPU: the transform of kwant/kpm.py:1212-1216 for
+    many channels at once."""
+    count = moments.shape[0]
+    n = SAMPLING * count
+    k = np.arange(n, dtype=np.float64)[:, None]
+    m = np.arange(count, dtype=np.float64)[None, :]
+    basis = 2 * np.cos(np.pi * m * (2 * k + 1) / (2 * n))
+    basis[:, 0] = 1.0
+    flat = moments.reshape(count, -1)
+    return _gemm_fp64(basis, flat).reshape((n,) + moments.shape[1:])
+
+
+def _chebval_gemm(x, moments):
+    """sum_m moments[m] T_m(x) for every channel (numpy.polynomial's chebval
+    for coefficient arrays), as a GEMM on the GPU; x is 1-D."""
+    count = moments.shape[0]
+    t = np.empty((len(x), count))
+    t[:, 0] = 1.0
+    if count > 1:
+        t[:, 1] = x
+    for j in range(2, count):
+        t[:, j] = 2 * x * t[:, j - 1] - t[:, j - 2]
+    flat = moments.reshape(count, -1)
+    return _gemm_fp64(t, flat).reshape((len(x),) + moments.shape[1:])
+
+
 def _calc_fft_moments(moments):
     """Densities (and Gauss-Chebyshev weights) at the 2M Chebyshev nodes by an
-    unnormalised DCT-III of the zero-padded moments (kwant/kpm.py:1197-1222)."""
+    unnormalised DCT-III of the zero-padded moments (kwant/kpm.py:1197-1222).
+    Large multi-channel arrays take the DCT as a GPU GEMM."""
     moments = np.asarray(moments)
+    if _transform_on_gpu(moments):
+        count = moments.shape[0]
+        nodes = _chebyshev_nodes(SAMPLING * count)
+        gammas = _dct3_gemm(moments)[::-1]
+        rho = np.transpose(np.divide(gammas.transpose(), np.pi * np.sqrt(1 - nodes ** 2)))
+        return rho, gammas
     count, *extra = moments.shape
     n_sampling = SAMPLING * count
     padded = np.zeros([n_sampling] + extra, dtype=moments.dtype)
@@ -442,6 +501,12 @@ class SpectralDensity:
         g_e = np.pi * np.sqrt(1 - e) * np.sqrt(1 + e)
         moments = self._moments()
         moments[1:] = 2 * moments[1:]
+        if _transform_on_gpu(moments) and e.ndim <= 1:
+            vals = _chebval_gemm(np.atleast_1d(e).astype(np.float64), moments)
+            vals = np.moveaxis(vals, 0, -1)          # chebval's layout: channels, then x
+            if e.ndim == 0:
+                vals = vals[..., 0]
+            return np.transpose(vals / g_e)
         return np.transpose(chebval(e, moments) / g_e)
 
     def integrate(self, distribution_function=None):
