@@ -387,6 +387,7 @@ class BlockRecursion:
         dev = torch_device()
         n = mat.shape[0]
         self.mat, self.width, self.mode, self.n_rows = mat, width, mode, n
+        self.variant = variant
         self.cdt, self.vt = complex_dtype(mat.precision)
         self.plan = plan if plan is not None else Plan(n, width, self.vt, variant=variant)
         ab = np.asarray(ab, dtype=np.float64).reshape(-1)

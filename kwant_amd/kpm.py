@@ -401,30 +401,7 @@ class SpectralDensity:
         self.eps = eps
         hamiltonian = _as_csr(hamiltonian, params)
 
-        self._op_matrix = None
-        self._layout = None
-        if operator is None:
-            self.operator = None
-        elif _is_local_operator(operator):
-            self.operator = operator.bind(params=params)
-            # Density, Source and Current are evaluated on the GPU: sum=True
-            # as the matrix operator sum_i Q_i, sum=False per site (inside the
-            # recursion kernel) or per hopping (bond records)
-            self._layout = operator_layout(self.operator)
-            if self._layout is not None and \
-                    self._layout.act_matrix.shape[0] != hamiltonian.shape[0]:
-                raise ValueError('vector is incorrect shape')
-            if self._layout is not None and self._layout.sum:
-                self._op_matrix = self._layout.act_matrix
-        elif callable(operator):
-            self.operator = operator
-        elif hasattr(operator, 'dot'):
-            op = csr_matrix(operator)
-            self._op_matrix = op
-            self.operator = lambda bra, ket: np.vdot(bra, op.dot(ket))
-        else:
-            raise ValueError("Parameter 'operator' has no '.dot' attribute "
-                             "and is not callable.")
+        self._set_operator(operator, params, hamiltonian)
 
         self.mean = mean
         rng = ensure_rng(rng)
@@ -450,6 +427,7 @@ class SpectralDensity:
         if num_moments <= 0 or num_moments != int(num_moments):
             raise ValueError("'num_moments' must be a positive integer")
 
+        self._rng, self._default_vectors = rng, vector_factory is None
         if vector_factory is None:
             self._vector_factory = _VectorFactory(
                 RandomVectors(hamiltonian, rng=rng), num_vectors=num_vectors,
@@ -474,6 +452,33 @@ class SpectralDensity:
 
         self.kernel = kernel if kernel is not None else jackson_kernel
         self.densities, self._gammas = _calc_fft_moments(self._moments())
+
+    def _set_operator(self, operator, params, hamiltonian):
+        """Normalise ``operator`` (kwant/kpm.py:186-198)."""
+        self._op_matrix = None
+        self._layout = None
+        if operator is None:
+            self.operator = None
+        elif _is_local_operator(operator):
+            self.operator = operator.bind(params=params)
+            # Density, Source and Current are evaluated on the GPU: sum=True
+            # as the matrix operator sum_i Q_i, sum=False per site (inside the
+            # recursion kernel) or per hopping (bond records)
+            self._layout = operator_layout(self.operator)
+            if self._layout is not None and \
+                    self._layout.act_matrix.shape[0] != hamiltonian.shape[0]:
+                raise ValueError('vector is incorrect shape')
+            if self._layout is not None and self._layout.sum:
+                self._op_matrix = self._layout.act_matrix
+        elif callable(operator):
+            self.operator = operator
+        elif hasattr(operator, 'dot'):
+            op = csr_matrix(operator)
+            self._op_matrix = op
+            self.operator = lambda bra, ket: np.vdot(bra, op.dot(ket))
+        else:
+            raise ValueError("Parameter 'operator' has no '.dot' attribute "
+                             "and is not callable.")
 
     @property
     def energies(self):
@@ -545,6 +550,22 @@ class SpectralDensity:
         num_vectors = self._vector_factory.num_vectors - self.num_vectors
         self._update_moments_list(self.num_moments, self.num_vectors + num_vectors)
         self.densities, self._gammas = _calc_fft_moments(self._moments())
+
+    def save(self, path):
+        """Write the expansion (moments, resume state, vectors drawn, the
+        random generator's state) to ``path``; see kwant_amd.persist."""
+        from . import persist
+        persist.save(self, path)
+
+    @classmethod
+    def load(cls, path, hamiltonian, params=None, operator=None, kernel=None,
+             vector_factory=None):
+        """An expansion saved by ``save``, ready for add_moments /
+        add_vectors; ``hamiltonian`` and ``operator`` are the ones it was
+        made with."""
+        from . import persist
+        return persist.load(path, hamiltonian, params=params, operator=operator,
+                            kernel=kernel, vector_factory=vector_factory)
 
     def _moments(self):
         """real_if_close, vectors to axis 1, mean, / a, kernel
@@ -628,50 +649,52 @@ class SpectralDensity:
             return [None] * blk.count
         return list(blk.moments)
 
-    def _new_block(self, vectors, width):
-        ab = (self._a, self._b)
+    def _block_mode(self, width):
+        """(dots mode, kernel variant) of a new block of ``width`` vectors."""
         if isinstance(self._layout, BondLayout) and not self._layout.sum:
-            import torch
-            blk = _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_NONE,
-                            variant=self._launch_shape(width, eng.DOTS_NONE))
+            mode, tune = eng.DOTS_NONE, eng.DOTS_NONE
+        elif isinstance(self._layout, SiteLayout) and not self._layout.sum:
+            mode = eng.DOTS_LOCAL if self.mean else eng.DOTS_LOCAL_COLS
+            tune = eng.DOTS_PROJECT
+        elif self.operator is None:
+            mode = tune = eng.DOTS_DOUBLING
+        elif self._op_matrix is not None:
+            mode = tune = eng.DOTS_PROJECT
+        else:
+            mode = tune = eng.DOTS_NONE
+        return mode, self._launch_shape(width, tune)
+
+    def _new_block(self, vectors, width):
+        mode, variant = self._block_mode(width)
+        blk = _DosBlock(self._dmat, (self._a, self._b), vectors, width, mode, variant=variant)
+        self._attach(blk, width)
+        return blk
+
+    def _attach(self, blk, width):
+        """The operator's per-block data, formed from alpha_0 (ring slot 0 of
+        a block that has not stepped yet)."""
+        import torch
+        a0 = blk.rec.ring[0]
+        if isinstance(self._layout, BondLayout) and not self._layout.sum:
             # a_0 is kept (the two-slot ring overwrites it); w = M^H a_0
-            blk.bra = blk.rec.ring[0].clone()
+            blk.bra = a0.clone()
             blk.w = blk.bra
             onsite = self._bond_device()[2]
             if onsite is not None:
                 blk.w = torch.empty_like(blk.bra)
                 onsite[1].spmm(blk.bra, blk.w, width)
-            return blk
-        if isinstance(self._layout, SiteLayout) and not self._layout.sum:
-            import torch
-            mode = eng.DOTS_LOCAL if self.mean else eng.DOTS_LOCAL_COLS
-            blk = _DosBlock(self._dmat, ab, vectors, width, mode,
-                            variant=self._launch_shape(width, eng.DOTS_PROJECT))
-            # w = M^H alpha_0: <w|alpha_n> per row is the row's part of
-            # vdot(alpha_0, M alpha_n) (kwant/operator.pyx:791-797)
-            op_h = eng.DeviceCSR(self._layout.matrix.conj().T.tocsr(), self._dmat.precision,
+        elif blk.rec.mode in (eng.DOTS_LOCAL, eng.DOTS_LOCAL_COLS, eng.DOTS_PROJECT):
+            # w = O^H alpha_0, so <w|alpha_n> = vdot(alpha_0, O alpha_n); per row
+            # for a site operator (kwant/operator.pyx:791-797)
+            op = self._layout.matrix if blk.rec.mode != eng.DOTS_PROJECT else self._op_matrix
+            op_h = eng.DeviceCSR(op.conj().T.tocsr(), self._dmat.precision,
                                  order=self._dmat.order)
-            w = torch.empty_like(blk.rec.ring[0])
-            op_h.spmm(blk.rec.ring[0], w, width)
+            w = torch.empty_like(a0)
+            op_h.spmm(a0, w, width)
             blk.rec.w = w
-            return blk
-        if self.operator is None:
-            return _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_DOUBLING,
-                             variant=self._launch_shape(width, eng.DOTS_DOUBLING))
-        if self._op_matrix is not None:
-            import torch
-            blk = _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_PROJECT,
-                            variant=self._launch_shape(width, eng.DOTS_PROJECT))
-            # w = O^H alpha_0, so <w|alpha_n> = vdot(alpha_0, O alpha_n)
-            op_h = eng.DeviceCSR(self._op_matrix.conj().T.tocsr(), self._dmat.precision,
-                                 order=self._dmat.order)
-            w = torch.empty_like(blk.rec.ring[0])
-            op_h.spmm(blk.rec.ring[0], w, width)
-            blk.rec.w = w
-            blk._op_h = op_h
-            return blk
-        return _DosBlock(self._dmat, ab, vectors, width, eng.DOTS_NONE,
-                         variant=self._launch_shape(width, eng.DOTS_NONE))
+        elif blk.rec.mode == eng.DOTS_NONE:
+            # a user callable gets alpha_0 on the host
+            blk.alpha0 = self._dmat.rows_out(a0.cpu().numpy().astype(np.complex128))
 
     def _expand_new(self, blk, n_moments):
         rec = blk.rec
@@ -864,7 +887,7 @@ class SpectralDensity:
         recursion runs on the GPU; each new alpha_n is handed to the callable
         on the host, one moment per step."""
         rec = blk.rec
-        alpha0 = self._dmat.rows_out(rec.slot(0).cpu().numpy().astype(np.complex128))
+        alpha0 = blk.alpha0     # kept on the host: the two-slot ring overwrites it
         moms = [list(old[c]) if old is not None else [] for c in range(blk.count)]
         if m_old == 0:
             alpha1 = None

@@ -235,6 +235,22 @@ def test_operator_matrix_matches_oracle():
     assert rel(np.array(gpu._moments_list), np.array(ora._moments_list), scale) < 1e-11
 
 
+def test_operator_callable_resume_matches_oracle():
+    """A user callable gets alpha_0 and alpha_n on the host; after
+    add_moments alpha_0 must still be the first vector (the two-slot ring no
+    longer holds it)."""
+    h = gue(24, 8)
+    op = gue(24, 9)
+    user = lambda bra, ket: np.vdot(bra, op @ ket)
+    kw = dict(num_moments=30, num_vectors=3, bounds=gershgorin(h), rng=4, operator=user)
+    gpu = kpm().SpectralDensity(h, **kw)
+    ora = ko.SpectralDensity(h, **kw)
+    gpu.add_moments(15)
+    ora.add_moments(15)
+    scale = 24 * np.abs(op).max()
+    assert rel(np.array(gpu._moments_list), np.array(ora._moments_list), scale) < 1e-11
+
+
 def test_operator_user_callable():
     """kwant/tests/test_kpm.py:378-390."""
     h = gue(20, 10)

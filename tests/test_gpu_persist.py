@@ -1,0 +1,81 @@
+"""Save / load of an expansion (kwant_amd.persist): after a round trip
+through the file, add_moments and add_vectors give bitwise the moments of
+the uninterrupted run, for every moment path (doubling, matrix operator,
+site-resolved Density, Current bond records, user callable), both
+precisions and a reordered matrix."""
+import numpy as np
+import pytest
+
+from fake_system import FakeSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def _bounds(h):
+    r = float(np.max(np.asarray(abs(h).sum(axis=1)).ravel()))
+    return (-r, r)
+
+
+SYST = FakeSystem([(30, 2), (20, 1), (10, 3)], seed=3)
+
+
+def _operators():
+    from kwant_amd.operator import Current, Density
+    op = (SYST.h + SYST.h.T.conj()) * 0.3
+    return {
+        'dos': None,
+        'matrix': op,
+        'density': Density(SYST, 0.5, where=[3, 40, 40, 55]),
+        'density_per_vector': Density(SYST, 0.5, where=[3, 40]),
+        'current': Current(SYST, 1, where=list(SYST.graph)[::9]),
+        'callable': (lambda bra, ket: np.vdot(bra, op @ ket)),
+    }
+
+
+def _make(name, precision='complex128', **kw):
+    from kwant_amd import kpm
+    ops = _operators()
+    mean = name != 'density_per_vector'
+    return kpm.SpectralDensity(SYST, operator=ops[name], num_moments=30, num_vectors=10,
+                               bounds=_bounds(SYST.h), rng=5, precision=precision,
+                               mean=mean, **kw), ops[name]
+
+
+@pytest.mark.parametrize("name", ['dos', 'matrix', 'density', 'density_per_vector', 'current',
+                                  'callable'])
+def test_round_trip_resumes_bitwise(name, tmp_path):
+    from kwant_amd import kpm
+    a, _ = _make(name)
+    a.add_moments(15)
+    a.add_vectors(3)
+    b, op = _make(name)
+    path = tmp_path / ("%s.npz" % name)
+    b.save(path)
+    c = kpm.SpectralDensity.load(path, SYST, operator=op)
+    assert np.array_equal(c._moments(), b._moments())
+    c.add_moments(15)
+    c.add_vectors(3)
+    assert np.array_equal(c._moments(), a._moments())
+    assert np.array_equal(c.densities, a.densities)
+
+
+@pytest.mark.parametrize("precision,reorder", [('complex64', False), ('complex128', True)])
+def test_round_trip_precision_and_order(precision, reorder, tmp_path):
+    from kwant_amd import kpm
+    a, _ = _make('dos', precision=precision, reorder=reorder)
+    a.add_moments(11)
+    b, _ = _make('dos', precision=precision, reorder=reorder)
+    b.save(tmp_path / "x.npz")
+    c = kpm.SpectralDensity.load(tmp_path / "x.npz", SYST)
+    assert (c._dmat.order is not None) == reorder
+    c.add_moments(11)
+    assert np.array_equal(c._moments(), a._moments())
+
+
+def test_load_checks_the_hamiltonian(tmp_path):
+    import scipy.sparse as sp
+    from kwant_amd import kpm
+    b, _ = _make('dos')
+    b.save(tmp_path / "x.npz")
+    with pytest.raises(ValueError):
+        kpm.SpectralDensity.load(tmp_path / "x.npz", sp.identity(7, format='csr'))
