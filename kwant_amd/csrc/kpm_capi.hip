@@ -122,7 +122,8 @@ int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch) {
 
 int kpm_variant_count(int32_t width, int32_t value_type, int32_t* count) {
   if (!count) return fail(KPM_ERR_ARG, "count is NULL");
-  *count = variants_for(width, value_type, nullptr, 0);
+  KernelShape ks[kMaxVariants];
+  *count = variants_for(width, value_type, ks, kMaxVariants);
   return KPM_OK;
 }
 

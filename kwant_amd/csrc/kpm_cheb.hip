@@ -1154,13 +1154,32 @@ int variants_for(int width, int value_type, KernelShape* out, int max_out) {
         }
     }
   };
+  // Further interleave chunks, listed after every shape above so that those
+  // keep their variant numbers.
+  auto add_chunks = [&](int lpr, int vec) {
+    const int n_unroll = lpr >= 4 ? 2 : 1;
+    const int first = lpr >= 4 && lpr < 32 ? 2 : 1;
+    for (int ui = 0; ui < n_unroll; ++ui) {
+      const int unroll = ui == 0 ? first : 3 - first;
+      for (int chunk : {4, 16})
+        for (int spare = 0; spare <= 1; ++spare) {
+          if (n < max_out) out[n] = KernelShape{lpr, vec, kWaves, 1, unroll, chunk, spare, 0};
+          ++n;
+        }
+    }
+  };
   if (value_type == KPM_C64) {
     if (width >= 2) add(width / 2, 2);
     if (width <= 64) add(width, 1);
+    if (width >= 2) add_chunks(width / 2, 2);
+    if (width <= 64) add_chunks(width, 1);
   } else if (value_type == KPM_C128) {
-    if (width <= 64) add(width, 1);
+    if (width <= 64) {
+      add(width, 1);
+      add_chunks(width, 1);
+    }
   }
-  return n;
+  return n < max_out ? n : max_out;  // never more than the caller's array holds
 }
 
 namespace {
