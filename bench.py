@@ -381,6 +381,46 @@ def transform_bench(moments, channels, log_fn):
             "gemm_flops": 8.0 * 2 * moments * moments * channels}
 
 
+def assembly_bench(cells, sample_cells, log_fn):
+    """Hamiltonian assembly of a disordered graphene flake with a Peierls
+    phase: kwant_amd.assembly (value functions evaluated over arrays) for
+    cells x cells unit cells, against the oracle's one-site-at-a-time loop
+    (kwant's make_sparse_full) on sample_cells^2 cells, scaled by the site
+    count."""
+    import numpy as np
+    from kwant_amd.assembly import honeycomb_system
+    from oracle import assembly_oracle
+
+    def onsite(pos, w):
+        x = np.sin(pos[:, 0] * 12.9898 + pos[:, 1] * 78.233) * 43758.5453
+        return w * (x - np.floor(x) - 0.5)
+
+    def hop(to, fr, t, phi):
+        return -t * np.exp(1j * phi * (to[:, 0] - fr[:, 0]) * (to[:, 1] + fr[:, 1]))
+
+    def build(c):
+        syst = honeycomb_system(c)
+        syst.set_onsite(onsite)
+        syst._hops = [(d, src, hop) for d, src, _ in syst._hops]
+        return syst
+
+    params = dict(w=0.5, t=1.0, phi=1e-3)
+    t0 = time.perf_counter()
+    big = build(cells)
+    h = big.hamiltonian_submatrix(params=params)
+    t_vec = time.perf_counter() - t0
+    small = build(sample_cells)
+    t0 = time.perf_counter()
+    ref = assembly_oracle.csr(small, params)
+    t_loop = (time.perf_counter() - t0) * (cells / sample_cells) ** 2
+    err = float(abs(small.hamiltonian_submatrix(params=params) - ref).max())
+    log_fn("assembly (graphene %dx%d cells, %d entries): vectorised %.2f s, one-at-a-time "
+           "loop %.0f s (from %dx%d cells); max |diff| %.1e on the sample"
+           % (cells, cells, h.nnz, t_vec, t_loop, sample_cells, sample_cells, err))
+    return {"cells": cells, "nnz": int(h.nnz), "vectorised_s": t_vec, "loop_s": t_loop,
+            "loop_sample_cells": sample_cells, "max_abs_diff_sample": err}
+
+
 def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
     """The batched disorder ensemble: ``count`` cubic 4-orbital systems with
     L ~ U{20..60}, generated on the GPU, ``moments`` moments with ``width``
@@ -726,9 +766,10 @@ def main():
     ensemble = None
     if not args.no_ensemble:
         ensemble = ensemble_bench(256, 512, 16, log, args.ens_variant, not args.ens_eager)
-    transform = None
+    transform = assembly = None
     if not args.no_density:
         transform = transform_bench(1024, 65536, log)
+        assembly = assembly_bench(1000, 60, log)
     surrogate = None
     if not args.no_surrogate:
         surrogate = surrogate_bench(args.sur_epochs, args.sur_tol, args.sur_points, 32, 32, log)
@@ -769,6 +810,7 @@ def main():
         "ensemble": ensemble,
         "site_density": site_density,
         "transform": transform,
+        "assembly": assembly,
         "surrogate": surrogate,
     }
     line.update(check)

@@ -304,3 +304,24 @@ def test_correlator_with_kwant_style_operators():
                         num_vectors=4, rng=13, bounds=bounds)
     for mu, temp in ((0.1, 0.0), (-0.3, 0.05)):
         assert abs(ours(mu, temp) - ref(mu, temp)) < 1e-10 * max(1.0, abs(ref(mu, temp)))
+
+
+def test_lattice_system_current_spectral_density():
+    """KPM straight from a vectorised-assembly system, with a Current over
+    its own hoppings, against the oracle on the same matrix."""
+    from kwant_amd import kpm
+    from kwant_amd.assembly import honeycomb_system
+    from kwant_amd.operator import Current
+    syst = honeycomb_system(8, onsite=0.0)
+    syst.set_onsite(lambda pos, w: w * np.cos(3.1 * pos[:, 0] + 1.7 * pos[:, 1]))
+    params = dict(w=0.4)
+    h = syst.hamiltonian_submatrix(params=params)
+    hops = list(syst.graph)[::5]
+    bounds = _bounds(h)
+    sd = kpm.SpectralDensity(syst, params=params, operator=Current(syst, where=hops),
+                             num_vectors=6, num_moments=30, bounds=bounds, rng=3)
+    ref = ko.SpectralDensity(
+        h, operator=oracle_current(syst.site_ranges, 1, hops,
+                                   [syst.hamiltonian(a, b, params=params) for a, b in hops]),
+        num_vectors=6, num_moments=30, bounds=bounds, rng=3)
+    assert _rel(sd._moments(), ref._moments()) < 1e-11
