@@ -187,14 +187,14 @@ def small_square_bench(log_fn):
             "densities_max_rel_diff_explicit_bounds": err}
 
 
-def surrogate_bench(epochs, tol, points, judge_points, judge_sets, log_fn):
+def surrogate_bench(epochs, tol, points, check_points, check_sets, log_fn):
     """The moment surrogate on the square-lattice family (512 x 512 sites,
     W in [0, 4], flux in [0, 0.02], mu in [-1, 1], 512 moments, 32 vectors):
     train on 4096 + 1024 GPU-KPM runs (the extra ones in the clean limit
     W < 0.5, whose sharp van Hove and Landau-level features the network
     needs most data for), sweep ``points`` random parameter points with the
-    spread gate and KPM fallback, and judge the result against converged
-    KPM (``judge_sets`` x 32 vectors) on ``judge_points`` of them."""
+    spread gate and KPM fallback, and check the result against converged
+    KPM (``check_sets`` x 32 vectors) on ``check_points`` of them."""
     import numpy as np
     import torch
     from scipy.stats import qmc
@@ -241,17 +241,17 @@ def surrogate_bench(epochs, tol, points, judge_points, judge_sets, log_fn):
     t_point = (time.perf_counter() - t0) / len(sample)
     t_plain = t_point * points
     log_fn("plain KPM: %.2f ms per point -> %.1f s for the sweep" % (1e3 * t_point, t_plain))
-    judge = np.zeros((judge_points, 512))
-    for k in range(judge_sets):
+    converged = np.zeros((check_points, 512))
+    for k in range(check_sets):
         rng = np.random.RandomState(1000 + k)
         rng.random_sample(wide.n)
         vecs = np.stack([np.exp(2j * np.pi * rng.random_sample(wide.n)) for _ in range(32)], 1)
         wide._vectors = torch.from_numpy(vecs).to(wide._vectors.device).to(wide.cdt)
-        judge += wide.damped_moments(sweep[:judge_points]) / judge_sets
-    err = dos_l1(moments[:judge_points], judge)
-    err_kpm = dos_l1(kpm[:judge_points], judge)
+        converged += wide.damped_moments(sweep[:check_points]) / check_sets
+    err = dos_l1(moments[:check_points], converged)
+    err_kpm = dos_l1(kpm[:check_points], converged)
     log_fn("DOS L1 vs converged KPM (%d vectors): sweep median %.2e p90 %.2e max %.2e; plain "
-           "KPM(32) median %.2e" % (32 * judge_sets, np.median(err), np.percentile(err, 90),
+           "KPM(32) median %.2e" % (32 * check_sets, np.median(err), np.percentile(err, 90),
                                    err.max(), np.median(err_kpm)))
     out.update({
         "workload": "square 512x512, W in [0,4], flux in [0,0.02], mu in [-1,1], 512 moments, "
@@ -265,8 +265,8 @@ def surrogate_bench(epochs, tol, points, judge_points, judge_sets, log_fn):
         "dos_l1_vs_converged_p90": float(np.percentile(err, 90)),
         "dos_l1_vs_converged_max": float(err.max()),
         "plain_kpm32_dos_l1_vs_converged_median": float(np.median(err_kpm)),
-        "judge": "KPM with %d random vectors on %d sweep points" % (32 * judge_sets,
-                                                                   judge_points),
+        "converged_reference": "KPM with %d random vectors on %d sweep points" % (32 * check_sets,
+                                                                   check_points),
         "training_data_s": t_data, "training_s": t_fit, "epochs": epochs,
         "members_kept": int(sur.active),
         "projected_other_tolerances": {
