@@ -1,5 +1,5 @@
 """Moment surrogate for parameter sweeps: a small MLP ensemble in place of the
-KPM recursion, with plain KPM as the fallback and as the judge.
+KPM recursion, with plain KPM as the fallback and as the accuracy reference.
 
 The family is the square-lattice Hamiltonian H(W, phi, mu) of
 :class:`kwant_amd.lattices.SquarePattern` (L x L sites, hopping 1 with a

@@ -247,7 +247,7 @@ def test_shift_connects_tight_and_wide_windows():
 
 def test_surrogate_gain_over_plain_kpm():
     """What the trained network gains over plain KPM on a 24 x 24 lattice:
-    judged against the exact moments (dense eigendecomposition), the trained
+    measured against the exact moments (dense eigendecomposition), the trained
     ensemble's DOS error is far below the untrained one's, and a gated sweep
     over 1024 points is faster than plain KPM with the same vector count."""
     import time
