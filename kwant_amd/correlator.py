@@ -303,18 +303,40 @@ class Correlator:
         self._integral_factor = integral_factor_gemm(mu_k, x)
 
 
+# moment matrices at least this large (M) form the integral factor's two
+# products on the GPU matrix cores; smaller ones on the host
+GPU_FACTOR_MIN = 256
+
+
+def _matmul_gpu(a, b):
+    """a @ b for host complex arrays, as one FP64 matrix-core GEMM
+    (kpm_corr_gemm computes mu += conj(A) B^T)."""
+    import torch
+    dev = eng.torch_device()
+    ta = torch.from_numpy(np.ascontiguousarray(np.conj(a), dtype=np.complex128)).to(dev)
+    tb = torch.from_numpy(np.ascontiguousarray(np.transpose(b), dtype=np.complex128)).to(dev)
+    out = torch.zeros((a.shape[0], b.shape[1]), dtype=torch.complex128, device=dev)
+    eng.corr_gemm(ta, tb, out, eng.GEMM_FP64)
+    return out.cpu().numpy()
+
+
 def integral_factor_gemm(mu_kernel, x):
     """factor_k = sum_i e+_ki (mu T^T)_ik + sum_j conj(e+_kj) (T mu)_kj with
     T_kn = cos(n arccos x_k) and e+_kn = (x_k - i n sqrt(1-x_k^2)) e^{i n arccos x_k};
-    equal to the per-energy loop of kwant/kpm.py:745-760."""
+    equal to the per-energy loop of kwant/kpm.py:745-760.  The two products
+    run on the GPU for M >= GPU_FACTOR_MIN."""
     m_count = mu_kernel.shape[-1]
     m = np.arange(m_count)
     theta = np.arccos(x)[:, None] * m[None, :]
     t = np.cos(theta)
     e_plus = (x[:, None] - 1j * np.sqrt(1 - x ** 2)[:, None] * m[None, :]) * np.exp(1j * theta)
     if mu_kernel.ndim == 2:
-        y = mu_kernel @ t.T                      # (M, 2M)
-        z = t @ mu_kernel                        # (2M, M)
+        if m_count >= GPU_FACTOR_MIN:
+            y = _matmul_gpu(mu_kernel, t.T)      # (M, 2M)
+            z = _matmul_gpu(t, mu_kernel)        # (2M, M)
+        else:
+            y = mu_kernel @ t.T                  # (M, 2M)
+            z = t @ mu_kernel                    # (2M, M)
         return np.sum(e_plus * y.T, axis=1) + np.sum(np.conj(e_plus) * z, axis=1)
     y = mu_kernel @ t.T                          # (R, M, 2M)
     z = t[None] @ mu_kernel                      # (R, 2M, M)

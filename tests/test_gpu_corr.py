@@ -194,3 +194,18 @@ def test_conductivity_reordered_equals_caller_order():
     assert b._spectrum_R.dmat.order is not None
     for mu in (-1.0, 0.0, 1.5):
         assert abs(a(mu, 0.05) - b(mu, 0.05)) < 1e-10 * (1 + abs(a(mu, 0.05)))
+
+
+def test_integral_factor_on_gpu_matches_host(monkeypatch):
+    """The integral factor's two products on the FP64 matrix cores equal
+    the host products (kwant/kpm.py:745-760 in GEMM form) to 1e-12."""
+    from kwant_amd import correlator
+    rng = np.random.default_rng(4)
+    m = 384
+    mu = rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))
+    x = np.cos(np.linspace(0.01, np.pi - 0.01, 2 * m))
+    monkeypatch.setattr(correlator, 'GPU_FACTOR_MIN', 1 << 30)
+    host = correlator.integral_factor_gemm(mu, x)
+    monkeypatch.setattr(correlator, 'GPU_FACTOR_MIN', 1)
+    gpu = correlator.integral_factor_gemm(mu, x)
+    assert np.max(np.abs(gpu - host)) < 1e-12 * np.max(np.abs(host))
