@@ -104,9 +104,9 @@ class LatticeSystem:
         pos = tags @ self.prim_vecs + self.basis[sub]
         keep = np.ones(len(pos), bool) if shape is None else \
             np.asarray(shape(pos), dtype=bool)
-        self.tags, self.sublattice, self.pos = tags[keep], sub[keep], pos[keep]
+        self.tags, self.sublattice, self._pos = tags[keep], sub[keep], pos[keep]
         self.norbs = int(norbs)
-        n = len(self.pos)
+        n = len(self._pos)
         self.site_ranges = [(0, self.norbs, 0), (n, 0, n * self.norbs)]
         # site index of every (tag, sublattice) of the box, -1 where none
         self._lo = np.array([lo for lo, _ in tag_range], dtype=np.int64)
@@ -149,7 +149,7 @@ class LatticeSystem:
                 fr = np.concatenate([s for _, s, _ in self._hops])
             else:
                 to = fr = np.zeros(0, dtype=np.int64)
-            graph = _Graph(len(self.pos), np.concatenate([fr, to]), np.concatenate([to, fr]))
+            graph = _Graph(len(self._pos), np.concatenate([fr, to]), np.concatenate([to, fr]))
             if np.any((graph.tails[1:] == graph.tails[:-1]) &
                       (graph.heads[1:] == graph.heads[:-1])):
                 raise ValueError("a hopping is defined twice")
@@ -159,7 +159,7 @@ class LatticeSystem:
     def hamiltonian(self, i, j, *args, params=None):
         """The block <i|H|j> (one site or one hopping, evaluated alone)."""
         if i == j:
-            return _blocks(_call(self.onsite, (self.pos[[i]],), params), 1, self.norbs)[0]
+            return _blocks(_call(self.onsite, (self._pos[[i]],), params), 1, self.norbs)[0]
         if self._lookup is None:
             self.graph  # noqa: B018  (checks for hoppings defined twice)
             self._lookup = {}
@@ -169,32 +169,41 @@ class LatticeSystem:
         kind = self._lookup.get((i, j))
         if kind is not None:
             value = self._hops[kind][2]
-            return _blocks(_call(value, (self.pos[[i]], self.pos[[j]]), params),
+            return _blocks(_call(value, (self._pos[[i]], self._pos[[j]]), params),
                            1, self.norbs)[0]
         kind = self._lookup.get((j, i))
         if kind is not None:
             value = self._hops[kind][2]
-            return _blocks(_call(value, (self.pos[[j]], self.pos[[i]]), params),
+            return _blocks(_call(value, (self._pos[[j]], self._pos[[i]]), params),
                            1, self.norbs)[0].conj().T
         raise ValueError("(%d, %d) is not a hopping of the system" % (i, j))
 
-    def hamiltonian_submatrix(self, args=(), sparse=True, params=None):
-        """The Hamiltonian as a CSR matrix: every value function is called
-        once, over all sites or all hoppings of a kind."""
+    def pos(self, i):
+        """Real-space position of site i (kwant's System.pos)."""
+        return self._pos[i]
+
+    @property
+    def pos_array(self):
+        return self._pos
+
+    def hamiltonian_submatrix(self, args=(), sparse=True, return_norb=False, params=None):
+        """The Hamiltonian as a CSR matrix (dense with sparse=False): every
+        value function is called once, over all sites or all hoppings of a
+        kind.  return_norb adds the orbitals per site, twice, as kwant does."""
         if args:
             raise TypeError("pass parameters by name (params=...)")
-        n, no = len(self.pos), self.norbs
+        n, no = len(self._pos), self.norbs
         self.graph  # noqa: B018  (checks for hoppings defined twice)
         r0 = np.arange(no)
         rows, cols, vals = [], [], []
-        onsite = _blocks(_call(self.onsite, (self.pos,), params), n, no)
+        onsite = _blocks(_call(self.onsite, (self._pos,), params), n, no)
         ii, jj = np.meshgrid(r0, r0, indexing='ij')
         base = (np.arange(n) * no)[:, None, None]
         rows.append((base + ii).ravel())
         cols.append((base + jj).ravel())
         vals.append(onsite.ravel())
         for to, fr, value in self._hops:
-            h = _blocks(_call(value, (self.pos[to], self.pos[fr]), params), len(to), no)
+            h = _blocks(_call(value, (self._pos[to], self._pos[fr]), params), len(to), no)
             bt = (to * no)[:, None, None]
             bf = (fr * no)[:, None, None]
             rows += [(bt + ii).ravel(), (bf + jj).ravel()]
@@ -205,9 +214,10 @@ class LatticeSystem:
         coo = scipy.sparse.coo_matrix(
             (data[keep], (np.concatenate(rows)[keep], np.concatenate(cols)[keep])),
             shape=(n * no, n * no))
-        out = coo.tocsr()
-        if not sparse:
-            return out.toarray()
+        out = coo.tocsr() if sparse else coo.toarray()
+        if return_norb:
+            norbs = np.full(n, no, dtype=np.int64)
+            return out, norbs, norbs
         return out
 
 

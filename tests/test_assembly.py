@@ -91,7 +91,7 @@ def test_two_orbitals_shape_and_zero_entries():
     syst.add_hoppings((0, 1), 0, 0, np.diag([0.5, 0.0]))      # a zero entry in every block
     params = dict(m=0.2, t=1.0)
     _same(syst.hamiltonian_submatrix(params=params), ao.csr(syst, params))
-    assert syst.site_ranges[-1][2] == 2 * len(syst.pos)
+    assert syst.site_ranges[-1][2] == 2 * len(syst.pos_array)
 
 
 def test_errors():
@@ -110,7 +110,7 @@ def test_operators_accept_lattice_systems():
     from oracle.operator_oracle import current, density
     syst = _disordered_honeycomb(6)
     params = dict(w=0.7, seed=3.0, t=1.1, phi=0.05)
-    n = len(syst.pos)
+    n = len(syst.pos_array)
     rng = np.random.default_rng(2)
     bra, ket = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
     rho = Density(syst, where=[0, 5, 9])
@@ -120,3 +120,17 @@ def test_operators_accept_lattice_systems():
     ref = current(syst.site_ranges, 1, hops,
                   [syst.hamiltonian(a, b, params=params) for a, b in hops])
     assert np.allclose(j(bra, ket), ref(bra, ket), rtol=1e-13, atol=1e-13)
+
+
+def test_velocity_from_lattice_system():
+    """conductivity's velocity operator (kwant/kpm.py:1042-1085) reads
+    hamiltonian_submatrix(return_norb=True) and pos(i) from a system."""
+    from kwant_amd.correlator import _velocity
+    syst = _disordered_honeycomb(5)
+    params = dict(w=0.7, seed=3.0, t=1.1, phi=0.05)
+    h = syst.hamiltonian_submatrix(params=params)
+    for direction in ('x', 'y'):
+        from_syst = _velocity(syst, params, direction, None)
+        from_matrix = _velocity(h, params, direction, syst.pos_array)
+        assert abs(from_syst - from_matrix).max() == 0
+    assert np.array_equal(syst.pos(3), syst.pos_array[3])

@@ -209,3 +209,21 @@ def test_integral_factor_on_gpu_matches_host(monkeypatch):
     monkeypatch.setattr(correlator, 'GPU_FACTOR_MIN', 1)
     gpu = correlator.integral_factor_gemm(mu, x)
     assert np.max(np.abs(gpu - host)) < 1e-12 * np.max(np.abs(host))
+
+
+def test_conductivity_of_a_lattice_system():
+    """conductivity takes a vectorised-assembly system as it takes a kwant
+    system (velocities from its positions) and equals the matrix call with
+    the same positions."""
+    from kwant_amd import kpm as k
+    from kwant_amd.assembly import honeycomb_system
+    syst = honeycomb_system(6)
+    syst.set_onsite(lambda pos, w: w * np.cos(2.3 * pos[:, 0] - 0.7 * pos[:, 1]))
+    params = dict(w=0.3)
+    h = syst.hamiltonian_submatrix(params=params)
+    r = float(np.max(np.asarray(abs(h).sum(axis=1)).ravel()))
+    kw = dict(num_moments=40, num_vectors=3, rng=2, bounds=(-r, r))
+    a = k.conductivity(syst, alpha='x', beta='y', params=params, **kw)
+    b = k.conductivity(h, alpha='x', beta='y', positions=syst.pos_array, **kw)
+    for mu in (-0.5, 0.0, 0.7):
+        assert a(mu, 0.05) == b(mu, 0.05)
