@@ -7,5 +7,6 @@ the HIP kernels of ``libkpm_amd.so`` (sources in ``kwant_amd/csrc``).
 """
 from . import kpm  # noqa: F401
 from . import operator  # noqa: F401
+from . import assembly  # noqa: F401
 
-__all__ = ['kpm', 'operator']
+__all__ = ['kpm', 'operator', 'assembly']

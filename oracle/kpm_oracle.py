@@ -18,8 +18,8 @@ cannot be imported here); every function takes matrices.
 
 Pinning (what checks this oracle, so that it is not its own reference):
   * moments: dense eigendecomposition brute force (oracle/brute.py) --
-    tests/test_oracle_pins.py, and the committed known-answer fixtures in
-    tests/golden/;
+    tests/test_oracle_pins.py (the reference ships no fixture data for this
+    path, so nothing is committed under tests/golden/);
   * kernels, nodes, DCT, chebval, integrate: closed forms and direct cosine
     sums (tests/test_oracle_pins.py);
   * random vectors: numpy's MT19937 ``RandomState`` itself;
