@@ -545,9 +545,6 @@ class SiteLayout:
         return np.add.reduceat(rows, self.starts, axis=1)
 
 
-# the name the site-resolved path was introduced under
-DensityLayout = SiteLayout
-
 
 class BondLayout:
     """A Current prepared for the bond-record kernel.
