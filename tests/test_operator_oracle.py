@@ -1,5 +1,6 @@
 """The Density oracle pinned by closed forms and brute force, and the host
-side of kwant_amd.operator (Density, DensityLayout) checked against it.
+side of kwant_amd.operator (Density, Current, Source and their layouts) checked
+against it.
 
 CPU only: the GPU evaluation of the same densities is in test_gpu_density.py.
 """
