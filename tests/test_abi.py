@@ -64,3 +64,17 @@ def test_pow2_widths():
     assert pow2_widths(100, 64) == [64, 32, 4]
     assert pow2_widths(1, 64) == [1]
     assert sum(pow2_widths(1000, 16)) == 1000
+
+
+def test_no_gpu_no_fallback():
+    """Without a visible GPU the product path raises; it never computes on
+    the host."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from kwant_amd import kpm
+    h = sp.diags([np.ones(9), np.ones(9)], [1, -1], format='csr', dtype=complex)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        kpm.SpectralDensity(h, num_moments=10, num_vectors=1, bounds=(-2.1, 2.1))
