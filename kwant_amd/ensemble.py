@@ -35,6 +35,11 @@ class CubicEnsemble:
         if num_moments < 2 or num_moments % 2:
             raise ValueError("num_moments must be even and >= 2")
         dev = eng.torch_device()
+        self._config = dict(sizes=[int(x) for x in np.asarray(sizes).ravel()], seed=int(seed),
+                            sigma=float(sigma), t=float(t), num_moments=int(num_moments),
+                            num_vectors=int(num_vectors), precision=precision, eps=float(eps),
+                            vector_seed=int(vector_seed))
+        self._ran = False
         self.sizes = np.asarray(sizes, dtype=np.int64)
         if self.sizes.ndim != 1 or len(self.sizes) == 0 or self.sizes.min() < 1:
             raise ValueError("sizes must be a non-empty list of positive side lengths")
@@ -115,7 +120,39 @@ class CubicEnsemble:
             self._launch()
         torch.cuda.synchronize()
         self.plan.set_step(self.steps)
+        self._ran = True
         return time.perf_counter() - t0
+
+    def save(self, path):
+        """Write the ensemble's definition and its per-step FP64 dots to
+        ``path`` (.npz with a JSON header, nothing pickled).  The systems and
+        vectors are not stored: ``load`` regenerates them from the seeds, and
+        the generator is deterministic."""
+        from . import persist
+        if not self._ran:
+            raise ValueError("run() the ensemble before saving its moments")
+        persist.save_arrays(path, persist.ENSEMBLE_FORMAT,
+                            dict(self._config, variant=int(self.variant)),
+                            dict(dots=self.dots.cpu().numpy()))
+
+    @classmethod
+    def load(cls, path):
+        """An ensemble saved by ``save``: systems regenerated on the GPU,
+        moments restored (moments() works without running again)."""
+        import torch
+        from . import persist
+        meta, arrays = persist.load_arrays(path, persist.ENSEMBLE_FORMAT)
+        cfg = {k: meta[k] for k in ("sizes", "seed", "sigma", "t", "num_moments",
+                                     "num_vectors", "precision", "eps", "vector_seed",
+                                     "variant")}
+        ens = cls(**cfg)
+        dots = torch.from_numpy(arrays["dots"])
+        if tuple(dots.shape) != tuple(ens.dots.shape):
+            raise ValueError("saved moments do not match the ensemble definition")
+        ens.dots.copy_(dots.to(ens.dots.device))
+        ens.plan.set_step(ens.steps)
+        ens._ran = True
+        return ens
 
     def moments(self, damped=True):
         """Per-system moments mu_n / mu_0 averaged over the vectors, (S, M);

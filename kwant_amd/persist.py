@@ -24,11 +24,26 @@ import json
 import numpy as np
 
 FORMAT = "kwant_amd.kpm.SpectralDensity"
+ENSEMBLE_FORMAT = "kwant_amd.ensemble.CubicEnsemble"
 VERSION = 1
 
 
 def _meta_array(meta):
     return np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+
+
+def save_arrays(path, fmt, meta, arrays):
+    """Numeric arrays and a JSON header in one .npz (no pickles)."""
+    meta = dict(meta, format=fmt, version=VERSION)
+    with open(path, "wb") as fh:
+        np.savez(fh, meta=_meta_array(meta), **arrays)
+
+
+def load_arrays(path, fmt):
+    """(header, arrays) of a file written by save_arrays with format ``fmt``."""
+    meta = read_meta(path, fmt)
+    with np.load(path, allow_pickle=False) as z:
+        return meta, {k: z[k] for k in z.files if k != "meta"}
 
 
 def save(sd, path):
@@ -78,12 +93,12 @@ def save(sd, path):
         np.savez(fh, **arrays)
 
 
-def read_meta(path):
-    """The JSON header of a saved expansion (no GPU needed)."""
+def read_meta(path, fmt=FORMAT):
+    """The JSON header of a saved expansion or ensemble (no GPU needed)."""
     with np.load(path, allow_pickle=False) as z:
         meta = json.loads(bytes(z["meta"]).decode())
-    if meta.get("format") != FORMAT:
-        raise ValueError("%s is not a saved kwant_amd SpectralDensity" % path)
+    if meta.get("format") != fmt:
+        raise ValueError("%s is not a saved %s" % (path, fmt))
     if meta.get("version") != VERSION:
         raise ValueError("unsupported format version %r" % meta.get("version"))
     return meta
