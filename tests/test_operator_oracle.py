@@ -269,3 +269,65 @@ def test_bond_layout_decomposition_matches_oracle():
         assert np.allclose(got, ref(bra, ket), rtol=1e-12, atol=1e-12)
         total = operator_layout(Current(syst, onsite, where=hops, sum=True).bind())
         assert abs(np.vdot(bra, total.act_matrix @ ket) - np.sum(ref(bra, ket))) < 1e-11
+
+
+# -- kwant's own operator objects, recognised through their public attributes --
+
+class _BlockSparse:
+    """The public fields of kwant.operator.BlockSparseMatrix: blocks stored
+    flattened in ``data`` from ``data_offsets``."""
+
+    def __init__(self, blocks):
+        flat = [np.asarray(b, dtype=complex).ravel() for b in blocks]
+        self.data = np.concatenate(flat) if flat else np.zeros(0, complex)
+        self.data_offsets = np.concatenate([[0], np.cumsum([len(f) for f in flat])])[:-1]
+
+
+def _kwant_like(kind, ours, bound_onsite=None, bound_ham=None):
+    """An object with the class name and public attributes of kwant's own
+    operator (operator.pxd: where, _site_ranges, onsite, sum,
+    check_hermiticity, _bound_onsite, _bound_hamiltonian)."""
+    obj = type(kind, (), {})()
+    obj.where = np.asarray(ours.where, dtype=np.int32)
+    obj._site_ranges = np.asarray(ours._site_ranges, dtype=np.int32)
+    obj.onsite = ours.onsite
+    obj.sum = ours.sum
+    obj.check_hermiticity = ours.check_hermiticity
+    obj._bound_onsite = bound_onsite
+    obj._bound_hamiltonian = bound_ham
+    return obj
+
+
+def _same_layout(a, b):
+    assert type(a) is type(b)
+    assert abs(a.act_matrix - b.act_matrix).max() == 0
+    assert abs(a.matrix - b.matrix).max() == 0
+    for name in ('orbs', 'starts', 'rows', 'cols', 'values', 'where_index'):
+        if hasattr(a, name):
+            assert np.array_equal(getattr(a, name), getattr(b, name)), name
+
+
+def test_kwant_objects_give_the_same_layouts():
+    syst = FakeSystem(RUNS, seed=40, density=0.05)
+    sites = [3, 40, 40, 55]
+    mats = {s: _herm(syst.hamiltonian(s, s).shape[0], s + 1) for s in set(sites)}
+    # Density with a constant onsite, and with a callable one (bound blocks)
+    ours = Density(syst, 0.5, where=sites)
+    _same_layout(operator_layout(_kwant_like('Density', ours)), operator_layout(ours))
+    ours = Density(syst, lambda s: mats[s], where=sites).bind()
+    kw = _kwant_like('Density', ours, bound_onsite=_BlockSparse(ours._bound_onsite))
+    kw.onsite = lambda s: mats[s]
+    _same_layout(operator_layout(kw), operator_layout(ours))
+    # Source: the onsite Hamiltonian blocks come from _bound_hamiltonian
+    ours = Source(syst, lambda s: mats[s], where=sites).bind()
+    kw = _kwant_like('Source', ours, bound_onsite=_BlockSparse(ours._bound_onsite),
+                     bound_ham=_BlockSparse(ours._bound_hamiltonian))
+    kw.onsite = lambda s: mats[s]
+    _same_layout(operator_layout(kw), operator_layout(ours))
+    # Current over some hoppings, with a constant onsite
+    hops = list(syst.graph)[::11]
+    ours = Current(syst, 1, where=hops).bind()
+    kw = _kwant_like('Current', ours, bound_ham=_BlockSparse(ours._bound_hamiltonian))
+    _same_layout(operator_layout(kw), operator_layout(ours))
+    # an unbound kwant Current (no Hamiltonian blocks yet) is not taken
+    assert operator_layout(_kwant_like('Current', ours)) is None
