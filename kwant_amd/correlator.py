@@ -104,6 +104,13 @@ class _Expansion:
         return self._a * _chebyshev_nodes(SAMPLING * self.num_moments) + self._b
 
 
+class _SavedNodes:
+    """The energies of a loaded Correlator (its expansions are not kept)."""
+
+    def __init__(self, energies):
+        self.energies = energies
+
+
 class Correlator:
     """Response of the correlation between two operators (kwant/kpm.py:493-555).
     ``operator1`` acts to the right as O_a|r>."""
@@ -160,9 +167,45 @@ class Correlator:
     def energies(self):
         return self._spectrum_R.energies
 
+    def save(self, path):
+        """Write what evaluating sigma needs -- (a, b), the moment count and
+        the integral factor at the 2M nodes -- and the mean moment matrix
+        mu_mn to ``path`` (.npz with a JSON header, nothing pickled)."""
+        from . import persist
+        meta = dict(a=float(self._a), b=float(self._b), num_moments=int(self.num_moments),
+                    num_vectors=int(self.num_vectors), mean=bool(self.mean))
+        arrays = dict(integral_factor=np.asarray(self._integral_factor),
+                      energies=np.asarray(self.energies))
+        if getattr(self, "moments_matrix", None) is not None:
+            arrays["moments_matrix"] = np.asarray(self.moments_matrix)
+        persist.save_arrays(path, persist.CORRELATOR_FORMAT, meta, arrays)
+
+    @classmethod
+    def load(cls, path):
+        """A Correlator saved by ``save``, for evaluating sigma(mu, T) with
+        ``__call__``; adding moments or vectors needs the Hamiltonian, so a
+        loaded Correlator refuses them."""
+        from . import persist
+        meta, arrays = persist.load_arrays(path, persist.CORRELATOR_FORMAT)
+        obj = cls.__new__(cls)
+        obj._a, obj._b = meta["a"], meta["b"]
+        obj.num_moments, obj.num_vectors = meta["num_moments"], meta["num_vectors"]
+        obj.mean = meta["mean"]
+        obj._integral_factor = arrays["integral_factor"]
+        obj._spectrum_R = _SavedNodes(arrays["energies"])
+        obj.moments_matrix = arrays.get("moments_matrix")
+        obj._loaded = True
+        return obj
+
+    def _check_live(self):
+        if getattr(self, "_loaded", False):
+            raise ValueError("a loaded Correlator evaluates sigma only; build it again "
+                             "from the Hamiltonian to add moments or vectors")
+
     def add_moments(self, num_moments=None, *, energy_resolution=None):
         """Extend both expansions (kwant/kpm.py:636-661); the moment matrix is
         recomputed from the saved vectors."""
+        self._check_live()
         spec = self._spectrum_R
         if not ((num_moments is None) ^ (energy_resolution is None)):
             raise TypeError("either 'num_moments' or 'energy_resolution' must be provided.")
@@ -189,6 +232,7 @@ class Correlator:
 
     def add_vectors(self, num_vectors=None):
         """Expand more vectors (kwant/kpm.py:663-683)."""
+        self._check_live()
         self._spectrum_R._vector_factory.add_vectors(num_vectors)
         self._add_vectors(num_vectors)
         self._build_integral_factor()

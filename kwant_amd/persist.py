@@ -25,6 +25,7 @@ import numpy as np
 
 FORMAT = "kwant_amd.kpm.SpectralDensity"
 ENSEMBLE_FORMAT = "kwant_amd.ensemble.CubicEnsemble"
+CORRELATOR_FORMAT = "kwant_amd.kpm.Correlator"
 VERSION = 1
 
 

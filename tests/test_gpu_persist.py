@@ -98,3 +98,23 @@ def test_ensemble_round_trip(tmp_path):
     fresh = CubicEnsemble([6, 9, 7], seed=4, num_moments=32, num_vectors=4)
     with pytest.raises(ValueError):
         fresh.save(tmp_path / "not_run.npz")
+
+
+def test_correlator_round_trip(tmp_path):
+    """A saved Correlator evaluates sigma(mu, T) bitwise as the original;
+    it refuses to add moments or vectors without the Hamiltonian."""
+    from kwant_amd import kpm
+    h = SYST.h
+    x = np.asarray([p[0] for p in np.random.default_rng(1).normal(size=(h.shape[0], 2))])
+    vel = 1j * (h.multiply(x[None, :]) - h.multiply(x[:, None]))
+    cor = kpm.Correlator(h, operator1=vel, operator2=vel, num_moments=40, num_vectors=2,
+                         rng=3, bounds=_bounds(h))
+    cor.save(tmp_path / "cor.npz")
+    back = kpm.Correlator.load(tmp_path / "cor.npz")
+    for mu, temp in ((0.0, 0.0), (0.3, 0.05)):
+        assert back(mu, temp) == cor(mu, temp)
+    assert np.array_equal(back.moments_matrix, cor.moments_matrix)
+    with pytest.raises(ValueError):
+        back.add_moments(10)
+    with pytest.raises(ValueError):
+        back.add_vectors(1)
