@@ -54,20 +54,28 @@ def key_of(dmat, width, mode):
     return "%d:%d:%d:%d:%d" % (dmat.shape[0], dmat.nnz, width, dmat.value_type, mode)
 
 
+# Modes timed as another: the site-resolved records cost the projection
+# vector's read like DOTS_PROJECT (their records are timed with it).
+_TIMED_AS = {eng.DOTS_LOCAL: eng.DOTS_PROJECT, eng.DOTS_LOCAL_COLS: eng.DOTS_PROJECT}
+
+
 def time_variant(dmat, ab, width, mode, variant, block, steps=TIMED_STEPS):
-    """Milliseconds per step of one launch shape on ``block`` (N x width)."""
+    """Milliseconds per step of one launch shape on ``block`` (N x width).
+    The projection modes read ``block`` itself as their vector w."""
     import torch
-    rec = eng.BlockRecursion(dmat, ab, block, width, mode, variant=variant)
+    mode = _TIMED_AS.get(mode, mode)
+    w = block if mode == eng.DOTS_PROJECT else None
+    rec = eng.BlockRecursion(dmat, ab, block, width, mode, w=w, variant=variant)
     dots = None
     if mode != eng.DOTS_NONE:
         dots = torch.empty((steps + 2, 1, width, 4), dtype=torch.float64, device=block.device)
     stride = dmat.shape[0] * width
-    rec.plan.steps(dmat, rec.ring, stride, 2, 2, mode, dots=dots, out_base=0)
+    rec.plan.steps(dmat, rec.ring, stride, 2, 2, mode, dots=dots, out_base=0, w=w)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     start.record()
-    rec.plan.steps(dmat, rec.ring, stride, 2, steps, mode, dots=dots, out_base=0)
+    rec.plan.steps(dmat, rec.ring, stride, 2, steps, mode, dots=dots, out_base=0, w=w)
     end.record()
     torch.cuda.synchronize()
     return start.elapsed_time(end) / steps

@@ -143,14 +143,18 @@ class Correlator:
         # the second expansion recomputes (a, b) from these bounds, as
         # kwant/kpm.py:578-579, 597 make it do
         self._aL, self._bL = _scale_from_bounds(self._b - shrunk, self._b + shrunk, spec.eps)
-        rng = ensure_rng(kwargs.get('rng'))
-        rng.random_sample(spec.csr.shape[0])  # the v0 draw of the second expansion
         self.num_moments = spec.num_moments
         self.num_vectors = 0
         self._dev_ops = {}
         self._mu_sum = None
         self._mu_list = []
+        # kwant draws the first expansion's v0, then its random vectors
+        # (inside its constructor), then the second expansion's v0
+        # (kwant/kpm.py:573-603): with a shared generator (None or a
+        # RandomState) the vectors must be drawn before that last v0
         self._add_vectors(spec._vector_factory.num_vectors)
+        rng = ensure_rng(kwargs.get('rng'))
+        rng.random_sample(spec.csr.shape[0])  # the v0 draw of the second expansion
         self._build_integral_factor()
 
     # -- public API (kwant/kpm.py:608-683) ---------------------------------

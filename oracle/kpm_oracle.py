@@ -24,8 +24,13 @@ Pinning (what checks this oracle, so that it is not its own reference):
     sums (tests/test_oracle_pins.py);
   * random vectors: numpy's MT19937 ``RandomState`` itself;
   * Correlator moments: brute force mu_mn = Tr-estimator via eigh on tiny H;
-    integral factor: the GEMM form against the direct loop here, and physics
-    invariants (sigma_xy = -sigma_yx; sigma_xx of a clean chain);
+    the whole Kubo-Bastin evaluation (integral factor, weight, simpson,
+    prefactor): with the exact trace, N * sigma_xy(mu, T) of a QWZ Chern
+    insulator torus converges to the eigenstate Kubo formula
+    (oracle/brute.py:kubo_hall_exact, whose normalisation is pinned by the
+    Chern number), at T = 0 and T > 0, in the gap and in the bands; planted
+    mistakes (conjugated e+, unhalved g_0, halved prefactor) fail it; plus
+    sigma_xy = -sigma_yx;
   * ``_rescale`` with bounds=None: the invariant spec((H-b)/a) in (-1, 1).
 """
 import math

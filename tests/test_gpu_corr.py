@@ -103,6 +103,32 @@ def test_conductivity_formats_and_add_moments():
     np.testing.assert_allclose(c102(), c(), rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("seeding", ["state", "global"])
+def test_correlator_shared_generator_matches_oracle(seeding):
+    """With one generator shared by both expansions (a RandomState, or None
+    after numpy.random.seed) the random vectors are the ones kwant draws:
+    first expansion's v0, its vectors, then the second expansion's v0
+    (kwant/kpm.py:573-603)."""
+    h = brute.gaussian_unitary(14, 8)
+    r = float(np.max(np.abs(np.linalg.eigvalsh(h)))) * 1.05
+    op = brute.gaussian_unitary(14, 9)
+    kw = dict(num_moments=16, num_vectors=3, bounds=(-r, r))
+
+    def make(module):
+        if seeding == "state":
+            return module.Correlator(h, operator1=op, rng=np.random.RandomState(0), **kw)
+        np.random.seed(5)
+        return module.Correlator(h, operator1=op, rng=None, **kw)
+
+    gpu = make(kpm())
+    after_gpu = np.random.random_sample() if seeding == "global" else None
+    ora = make(ko)
+    after_ora = np.random.random_sample() if seeding == "global" else None
+    assert rel(gpu.moments_matrix, ora.moments_matrix,
+               np.abs(ora.moments_matrix).max()) < 1e-10
+    assert after_gpu == after_ora       # the same number of draws
+
+
 def test_conductivity_exact_trace_antisymmetric():
     from kwant_amd.lattices import square
     h, pos = square(6, flux=0.1)
@@ -227,3 +253,76 @@ def test_conductivity_of_a_lattice_system():
     b = k.conductivity(h, alpha='x', beta='y', positions=syst.pos_array, **kw)
     for mu in (-0.5, 0.0, 0.7):
         assert a(mu, 0.05) == b(mu, 0.05)
+
+
+def test_conductivity_qwz_matches_exact_kubo():
+    """The HIP Kubo-Bastin chain (complex128 expansions, FP64 GEMM) on the
+    QWZ Chern-insulator torus with the exact trace: equal to the oracle and,
+    like it, converged to the eigenstate Kubo formula (in the gap at T = 0 and
+    T > 0, and inside the bands)."""
+    from kubo_cases import kubo_deviation, qwz_conductivity, sigma_trace
+    gpu, n, exact = qwz_conductivity(kpm(), 256)
+    ora, _, _ = qwz_conductivity(ko, 256)
+    got, want = sigma_trace(gpu, n), sigma_trace(ora, n)
+    assert np.max(np.abs(got - want)) < 1e-9 * np.max(np.abs(want))
+    assert kubo_deviation(gpu, n, exact) < 4e-3
+
+
+def _dirac_small():
+    """The four-band spin-orbit model of the sigma_xy benchmark at 8 x 8
+    (A = 1, B = -1, M = -2, Anderson W = 1, Peierls flux 1/64)."""
+    from kwant_amd.lattices import dirac_4band
+    h, pos = dirac_4band(8, flux=1 / 64, disorder=1.0, seed=6)
+    r = float(np.abs(h).sum(axis=1).max())           # Gershgorin radius
+    return h, pos, dict(num_moments=64, num_vectors=8, bounds=(-r, r), rng=7)
+
+
+def test_conductivity_dirac_4band_matches_oracle():
+    """sigma_xy of the four-band Dirac model (complex 4 x 4 hoppings, Peierls
+    phase, disorder) through the HIP path against the oracle: complex128
+    expansions with the FP64 GEMM to 1e-10; the sliced BF16 GEMM (4 slices)
+    and the complex64 / FP32 path within their accuracy."""
+    h, pos, kw = _dirac_small()
+    ora = ko.conductivity(h, 'x', 'y', positions=pos, **kw)
+    mus = (-1.5, -0.5, 0.0, 0.4, 1.2)
+    want = np.array([ora(mu, 0.05) for mu in mus])
+    scale = np.max(np.abs(want))
+    fp64 = kpm().conductivity(h, 'x', 'y', positions=pos, **kw)
+    assert rel(fp64.moments_matrix, ora.moments_matrix,
+               np.abs(ora.moments_matrix).max()) < 1e-10
+    assert rel([fp64(mu, 0.05) for mu in mus], want, scale) < 1e-10
+    sl4 = kpm().conductivity(h, 'x', 'y', positions=pos, gemm_precision='sliced4', **kw)
+    assert rel([sl4(mu, 0.05) for mu in mus], want, scale) < 1e-6
+    f32 = kpm().conductivity(h, 'x', 'y', positions=pos, precision='complex64', **kw)
+    assert rel([f32(mu, 0.05) for mu in mus], want, scale) < 2e-3
+
+
+def test_gamma_gemm_full_shape_sliced_vs_fp64():
+    """The Gamma_nm product at the benchmark's shape (1024 x 1024 moments,
+    K = 2^21 = N x 2 interleaved (site, vector) pairs): the sliced BF16 GEMM
+    (4 slices) against the FP64 matrix cores, and a few entries of both
+    against numpy on the host."""
+    import torch
+    from kwant_amd import _engine as eng
+    dev = eng.torch_device()
+    m, k = 1024, 1 << 21
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(11)
+    a = torch.randn((m, k), dtype=torch.complex128, device=dev, generator=gen)
+    b = torch.randn((m, k), dtype=torch.complex128, device=dev, generator=gen)
+    mu64 = torch.zeros((m, m), dtype=torch.complex128, device=dev)
+    eng.corr_gemm(a, b, mu64, eng.GEMM_FP64)
+    mus = torch.zeros_like(mu64)
+    eng.corr_gemm_sliced(a, b, mus, slices=4)
+    # entries are sums of k products of unit-variance complex numbers
+    natural = np.sqrt(2.0 * k)
+    err = float((mus - mu64).abs().max()) / natural
+    assert err < 1e-7, err
+    rows = [0, 1, 517, 1023]
+    ha = a[rows].cpu().numpy()
+    hb = b[rows].cpu().numpy()
+    ref = ha.conj() @ hb.T
+    got = mu64[rows][:, rows].cpu().numpy()
+    assert np.max(np.abs(got - ref)) / natural < 1e-12
+    del a, b
+    torch.cuda.empty_cache()

@@ -235,6 +235,36 @@ def test_operator_matrix_matches_oracle():
     assert rel(np.array(gpu._moments_list), np.array(ora._moments_list), scale) < 1e-11
 
 
+def test_autotune_times_every_dots_mode():
+    """The autotuner times the projection modes with a projection vector
+    (they read one) and the recursion-only mode without dots."""
+    from kwant_amd import _engine as eng
+    from kwant_amd import autotune
+    from kwant_amd.lattices import square
+    h, _ = square(24, disorder=0.5, seed=3)
+    dmat = eng.DeviceCSR(h, 'c64')
+    n_var = eng.variant_count(8, dmat.value_type)
+    for mode in (eng.DOTS_PROJECT, eng.DOTS_NONE, eng.DOTS_LOCAL, eng.DOTS_DOUBLING):
+        v = autotune.pick_variant(dmat, (4.5, 0.0), 8, mode, force=True)
+        assert 0 <= v < n_var
+
+
+def test_matrix_operator_above_the_tuning_threshold():
+    """A matrix operator with N x width above the autotuner's threshold (the
+    default variant=-1 tunes with DOTS_PROJECT) matches the oracle."""
+    import scipy.sparse as sp
+    from kwant_amd import autotune
+    from kwant_amd.lattices import square
+    h, _ = square(256, disorder=1.0, seed=5)
+    n = h.shape[0]
+    assert n * 16 >= autotune.MIN_ELEMENTS
+    op = sp.diags(np.cos(np.arange(n) * 0.37)).tocsr()
+    kw = dict(num_moments=12, num_vectors=16, bounds=(-4.6, 4.6), rng=2, operator=op)
+    gpu = kpm().SpectralDensity(h, **kw)
+    ora = ko.SpectralDensity(h, **kw)
+    assert rel(np.array(gpu._moments_list), np.array(ora._moments_list), n) < 1e-11
+
+
 def test_operator_callable_resume_matches_oracle():
     """A user callable gets alpha_0 and alpha_n on the host; after
     add_moments alpha_0 must still be the first vector (the two-slot ring no

@@ -9,6 +9,7 @@ from numpy.polynomial.chebyshev import chebval
 
 from oracle import brute
 from oracle import kpm_oracle as ko
+from kubo_cases import kubo_deviation, qwz_conductivity
 
 
 def _gue(n, seed):
@@ -228,6 +229,67 @@ def test_integral_factor_gemm_form_equals_loop():
     loop3 = ko.integral_factor(mus, x)
     gemm3 = integral_factor_gemm(mus, x)
     assert np.allclose(gemm3, loop3, atol=1e-12 * np.abs(loop3).max(), rtol=0)
+
+
+def test_exact_kubo_normalisation_is_the_chern_number():
+    """The eigenstate Kubo formula (oracle/brute.py) per unit cell of the QWZ
+    torus at mu = 0 in the gap tends to the Chern number 1 (TKNN), which fixes
+    its 4 pi and its sign independently of kwant's KPM formulas."""
+    per_cell = []
+    for L in (10, 16):
+        h, vx, vy = brute.qwz_torus(L, -1.0)
+        per_cell.append(brute.kubo_hall_exact(h, vx, vy, 0.0, 0) / L ** 2)
+    assert abs(per_cell[0] - 1) < 5e-3
+    assert abs(per_cell[1] - 1) < 1e-4
+    # sigma_yx = -sigma_xy
+    h, vx, vy = brute.qwz_torus(10, -1.0)
+    assert abs(brute.kubo_hall_exact(h, vy, vx, 0.0, 0) / 100 + per_cell[0]) < 1e-12
+
+
+def test_conductivity_converges_to_exact_kubo():
+    """The oracle's Kubo-Bastin chain (integral factor, (1 - x^2)^2 weight,
+    simpson, prefactor 2 4^2 / (2a)^2 of kwant/kpm.py:608-630, 720-760) times N
+    tends to the eigenstate Kubo formula, in the gap and in the bands, at T = 0
+    and T > 0; the deviation falls by ~4x per doubling of the moments (the
+    Jackson kernel's 1/M^2 broadening)."""
+    c128, n, exact = qwz_conductivity(ko, 128)
+    c256, _, _ = qwz_conductivity(ko, 256)
+    d128, d256 = kubo_deviation(c128, n, exact), kubo_deviation(c256, n, exact)
+    assert d256 < 4e-3, d256
+    assert d256 < d128 / 3, (d128, d256)
+
+
+def test_kubo_pin_catches_planted_mistakes():
+    """Mistakes planted in the oracle's evaluation move sigma far outside the
+    tolerance of test_conductivity_converges_to_exact_kubo: a conjugated
+    e+ (kwant/kpm.py:750-752), an unhalved g_0 (kwant/kpm.py:738) and a halved
+    prefactor (kwant/kpm.py:629)."""
+    cond, n, exact = qwz_conductivity(ko, 128)
+    assert kubo_deviation(cond, n, exact) < 2e-2
+    m_count = cond.num_moments
+    x = (cond.energies - cond._b) / cond._a
+    g = cond._spectrum_R.kernel(np.ones(m_count))
+    # (1) conjugated e+: gamma_mn = T_m conj(e+_n) + h.c.
+    mu_k = np.outer(g, g) * cond.moments_matrix
+    mu_k[0] /= 2
+    mu_k[:, 0] /= 2
+    good = cond._integral_factor
+    m = np.arange(m_count)
+    wrong = []
+    for e in x:
+        phase = np.exp(1j * np.arccos(e) * m)
+        e_plus = np.conj((e - 1j * np.sqrt(1 - e ** 2) * m) * phase)
+        gamma = e_plus[None, :] * np.real(phase)[:, None]
+        gamma = gamma + gamma.conj().T
+        wrong.append(np.tensordot(mu_k, gamma.T))
+    cond._integral_factor = np.array(wrong)
+    assert kubo_deviation(cond, n, exact) > 0.5
+    # (2) g_0 not halved
+    cond._integral_factor = ko.integral_factor(np.outer(g, g) * cond.moments_matrix, x)
+    assert kubo_deviation(cond, n, exact) > 0.3
+    # (3) halved prefactor
+    cond._integral_factor = good / 2
+    assert kubo_deviation(cond, n, exact) > 0.4
 
 
 def test_conductivity_antisymmetry():
