@@ -108,6 +108,19 @@ int kpm_plan_destroy(kpm_plan* plan);
  * Synchronous upload. */
 int kpm_plan_set_scale(kpm_plan* plan, const double* h_ab);
 
+/* Grid of the exact sums of Im <a_{n+1}|a_n> (KPM_DOTS_DOUBLING), fixed from
+ * the block's start vectors a0 (n_rows x width device block, the plan's value
+ * type): per segment and column, with 2^E > <a0|a0>, every term is cut into
+ * integers on the grid 2^(E-73) and summed exactly, so the imaginary part of
+ * mu_{2n+1} (kwant/kpm.py:470-473), rounding noise of an exactly-zero sum,
+ * comes out the same for every launch shape, block width and resumed run --
+ * and kwant's real_if_close (kwant/kpm.py:379) sees a nonzero imaginary part
+ * as it does with numpy's vdot.  Required before doubling-mode steps; call it
+ * once per block, before the first step (the grid must not change while a
+ * block is extended).  Fails with KPM_ERR_ARG when a0 holds NaN or Inf.
+ * Synchronises the stream (reads <a0|a0> back). */
+int kpm_plan_set_dots_grid(kpm_plan* plan, const void* a0, kpm_stream_t stream);
+
 /* Step counter n of the plan (host side): the next kpm_cheb_steps launch
  * computes a_{n+1}.  The stream argument is unused (kept for symmetry). */
 int kpm_plan_set_step(kpm_plan* plan, int32_t n, kpm_stream_t stream);
@@ -128,12 +141,12 @@ int kpm_plan_get_step(kpm_plan* plan, int32_t* n, kpm_stream_t stream);
  *
  * dots_mode KPM_DOTS_DOUBLING writes, per step, segment and column c,
  *   d_dots[((n - out_base) * n_segments + s) * width + c] = 4 doubles
- *   { <a_n|a_n>, 0, Re <a_{n+1}|a_n>, 0 }
+ *   { <a_n|a_n>, 0, Re <a_{n+1}|a_n>, Im <a_{n+1}|a_n> }
  * (the two vdot()s of kwant/kpm.py:444-445, 470-473; at n == 0 the second
- * is <a_1|a_0>, the conjugate of mu_1).  H must be Hermitian, as the
- * expansion assumes: then Im <a_{n+1}|a_n> is exactly 0 (it equals
- * -Im <a_{n-1}|a_n>, down to Im <a_1|a_0> = 0) and is written as 0 rather
- * than summed from rounding noise.  KPM_DOTS_PROJECT writes
+ * is <a_1|a_0>, the conjugate of mu_1).  <a_n|a_n> is real (numpy's vdot of
+ * a vector with itself has a zero imaginary part); Im <a_{n+1}|a_n> is the
+ * exact sum on the grid of kpm_plan_set_dots_grid (zero for Hermitian H in
+ * exact arithmetic, rounding noise in practice).  KPM_DOTS_PROJECT writes
  * { <w|a_n>, <w|a_{n+1}> } with w the device block `w` (the matrix-operator
  * moments vdot(a_0, O a_n) of kwant/kpm.py:193-195, 484 with w = O^H a_0).
  * Sums are FP64 and deterministic (fixed order), so repeated runs and

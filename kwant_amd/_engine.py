@@ -37,7 +37,7 @@ MLP_ACCUMULATE = 32
 EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
-    "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
+    "kpm_plan_set_dots_grid", "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
     "kpm_bond_records", "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
     "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
     "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
@@ -119,6 +119,7 @@ def load_library(path=LIB_PATH):
                                            ctypes.POINTER(vp)]),
         "kpm_plan_destroy": (ctypes.c_int, [vp]),
         "kpm_plan_set_scale": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double)]),
+        "kpm_plan_set_dots_grid": (ctypes.c_int, [vp, vp, vp]),
         "kpm_plan_set_step": (ctypes.c_int, [vp, i32, vp]),
         "kpm_plan_get_step": (ctypes.c_int, [vp, pi32, vp]),
         "kpm_cheb_steps": (ctypes.c_int, [vp, ctypes.POINTER(KpmCsr), vp, i64, i32, vp,
@@ -338,6 +339,14 @@ class Plan:
             self.handle, ab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
             "kpm_plan_set_scale")
 
+    def set_dots_grid(self, a0):
+        """Fix the grid of the exact Im <a_{n+1}|a_n> sums from the block's
+        start vectors (an (N, width) device block); raises ValueError when
+        they hold NaN or Inf."""
+        check(load_library().kpm_plan_set_dots_grid(self.handle, a0.data_ptr(),
+                                                    stream_handle()),
+              "kpm_plan_set_dots_grid")
+
     def set_step(self, n):
         check(load_library().kpm_plan_set_step(self.handle, n, stream_handle()),
               "kpm_plan_set_step")
@@ -399,6 +408,8 @@ class BlockRecursion:
             raise ValueError("a ring needs at least two slots")
         self.ring = torch.zeros((self.slots, n, width), dtype=self.cdt, device=dev)
         self.ring[0, :, :vectors.shape[1]] = vectors
+        if mode == DOTS_DOUBLING:
+            self.plan.set_dots_grid(self.ring[0])
         self.w = w
         self.n = 0  # steps done: ring holds a_0 .. a_n
         self.plan.set_step(0)

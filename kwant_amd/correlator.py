@@ -25,7 +25,8 @@ from scipy.sparse import coo_matrix, csr_matrix
 from . import _engine as eng
 from . import ordering
 from .operator import operator_layout
-from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _is_local_operator,
+from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _check_finite_vectors,
+                  _is_local_operator,
                   _is_system, _rescale, _scale_from_bounds, ensure_rng, fermi_distribution,
                   jackson_kernel, SAMPLING, _chebyshev_nodes)
 from collections.abc import Iterable
@@ -289,6 +290,7 @@ class Correlator:
             vecs = [factory[r] for r in range(start, start + width)]
             start += width
             host = np.stack([np.asarray(v) for v in vecs], axis=1).astype(np.complex128)
+            _check_finite_vectors(host)
             block = torch.from_numpy(np.ascontiguousarray(dmat.rows_in(host))).to(dev).to(cdt)
             # right expansion: all T_n(H~)|r>
             rec_r = eng.BlockRecursion(dmat, (self._a, self._b), block, width,

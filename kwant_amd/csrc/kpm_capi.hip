@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -43,7 +44,10 @@ struct kpm_plan {
   double* scale = nullptr;
   double* part = nullptr;   // two buffers of part_half doubles (step parity)
   int64_t part_half = 0;
+  int32_t* im_exp = nullptr;    // nseg x width grid exponents of the exact Im sums
+  double* grid_dots = nullptr;  // nseg x width x 4: <a_0|a_0> for the grid
   bool scale_set = false;
+  bool grid_set = false;
 };
 
 namespace {
@@ -72,6 +76,8 @@ void free_plan(kpm_plan* p) {
   (void)hipFree(p->seg_blk);
   (void)hipFree(p->scale);
   (void)hipFree(p->part);
+  (void)hipFree(p->im_exp);
+  (void)hipFree(p->grid_dots);
   delete p;
 }
 
@@ -83,6 +89,7 @@ int fill_params(const kpm_plan* plan, StepParams& p) {
   p.seg_blk = plan->seg_blk;
   p.scale = plan->scale;
   p.part = plan->part;
+  p.im_exp = plan->grid_set ? plan->im_exp : nullptr;
   p.prev_part = nullptr;
   p.prev_dots = nullptr;
   p.nblocks = plan->nblocks;
@@ -100,7 +107,7 @@ extern "C" {
 
 const char* kpm_last_error(void) { return g_last_error.c_str(); }
 
-int kpm_abi_version(void) { return 104; }
+int kpm_abi_version(void) { return 105; }
 
 int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch) {
   hipDeviceProp_t prop;
@@ -232,7 +239,7 @@ int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n
     p->shape.iters = int(max_chunks * p->shape.chunk);
   }
 
-  const int maxnc = 4;
+  const int maxnc = 4;  // doubling: 2 TwoSum accumulators + 2 grid sums
   int rc = KPM_OK;
   if (!rc) rc = dev_alloc(&p->blk_rows, blk_rows.size(), "blk_rows");
   if (!rc) rc = dev_alloc(&p->blk_stride, blk_stride.size(), "blk_stride");
@@ -241,6 +248,8 @@ int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n
   if (!rc) rc = dev_alloc(&p->scale, size_t(2) * n_segments, "scale");
   p->part_half = int64_t(p->nblocks) * maxnc * width * 2;
   if (!rc) rc = dev_alloc(&p->part, size_t(2 * p->part_half), "partials");
+  if (!rc) rc = dev_alloc(&p->im_exp, size_t(n_segments) * width, "grid exponents");
+  if (!rc) rc = dev_alloc(&p->grid_dots, size_t(n_segments) * width * 4, "grid dots");
   if (!rc) rc = upload(p->blk_rows, blk_rows, "upload blk_rows");
   if (!rc) rc = upload(p->blk_stride, blk_stride, "upload blk_stride");
   if (!rc) rc = upload(p->blk_seg, blk_seg, "upload blk_seg");
@@ -271,6 +280,46 @@ int kpm_plan_set_scale(kpm_plan* plan, const double* h_ab) {
                                hipMemcpyHostToDevice),
                      "upload scale");
   if (!rc) plan->scale_set = true;
+  return rc;
+}
+
+int kpm_plan_set_dots_grid(kpm_plan* plan, const void* a0, kpm_stream_t stream) {
+  if (!plan || !a0) return fail(KPM_ERR_ARG, "NULL argument");
+  if (!plan->scale_set) return fail(KPM_ERR_ARG, "scale not set (kpm_plan_set_scale)");
+  hipStream_t s = (hipStream_t)stream;
+  // <a_0|a_0> per segment and column: a sum of non-negative terms, so the
+  // TwoSum result is its correctly rounded value whatever the launch shape
+  StepParams p{};
+  p.ring = reinterpret_cast<char*>(const_cast<void*>(a0));
+  p.slot_bytes = 0;
+  p.slots = 1;
+  p.dots = plan->grid_dots;
+  fill_params(plan, p);
+  p.im_exp = nullptr;
+  int rc = launch_cheb(plan->value_type, plan->shape, KPM_DOTS_DOUBLING, true, 0, 0, p, s);
+  if (!rc) rc = launch_reduce(KPM_DOTS_DOUBLING, plan->width, p, s);
+  const int64_t cols = int64_t(plan->nseg) * plan->width;
+  std::vector<double> dots(static_cast<size_t>(cols) * 4);
+  std::vector<int32_t> kexp(static_cast<size_t>(cols));
+  if (!rc)
+    rc = check_hip(hipMemcpyAsync(dots.data(), plan->grid_dots, sizeof(double) * dots.size(),
+                                  hipMemcpyDeviceToHost, s),
+                   "copy grid dots");
+  if (!rc) rc = check_hip(hipStreamSynchronize(s), "grid dots");
+  if (rc) return rc;
+  for (int64_t c = 0; c < cols; ++c) {
+    const double mu0 = dots[size_t(c) * 4];
+    if (!std::isfinite(mu0))
+      return fail(KPM_ERR_ARG, "start vectors contain NaN or Inf (column " +
+                                   std::to_string(c % plan->width) + ")");
+    int e = 0;
+    if (mu0 > 0.0) (void)frexp(mu0, &e);  // mu0 < 2^e
+    kexp[size_t(c)] = 51 - e;
+  }
+  rc = check_hip(hipMemcpy(plan->im_exp, kexp.data(), sizeof(int32_t) * kexp.size(),
+                           hipMemcpyHostToDevice),
+                 "upload grid exponents");
+  if (!rc) plan->grid_set = true;
   return rc;
 }
 
@@ -317,6 +366,8 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
     return fail(KPM_ERR_ARG, "unknown dots_mode");
   if (dots_mode != KPM_DOTS_NONE && (!d_dots || out_cap <= 0))
     return fail(KPM_ERR_ARG, "dots output buffer missing");
+  if (dots_mode == KPM_DOTS_DOUBLING && !plan->grid_set)
+    return fail(KPM_ERR_ARG, "dots grid not set (kpm_plan_set_dots_grid)");
   const bool local = dots_mode == KPM_DOTS_LOCAL || dots_mode == KPM_DOTS_LOCAL_COLS;
   if ((dots_mode == KPM_DOTS_PROJECT || local) && !w)
     return fail(KPM_ERR_ARG, "projection vector missing");

@@ -144,14 +144,39 @@ __device__ inline void acc2(double& s, double& c, double t) {
   c += e;
 }
 
-// Number of FP64 accumulators per column for a dots mode.  The doubling
-// mode sums <a_n|a_n> and Re <a_{n+1}|a_n>: for a Hermitian H (which the
-// expansion assumes) Im <a_{n+1}|a_n> = -Im <a_{n-1}|a_n> = ... =
-// Im <a_1|a_0> = 0 exactly, so only rounding noise would be summed there.
+// Number of TwoSum (sum, error) accumulators per column for a dots mode.
+// The doubling mode sums <a_n|a_n> and Re <a_{n+1}|a_n> this way.
 template <int MODE>
 struct NComp {
   static constexpr int value = MODE == KPM_DOTS_PROJECT ? 4 : (MODE == KPM_DOTS_DOUBLING ? 2 : 1);
 };
+
+// Im <a_{n+1}|a_n> of the doubling mode.  For a Hermitian H it is zero in
+// exact arithmetic (it equals -Im <a_{n-1}|a_n>, down to Im <a_1|a_0> = 0),
+// so what the vectors give is rounding noise: a sum that cancels.  A TwoSum
+// chain rounds such a sum differently for every launch shape, so it is
+// summed exactly instead.  Each term t (an FP64 value, computed the same way
+// in every shape) is cut on a fixed grid into two integers,
+//   t * 2^k = h + l 2^-22 (+ a tail below 2^-23, dropped),   |l| <= 2^21,
+// with k = 51 - E and 2^E > <a_0|a_0> of the column (the start vectors'
+// squared norm; sum |t| <= ||a_{n+1}|| ||a_n|| <= <a_0|a_0> while the
+// spectrum of the rescaled H lies in [-1, 1]).  Both integer sums stay below
+// 2^52 for up to 2^31 rows, so every FP64 addition of them is exact and the
+// result does not depend on the order: the same bits for every launch shape,
+// block width and resumed run.  The grid 2^(E-73) is 2^-73 of <a_0|a_0>.
+template <int MODE>
+struct NQuant {
+  static constexpr int value = MODE == KPM_DOTS_DOUBLING ? 2 : 0;
+};
+constexpr double kQuantStep = 4194304.0;          // 2^22
+constexpr double kQuantInv = 1.0 / 4194304.0;     // 2^-22
+
+__device__ inline void quant_add(double& h, double& l, double t, int k) {
+  const double scaled = ldexp(t, k);
+  const double hi = rint(scaled);
+  h += hi;
+  l += rint((scaled - hi) * kQuantStep);          // scaled - hi is exact
+}
 
 // Position of accumulator c in the 4-double output record of a column.
 template <int MODE>
@@ -247,18 +272,23 @@ __device__ inline void spmm_rows_scalar(const int32_t* __restrict__ indptr,
   }
 }
 
-// FP64 dot contributions of one row (lane columns) to the accumulators.
-template <typename T, int VEC, int MODE, int NC>
+// FP64 dot contributions of one row (lane columns) to the accumulators:
+// TwoSum pairs (as, ac) and, in the doubling mode, the exact grid sums aq of
+// Im <a_{n+1}|a_n> with the columns' grid exponents qk.
+template <typename T, int VEC, int MODE, int NC, int NQ>
 __device__ inline void add_dots(double (&as)[NC][VEC], double (&ac)[NC][VEC],
+                                double (&aq)[NQ][VEC], const int (&qk)[VEC],
                                 const CLane<T, VEC>& xs, const CLane<T, VEC>& y,
                                 const CLane<T, VEC>& ww) {
-  if (MODE == KPM_DOTS_DOUBLING) {
+  if constexpr (MODE == KPM_DOTS_DOUBLING) {
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
       const double xr = xs.re[v], xi = xs.im[v];
       const double yr = y.re[v], yi = y.im[v];
       acc2(as[0][v], ac[0][v], fma(xr, xr, xi * xi));
       acc2(as[1][v], ac[1][v], fma(yr, xr, yi * xi));
+      const double t = fma(yr, xi, -(yi * xr));
+      quant_add(aq[0][v], aq[1][v], t, qk[v]);
     }
   } else if (MODE == KPM_DOTS_PROJECT) {
 #pragma unroll
@@ -405,9 +435,10 @@ __device__ inline int64_t iter_row0(int it, int64_t begin, int64_t stride, int i
   return begin + m * stride + int64_t(it - m * ipc) * rows_per_iter;
 }
 
-template <int NC, int MODE>
+template <int MODE>
 __device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
                                    const int32_t* __restrict__ seg_blk, int nseg, int W,
+                                   const int32_t* __restrict__ qexp,
                                    double* __restrict__ rec_out, int first, int stride);
 
 // The CSR arrays are separate __restrict__ arguments: nothing the kernel
@@ -441,8 +472,8 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
 
   if (MODE != KPM_DOTS_NONE && MODE != KPM_DOTS_LOCAL && p.prev_part != nullptr) {
     // the previous step's dots: its partials are complete (kernel boundary)
-    reduce_sums<NC, MODE>(p.prev_part, p.nblocks, p.seg_blk, p.nseg, W, p.prev_dots,
-                          lb * WAVES + wave, p.nblocks * WAVES);
+    reduce_sums<MODE>(p.prev_part, p.nblocks, p.seg_blk, p.nseg, W, p.im_exp, p.prev_dots,
+                      lb * WAVES + wave, p.nblocks * WAVES);
   }
   const int seg = p.blk_seg[lb];
   const int2 rr = p.blk_rows[lb];
@@ -471,6 +502,17 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
       as[c][v] = 0.0;
       ac[c][v] = 0.0;
     }
+  // exact grid sums (doubling mode) and the columns' grid scales
+  constexpr int NQ = NQuant<MODE>::value;
+  constexpr int NQA = NQ > 0 ? NQ : 1;
+  double aq[NQA][VEC];
+  int qk[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+#pragma unroll
+    for (int c = 0; c < NQA; ++c) aq[c][v] = 0.0;
+    qk[v] = NQ > 0 && p.im_exp != nullptr ? p.im_exp[seg * W + col + v] : 0;
+  }
 
   if constexpr (RPW <= 2) {
     // In each chunk a wave owns a contiguous range of ipc * RPW rows and walks
@@ -523,7 +565,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
           zero(xs[u]);
           zero(y[u]);
         }
-        add_dots<T, VEC, MODE, NC>(as, ac, xs[u], y[u], ww[u]);
+        add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, xs[u], y[u], ww[u]);
         if constexpr (MODE == KPM_DOTS_LOCAL)
           write_local<T, VEC, LPR>(p, r0 + u * RPW + grp, live[u], col, only_dots ? xs[u] : y[u],
                                    ww[u]);
@@ -659,7 +701,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         zero(xs);
         zero(y);
       }
-      add_dots<T, VEC, MODE, NC>(as, ac, xs, y, ww);
+      add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, xs, y, ww);
       if constexpr (MODE == KPM_DOTS_LOCAL)
         write_local<T, VEC, LPR>(p, row, live, col, only_dots ? xs : y, ww);
 #pragma unroll
@@ -708,7 +750,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         zero(xs);
         zero(y);
       }
-      add_dots<T, VEC, MODE, NC>(as, ac, xs, y, ww);
+      add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, xs, y, ww);
       if constexpr (MODE == KPM_DOTS_LOCAL)
         write_local<T, VEC, LPR>(p, row, live, col, only_dots ? xs : y, ww);
     }
@@ -728,9 +770,14 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         acc2(as[c][v], ac[c][v], os);
         ac[c][v] += oc;
       }
+#pragma unroll
+    for (int c = 0; c < NQ; ++c)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) aq[c][v] += __shfl_xor(aq[c][v], off);  // exact
   }
   __shared__ double red_s[WAVES][NC][W];
   __shared__ double red_c[WAVES][NC][W];
+  __shared__ double red_q[WAVES][NQA][W];
   if (grp == 0) {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -739,18 +786,28 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         red_s[wave][c][col + v] = as[c][v];
         red_c[wave][c][col + v] = ac[c][v];
       }
+#pragma unroll
+    for (int c = 0; c < NQ; ++c)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) red_q[wave][c][col + v] = aq[c][v];
   }
   __syncthreads();
   // Workgroup partials are (sum, error) pairs, stored column-major over
   // workgroups -- part[((c * W + col) * nblocks + blk) * 2 + {0, 1}] -- so that
-  // the reduction of one sum reads contiguous memory.
-  for (int t = tid; t < NC * W; t += WAVES * 64) {
+  // the reduction of one sum reads contiguous memory.  The grid sums follow
+  // the TwoSum accumulators as c = NC .. NC + NQ - 1, with a zero error.
+  for (int t = tid; t < (NC + NQ) * W; t += WAVES * 64) {
     const int c = t / W, cc = t % W;
     double s = 0.0, e = 0.0;
+    if (c < NC) {
 #pragma unroll
-    for (int wv2 = 0; wv2 < WAVES; ++wv2) {
-      acc2(s, e, red_s[wv2][c][cc]);
-      e += red_c[wv2][c][cc];
+      for (int wv2 = 0; wv2 < WAVES; ++wv2) {
+        acc2(s, e, red_s[wv2][c][cc]);
+        e += red_c[wv2][c][cc];
+      }
+    } else {
+#pragma unroll
+      for (int wv2 = 0; wv2 < WAVES; ++wv2) s += red_q[wv2][c - NC][cc];
     }
     double* dst = p.part + ((int64_t(c) * W + cc) * p.nblocks + lb) * 2;
     dst[0] = s;
@@ -761,18 +818,45 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
 // One wavefront sums the workgroup partials of one (segment, accumulator,
 // column) and writes it to the dots record: lane l takes the segment's
 // blocks l, l + 64, ... in order, then a fixed butterfly combines the lanes.
-// Sums q = first, first + stride, ... of the nseg * NC * W are done.
-template <int NC, int MODE>
+// The doubling mode's exact grid sums of Im <a_{n+1}|a_n> are one more
+// group: one wavefront adds up the three integer levels (exact) and scales
+// them back.  Groups q = first, first + stride, ... of the
+// nseg * (NC + grid) * W are done.
+template <int MODE>
 __device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
                                    const int32_t* __restrict__ seg_blk, int nseg, int W,
+                                   const int32_t* __restrict__ qexp,
                                    double* __restrict__ rec_out, int first, int stride) {
+  constexpr int NC = NComp<MODE>::value;
+  constexpr int NQ = NQuant<MODE>::value;
+  constexpr int NG = NC + (NQ > 0 ? 1 : 0);
   const int lane = threadIdx.x & 63;
-  const int total = nseg * NC * W;
+  const int total = nseg * NG * W;
   for (int q = first; q < total; q += stride) {
-    const int seg = q / (NC * W);
-    const int rem = q - seg * NC * W;
+    const int seg = q / (NG * W);
+    const int rem = q - seg * NG * W;
     const int c = rem / W, cc = rem - (rem / W) * W;
     const int b0 = seg_blk[seg], b1 = seg_blk[seg + 1];
+    double* rec = rec_out + (int64_t(seg) * W + cc) * 4;
+    if (NQ > 0 && c == NC) {
+      double lev[NQ > 0 ? NQ : 1];
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const double* src = part + (int64_t(NC + j) * W + cc) * nblocks * 2;
+        double s = 0.0;
+        for (int b = b0 + lane; b < b1; b += 64) s += src[2 * b];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+        lev[j] = s;
+      }
+      if (lane == 0) {
+        // t 2^k = h + l 2^-22: both terms are scaled back exactly, then
+        // added with one rounding
+        const int k = qexp != nullptr ? qexp[int64_t(seg) * W + cc] : 0;
+        rec[3] = ldexp(lev[0], -k) + ldexp(lev[1] * kQuantInv, -k);
+      }
+      continue;
+    }
     const double* src = part + (int64_t(c) * W + cc) * nblocks * 2;
     double s = 0.0, e = 0.0;
 #pragma unroll 4
@@ -788,12 +872,8 @@ __device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
       e += oe;
     }
     if (lane == 0) {
-      double* rec = rec_out + (int64_t(seg) * W + cc) * 4;
       rec[out_pos<MODE>(c)] = s + e;
-      if (MODE == KPM_DOTS_DOUBLING && c == 0) {
-        rec[1] = 0.0;
-        rec[3] = 0.0;
-      }
+      if (MODE == KPM_DOTS_DOUBLING && c == 0) rec[1] = 0.0;  // <a_n|a_n> is real
     }
   }
 }
@@ -803,10 +883,11 @@ __device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
 template <int MODE>
 __global__ __launch_bounds__(256) void cheb_reduce(const double* __restrict__ part, int nblocks,
                                                    const int32_t* __restrict__ seg_blk,
-                                                   int nseg, int W, double* __restrict__ out) {
-  constexpr int NC = NComp<MODE>::value;
-  reduce_sums<NC, MODE>(part, nblocks, seg_blk, nseg, W, out,
-                        blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+                                                   int nseg, int W,
+                                                   const int32_t* __restrict__ qexp,
+                                                   double* __restrict__ out) {
+  reduce_sums<MODE>(part, nblocks, seg_blk, nseg, W, qexp, out,
+                    blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 
 // y[slot] = op x[slot] for an n_rows x W block (no recursion, no dots).
@@ -989,10 +1070,10 @@ constexpr int kDefaultPrefetch = 12;
 
 template <int MODE>
 int launch_reduce_mode(const StepParams& p, int width, hipStream_t stream) {
-  constexpr int NC = NComp<MODE>::value;
-  const int sums = p.nseg * NC * width;
+  constexpr int NG = NComp<MODE>::value + (NQuant<MODE>::value > 0 ? 1 : 0);
+  const int sums = p.nseg * NG * width;
   hipLaunchKernelGGL((cheb_reduce<MODE>), dim3((sums + 3) / 4), dim3(256), 0, stream, p.part,
-                     p.nblocks, p.seg_blk, p.nseg, width, p.dots);
+                     p.nblocks, p.seg_blk, p.nseg, width, p.im_exp, p.dots);
   return check_hip(hipGetLastError(), "cheb_reduce launch");
 }
 

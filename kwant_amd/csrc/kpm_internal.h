@@ -49,6 +49,8 @@ struct StepParams {
   double* part;            // this launch's block partials, (sum, error) pairs
   const double* prev_part; // previous step's partials to reduce first (or null)
   double* prev_dots;       // ... into this record (nseg x W x 4 doubles)
+  const int32_t* im_exp;   // segment x column -> k = 51 - E, the grid 2^-k of the
+                           // exact Im <a_{n+1}|a_n> sums (doubling mode; null: 0)
   double* local;           // site-resolved record of this step (KPM_DOTS_LOCAL*)
   int32_t local_cols;      // 1: one value per row and column, 0: per row
   int32_t nblocks;

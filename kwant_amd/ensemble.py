@@ -84,6 +84,7 @@ class CubicEnsemble:
             u = torch.rand((r1 - r0, num_vectors), generator=gen, device=dev,
                            dtype=torch.float64)
             self.vectors[r0:r1] = torch.polar(torch.ones_like(u), 2 * np.pi * u)
+        self.plan.set_dots_grid(self.vectors)
         self.steps = num_moments // 2
         self.dots = torch.empty((self.steps, len(self.sizes), num_vectors, 4),
                                 dtype=torch.float64, device=dev)

@@ -367,10 +367,39 @@ class _DosBlock:
         dev = eng.torch_device()
         cdt, _ = eng.complex_dtype(dmat.precision)
         host = np.stack([np.asarray(v) for v in vectors], axis=1).astype(np.complex128)
+        _check_finite_vectors(host)
         block = torch.from_numpy(np.ascontiguousarray(dmat.rows_in(host))).to(dev).to(cdt)
         self.count = len(vectors)
         self.rec = eng.BlockRecursion(dmat, ab, block, width, mode, w=w, variant=variant)
         self.moments = None  # (count, M) complex128
+
+
+def _check_finite_vectors(host):
+    """Start vectors with NaN or Inf would spread through every moment."""
+    if not np.all(np.isfinite(host)):
+        bad = sorted(set(np.nonzero(~np.isfinite(host))[1].tolist()))
+        raise ValueError("vectors %s of the block contain NaN or Inf" % bad)
+
+
+# |mu_n| <= mu_0 for every n while the rescaled spectrum lies in [-1, 1]
+# (|T_n| <= 1 there); a spectrum outside makes the moments grow like
+# cosh(n arccosh|x|), so a moment past mu_0 by this margin means the bounds
+# miss the spectrum.
+MOMENT_GROWTH_TOL = 1e-2
+
+
+def _check_moment_growth(mom, bounds):
+    """Raise if the doubling moments (count, M) of a block are not finite or
+    grow past mu_0 (their first column)."""
+    if not np.all(np.isfinite(mom)):
+        raise ValueError("the Chebyshev moments are not finite (NaN or Inf in the "
+                         "Hamiltonian or the vectors)")
+    mu0 = np.abs(mom[:, :1])
+    if np.any(np.abs(mom) > mu0 * (1 + MOMENT_GROWTH_TOL)):
+        n = int(np.nonzero(np.any(np.abs(mom) > mu0 * (1 + MOMENT_GROWTH_TOL), axis=0))[0][0])
+        raise ValueError("the Chebyshev moments grow past mu_0 (first at moment %d): the "
+                         "bounds %s do not contain the spectrum of the Hamiltonian"
+                         % (n, tuple(float(x) for x in bounds)))
 
 
 def _doubling_fill(mom, dots, first_step, mu0, mu1):
@@ -708,6 +737,7 @@ class SpectralDensity:
             if n_moments % 2:
                 tail = rec.norms()
                 mom[:, n_moments - 1] = 2 * tail[:blk.count, 0] - mu0
+            _check_moment_growth(mom, self.bounds)
             blk.moments = mom
         elif rec.mode == eng.DOTS_PROJECT:
             mom = np.zeros((blk.count, n_moments), dtype=np.complex128)
@@ -736,6 +766,7 @@ class SpectralDensity:
             if m_new % 2:
                 tail = rec.norms()
                 mom[:, m_new - 1] = 2 * tail[:blk.count, 0] - mu0
+            _check_moment_growth(mom, self.bounds)
             blk.moments = mom
         elif rec.mode == eng.DOTS_PROJECT:
             mom = np.zeros((blk.count, m_new), dtype=np.complex128)

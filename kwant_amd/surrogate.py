@@ -129,6 +129,11 @@ class SquareFamily:
                              variant=variant)
         self.plan.set_scale([self.a, self.b] * batch)
         self.ring = torch.empty((2, batch * n, num_vectors), dtype=self.cdt, device=dev)
+        # every batch's vectors are unit-modulus phases, so <a_0|a_0> = n for
+        # the seeded and the fresh ones alike: one grid serves them all
+        self.ring[0].view(batch, n, num_vectors).copy_(
+            self._vectors[None].expand(batch, n, num_vectors))
+        self.plan.set_dots_grid(self.ring[0])
         self.steps = num_moments // 2
         self.g = jackson_factors(num_moments)
 

@@ -70,6 +70,58 @@ def test_square_lattice_100x100_c128():
     assert rel(gpu.densities, ora.densities, np.abs(ora.densities).max()) < 1e-11
 
 
+def test_guards_bounds_and_non_finite_vectors():
+    """Bounds that miss the spectrum, and NaN/Inf in the start vectors, end
+    with a ValueError instead of meaningless moments."""
+    import torch
+    from kwant_amd import _engine as eng
+    h = gue(30, 6)
+    lam = np.abs(np.linalg.eigvalsh(h)).max()
+    with pytest.raises(ValueError, match="do not contain the spectrum"):
+        kpm().SpectralDensity(h, num_moments=200, num_vectors=2, bounds=(-0.8 * lam, 0.8 * lam),
+                              rng=1)
+    vecs = [np.ones(30, complex), np.ones(30, complex)]
+    vecs[1][4] = np.nan
+    with pytest.raises(ValueError, match="NaN or Inf"):
+        kpm().SpectralDensity(h, num_moments=20, num_vectors=2, vector_factory=vecs,
+                              bounds=(-1.1 * lam, 1.1 * lam))
+    # the library's own check (vectors made on the device)
+    dmat = eng.DeviceCSR(h, 'c128')
+    plan = eng.Plan(30, 4, eng.KPM_C128)
+    plan.set_scale([1.1 * lam, 0.0])
+    block = torch.ones((30, 4), dtype=torch.complex128, device=eng.torch_device())
+    block[3, 2] = float('inf')
+    with pytest.raises(ValueError, match="NaN or Inf"):
+        plan.set_dots_grid(block)
+    del dmat
+
+
+def test_moments_dtype_follows_real_if_close():
+    """kwant's _moments takes real_if_close of the moments
+    (kwant/kpm.py:378-380): the imaginary rounding noise of
+    mu_{2n+1} = 2 <a_{n+1}|a_n> - mu_1 exceeds 100 eps on the 100 x 100
+    square lattice, so kwant's densities are complex128.  The GPU sums that
+    noise exactly (on a fixed grid), so its densities are complex128 too, with
+    imaginary parts of the same (noise) size, and the same bits for every
+    block width."""
+    from kwant_amd.lattices import square
+    h, _ = square(100, disorder=1.0, seed=1)
+    kw = dict(num_moments=100, num_vectors=10, bounds=(-4.5, 4.5), rng=0)
+    gpu = kpm().SpectralDensity(h, **kw)
+    ora = ko.SpectralDensity(h, **kw)
+    g, o = np.array(gpu._moments_list), np.array(ora._moments_list)
+    assert np.iscomplexobj(np.real_if_close(o)) and np.iscomplexobj(np.real_if_close(g))
+    assert ora.densities.dtype == gpu.densities.dtype == np.complex128
+    # mu_0 and the even moments are real, as numpy's vdot(a, a) makes them
+    assert np.all(g[:, 0::2].imag == 0)
+    noise_g, noise_o = np.abs(g.imag).max(), np.abs(o.imag).max()
+    assert 2.2e-14 < noise_g < 1e-10 and 2.2e-14 < noise_o < 1e-10, (noise_g, noise_o)
+    # block widths 1, 2 and 8 give the same complex moments, bit for bit
+    for w in (1, 2, 8):
+        other = kpm().SpectralDensity(h, max_block_width=w, **kw)
+        assert np.array_equal(np.array(other._moments_list), g), w
+
+
 @pytest.mark.parametrize("width", [64, 16, 4, 1])
 def test_graphene_c64_within_1e5(width):
     from kwant_amd.lattices import honeycomb, honeycomb_bounds
@@ -433,6 +485,7 @@ def test_ensemble_segments_equal_separate_runs():
     dev = eng.torch_device()
     ring = torch.zeros((2, seg[-1], width), dtype=torch.complex128, device=dev)
     ring[0] = torch.from_numpy(vecs).to(dev)
+    plan.set_dots_grid(ring[0])
     plan.set_step(0)
     steps = 20
     dots = torch.zeros((steps, 3, width, 4), dtype=torch.float64, device=dev)
