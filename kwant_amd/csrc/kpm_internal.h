@@ -74,7 +74,7 @@ int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x,
                 int64_t x_stride, void* y, int64_t y_stride, int n_slots,
                 hipStream_t stream);
 // At most kMaxVariants launch shapes per (width, value type).
-constexpr int kMaxVariants = 64;
+constexpr int kMaxVariants = 96;
 int variants_for(int width, int value_type, KernelShape* out, int max_out);
 int blocks_per_cu(int value_type, const KernelShape& ks, int* out);
 
