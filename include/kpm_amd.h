@@ -172,6 +172,31 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring,
                    double* d_dots, int32_t out_base, int32_t out_cap,
                    int32_t n_steps, int32_t dots_mode, kpm_stream_t stream);
 
+/* The steps of kpm_cheb_steps for one system (one segment) without a plan
+ * and without allocating: no counterpart in the reference; for callers that
+ * cannot call hipMalloc (code under HIP graph capture, tight serving loops).
+ * The workspace (workgroup partials, workgroup row ranges, the rescaling) is
+ * a fixed 16 MiB device array per GPU inside the library, filled by kernels
+ * on `stream`: a call is a fixed sequence of kernel launches with no host
+ * synchronisation and no host-to-device copy.  The call fails with
+ * KPM_ERR_UNSUPPORTED when its workspace exceeds that array (about
+ * 2 x blocks x 4 x width x 16 bytes, blocks ~ CUs x resident workgroups x
+ * residency rounds of the variant).  Every call on a GPU shares the array:
+ * calls that overlap on one GPU race (serialise them on one stream).
+ * Arguments as kpm_cheb_steps, with the plan's state passed explicitly:
+ * (a, b) the rescaling, width and variant the plan's, first_step the step
+ * counter n of the first step.  grid_exp: device array of width int32, the
+ * grid of the exact Im <a_{n+1}|a_n> sums (kpm_plan_set_dots_grid): written
+ * from ring slot 0 (a_0) when first_step == 0, read otherwise -- keep it
+ * with the ring between calls; required in the doubling mode.  Unlike
+ * kpm_plan_set_dots_grid it does not report NaN/Inf in a_0 (that would need
+ * a synchronisation); such moments come out NaN.  Asynchronous. */
+int kpm_cheb_steps_static(const kpm_csr* h, double a, double b, int32_t width,
+                          int32_t variant, void* ring, int64_t slot_stride, int32_t slots,
+                          const void* w, int32_t* grid_exp, int32_t first_step,
+                          int32_t n_steps, double* d_dots, int32_t out_base, int32_t out_cap,
+                          int32_t dots_mode, kpm_stream_t stream);
+
 /* Dots of one block without a matrix product: per segment and column,
  * DOUBLING: { <x|x>, 0, 0, 0 }; PROJECT: { <w|x>, 0, 0 }.  Written to
  * d_out[s * width + c] (4 doubles).  This is the odd-moment tail

@@ -37,7 +37,8 @@ MLP_ACCUMULATE = 32
 EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
-    "kpm_plan_set_dots_grid", "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps", "kpm_block_dots",
+    "kpm_plan_set_dots_grid", "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps",
+    "kpm_cheb_steps_static", "kpm_block_dots",
     "kpm_bond_records", "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
     "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
     "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
@@ -124,6 +125,9 @@ def load_library(path=LIB_PATH):
         "kpm_plan_get_step": (ctypes.c_int, [vp, pi32, vp]),
         "kpm_cheb_steps": (ctypes.c_int, [vp, ctypes.POINTER(KpmCsr), vp, i64, i32, vp,
                                           vp, i32, i32, i32, i32, vp]),
+        "kpm_cheb_steps_static": (ctypes.c_int, [ctypes.POINTER(KpmCsr), ctypes.c_double,
+                                                 ctypes.c_double, i32, i32, vp, i64, i32, vp,
+                                                 vp, i32, i32, vp, i32, i32, i32, vp]),
         "kpm_block_dots": (ctypes.c_int, [vp, vp, vp, vp, i32, vp]),
         "kpm_bond_records": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, vp, vp, vp, i32, i32,
                                             vp, vp]),
@@ -370,6 +374,23 @@ class Plan:
         check(load_library().kpm_block_dots(
             self.handle, x.data_ptr(), None if w is None else w.data_ptr(), out.data_ptr(),
             mode, stream_handle()), "kpm_block_dots")
+
+
+def cheb_steps_static(mat, ab, ring, slots, first_step, n_steps, mode, dots=None,
+                      out_base=0, w=None, grid_exp=None, variant=0):
+    """kpm_cheb_steps_static: ``n_steps`` steps from step ``first_step`` of
+    the (slots, N, width) ring, with no plan and no device allocation (the
+    library's static workspace).  ``grid_exp``: (width,) int32 device tensor,
+    written when first_step == 0, read otherwise (doubling mode)."""
+    width = int(ring.shape[-1])
+    out_cap = 0 if dots is None else int(dots.shape[0])
+    a, b = (float(x) for x in ab)
+    check(load_library().kpm_cheb_steps_static(
+        ctypes.byref(mat.desc), a, b, width, variant, ring.data_ptr(),
+        mat.shape[0] * width, slots, None if w is None else w.data_ptr(),
+        None if grid_exp is None else grid_exp.data_ptr(), first_step, n_steps,
+        None if dots is None else dots.data_ptr(), out_base, out_cap, mode,
+        stream_handle()), "kpm_cheb_steps_static")
 
 
 def pow2_widths(count, max_width):
