@@ -64,6 +64,22 @@ typedef struct {
   int32_t row_nnz_hint;
 } kpm_csr;
 
+/* The same matrix in block-CSR with 4 x 4 blocks (the norbs x norbs site
+ * blocks kwant/_system.pyx:94-153 assembles for 4-orbital sites): indptr has
+ * n_block_rows+1 int32 entries, indices nnzb int32 block columns, values
+ * nnzb x 4 x 4 complex entries of value_type, each block row-major.  Zeros
+ * inside a block are stored.  block_size must be 4. */
+typedef struct {
+  int32_t n_block_rows;
+  int32_t n_block_cols;
+  int64_t nnzb;
+  const int32_t* indptr;
+  const int32_t* indices;
+  const void* values;
+  int32_t value_type;
+  int32_t block_size;
+} kpm_bsr;
+
 typedef struct kpm_plan kpm_plan;
 
 /* Last error message of the calling thread ("" if none). */
@@ -171,6 +187,17 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring,
                    int64_t slot_stride, int32_t slots, const void* w,
                    double* d_dots, int32_t out_base, int32_t out_cap,
                    int32_t n_steps, int32_t dots_mode, kpm_stream_t stream);
+
+/* kpm_cheb_steps with the matrix in 4 x 4 block-CSR: the same recursion
+ * (kwant/kpm.py:436-473 with the rescaled matvec of :1182-1183), the four
+ * rows of a site gathered as one block.  Modes KPM_DOTS_DOUBLING and
+ * KPM_DOTS_NONE; the plan's rows and segment boundaries must be multiples
+ * of 4 and its variant a 4-wave shape with at most 16 lanes per row.  With
+ * the blocks' columns in order the moments are bitwise those of the
+ * column-sorted CSR (the padding zeros add exact zeros). */
+int kpm_bsr_steps(kpm_plan* plan, const kpm_bsr* h, void* ring, int64_t slot_stride,
+                  int32_t slots, double* d_dots, int32_t out_base, int32_t out_cap,
+                  int32_t n_steps, int32_t dots_mode, kpm_stream_t stream);
 
 /* The steps of kpm_cheb_steps for one system (one segment) without a plan
  * and without allocating: no counterpart in the reference; for callers that

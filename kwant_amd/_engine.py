@@ -38,7 +38,7 @@ EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
     "kpm_plan_set_dots_grid", "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps",
-    "kpm_cheb_steps_static", "kpm_block_dots",
+    "kpm_cheb_steps_static", "kpm_bsr_steps", "kpm_block_dots",
     "kpm_bond_records", "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
     "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
     "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
@@ -128,6 +128,8 @@ def load_library(path=LIB_PATH):
         "kpm_cheb_steps_static": (ctypes.c_int, [ctypes.POINTER(KpmCsr), ctypes.c_double,
                                                  ctypes.c_double, i32, i32, vp, i64, i32, vp,
                                                  vp, i32, i32, vp, i32, i32, i32, vp]),
+        "kpm_bsr_steps": (ctypes.c_int, [vp, ctypes.POINTER(KpmBsr), vp, i64, i32, vp, i32,
+                                         i32, i32, i32, vp]),
         "kpm_block_dots": (ctypes.c_int, [vp, vp, vp, vp, i32, vp]),
         "kpm_bond_records": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, vp, vp, vp, i32, i32,
                                             vp, vp]),
@@ -308,6 +310,72 @@ class DeviceCSR:
                            y_stride, width, n_slots, stream_handle()), "kpm_spmm")
 
 
+class KpmBsr(ctypes.Structure):
+    _fields_ = [
+        ("n_block_rows", ctypes.c_int32),
+        ("n_block_cols", ctypes.c_int32),
+        ("nnzb", ctypes.c_int64),
+        ("indptr", ctypes.c_void_p),
+        ("indices", ctypes.c_void_p),
+        ("values", ctypes.c_void_p),
+        ("value_type", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+    ]
+
+
+class DeviceBSR:
+    """The same matrix in 4 x 4 block-CSR on the GPU (kpm_bsr_steps): the
+    site blocks of a 4-orbital system, the four rows of a site gathered as one
+    block.  Built from the column-sorted CSR, so the moments equal the CSR
+    path's bit for bit.  Rows stay in the caller's order."""
+
+    format = "bsr"
+    BLOCK = 4
+
+    def __init__(self, matrix, precision="c128"):
+        import torch
+        dev = torch_device()
+        csr = scipy.sparse.csr_matrix(matrix)
+        n = csr.shape[0]
+        if csr.shape[1] != n or n % self.BLOCK:
+            raise ValueError("block-CSR needs a square matrix whose size is a multiple of 4")
+        csr = csr.copy()
+        csr.sort_indices()
+        bsr = scipy.sparse.bsr_matrix(csr, blocksize=(self.BLOCK, self.BLOCK))
+        bsr.sort_indices()
+        if bsr.indices.size * 16 >= 2 ** 31:
+            raise ValueError("matrix too large for int32 block indices")
+        self.order = self.inverse = None
+        self.shape = csr.shape
+        self.nnz = int(csr.nnz)                 # entries of the CSR it holds
+        self.nnzb = int(bsr.indices.size)       # stored 4 x 4 blocks
+        self.precision = precision
+        cdt, self.value_type = complex_dtype(precision)
+        np_cdt = np.complex128 if self.value_type == KPM_C128 else np.complex64
+        self.indptr = torch.from_numpy(np.ascontiguousarray(bsr.indptr, dtype=np.int32)).to(dev)
+        self.indices = torch.from_numpy(
+            np.ascontiguousarray(bsr.indices, dtype=np.int32)).to(dev)
+        vals = np.ascontiguousarray(bsr.data, dtype=np.complex128).astype(np_cdt).reshape(-1)
+        if vals.size == 0:
+            vals = np.zeros(1, dtype=np_cdt)
+        self.values = torch.from_numpy(vals).to(dev)
+        self.desc = KpmBsr(n // self.BLOCK, n // self.BLOCK, self.nnzb, self.indptr.data_ptr(),
+                           self.indices.data_ptr(), self.values.data_ptr(), self.value_type,
+                           self.BLOCK)
+
+    def rows_in(self, host):
+        return host
+
+    def rows_out(self, host):
+        return host
+
+    @staticmethod
+    def variant_ok(width, value_type, variant):
+        """Block-CSR steps run 4-wave shapes with at most 16 lanes per row."""
+        lpr, _, waves, *_ = variant_info(width, value_type, variant)
+        return lpr <= 16 and waves == 4
+
+
 class Plan:
     """A recursion plan (kernel shape, row partition into segments, workspace)."""
 
@@ -364,6 +432,14 @@ class Plan:
     def steps(self, mat, ring, slot_stride, slots, n_steps, mode, dots=None,
               out_base=0, w=None):
         out_cap = 0 if dots is None else int(dots.shape[0])
+        if isinstance(mat, DeviceBSR):
+            if w is not None:
+                raise ValueError("block-CSR steps have no projection modes")
+            check(load_library().kpm_bsr_steps(
+                self.handle, ctypes.byref(mat.desc), ring.data_ptr(), slot_stride, slots,
+                None if dots is None else dots.data_ptr(), out_base, out_cap, n_steps, mode,
+                stream_handle()), "kpm_bsr_steps")
+            return
         check(load_library().kpm_cheb_steps(
             self.handle, ctypes.byref(mat.desc), ring.data_ptr(), slot_stride, slots,
             None if w is None else w.data_ptr(),

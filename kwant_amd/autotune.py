@@ -51,7 +51,17 @@ def _store():
 
 
 def key_of(dmat, width, mode):
-    return "%d:%d:%d:%d:%d" % (dmat.shape[0], dmat.nnz, width, dmat.value_type, mode)
+    key = "%d:%d:%d:%d:%d" % (dmat.shape[0], dmat.nnz, width, dmat.value_type, mode)
+    fmt = getattr(dmat, "format", "csr")
+    return key if fmt == "csr" else key + ":" + fmt
+
+
+def candidates(dmat, width):
+    """The launch shapes that can run this matrix (block-CSR: 4-wave shapes
+    with at most 16 lanes per row)."""
+    n_var = eng.variant_count(width, dmat.value_type)
+    ok = getattr(dmat, "variant_ok", None)
+    return [v for v in range(n_var) if ok is None or ok(width, dmat.value_type, v)]
 
 
 # Modes timed as another: the site-resolved records cost the projection
@@ -82,14 +92,14 @@ def time_variant(dmat, ab, width, mode, variant, block, steps=TIMED_STEPS):
 
 
 def pick_variant(dmat, ab, width, mode=eng.DOTS_DOUBLING, force=False):
-    """The fastest launch shape for this matrix and width (0 below the size
-    threshold unless ``force``)."""
+    """The fastest launch shape for this matrix and width (below the size
+    threshold, unless ``force``, the first shape that can run it)."""
     import torch
-    n_var = eng.variant_count(width, dmat.value_type)
-    if n_var <= 1:
-        return 0
-    if not force and dmat.shape[0] * width < MIN_ELEMENTS:
-        return 0
+    cands = candidates(dmat, width)
+    if not cands:
+        raise ValueError("no launch shape runs this matrix at width %d" % width)
+    if len(cands) == 1 or (not force and dmat.shape[0] * width < MIN_ELEMENTS):
+        return cands[0]
     if not _CACHE:
         _load()
     key = key_of(dmat, width, mode)
@@ -101,12 +111,13 @@ def pick_variant(dmat, ab, width, mode=eng.DOTS_DOUBLING, force=False):
     gen.manual_seed(0)
     phase = torch.rand((dmat.shape[0], width), generator=gen, device=dev, dtype=torch.float64)
     block = torch.exp(2j * 3.141592653589793 * phase).to(cdt)
-    times = [time_variant(dmat, ab, width, mode, v, block) for v in range(n_var)]
+    times = {v: time_variant(dmat, ab, width, mode, v, block) for v in cands}
     # the fastest few are timed again, three times longer, so that run-to-run
     # noise does not decide between near-equal shapes
-    short = sorted(range(n_var), key=lambda v: times[v])[:FINALISTS]
+    short = sorted(cands, key=lambda v: times[v])[:FINALISTS]
     final = {v: time_variant(dmat, ab, width, mode, v, block, 3 * TIMED_STEPS) for v in short}
     best = min(final, key=lambda v: final[v])
-    _CACHE[key] = {"variant": best, "ms_per_step": times}
+    _CACHE[key] = {"variant": best, "ms_per_step": [times.get(v) for v in
+                                                    range(max(cands) + 1)]}
     _store()
     return best

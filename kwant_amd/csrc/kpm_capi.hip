@@ -48,6 +48,7 @@ struct kpm_plan {
   double* grid_dots = nullptr;  // nseg x width x 4: <a_0|a_0> for the grid
   bool scale_set = false;
   bool grid_set = false;
+  std::vector<int32_t> seg_rows;  // segment row boundaries (host copy)
 };
 
 namespace {
@@ -176,6 +177,7 @@ int kpm_plan_create(int32_t n_rows, int32_t width, int32_t value_type, int32_t n
   p->value_type = value_type;
   p->nseg = n_segments;
   p->shape = ks[variant];
+  p->seg_rows.assign(seg_row_begin, seg_row_begin + n_segments + 1);
   int occ = 0, cus = 0, dev = 0;
   int rc0 = check_hip(hipGetDevice(&dev), "hipGetDevice");
   if (!rc0)
@@ -410,6 +412,69 @@ int kpm_cheb_steps(kpm_plan* plan, const kpm_csr* h, void* ring, int64_t slot_st
     plan->step = n + 1;
   }
   if (dots_mode != KPM_DOTS_NONE && !local && n_steps > 0) {
+    const int last = plan->step - 1;
+    p.part = plan->part + (last & 1) * half;
+    p.dots = d_dots + int64_t(last - out_base) * rec;
+    rc = launch_reduce(dots_mode, plan->width, p, (hipStream_t)stream);
+  }
+  return rc;
+}
+
+int kpm_bsr_steps(kpm_plan* plan, const kpm_bsr* h, void* ring, int64_t slot_stride,
+                  int32_t slots, double* d_dots, int32_t out_base, int32_t out_cap,
+                  int32_t n_steps, int32_t dots_mode, kpm_stream_t stream) {
+  if (!plan || !h) return fail(KPM_ERR_ARG, "NULL argument");
+  if (h->block_size != 4) return fail(KPM_ERR_UNSUPPORTED, "block size must be 4");
+  if (h->value_type != plan->value_type)
+    return fail(KPM_ERR_ARG, "matrix value type differs from the plan's");
+  if (int64_t(h->n_block_rows) * 4 != plan->n_rows || h->n_block_cols != h->n_block_rows)
+    return fail(KPM_ERR_ARG, "block matrix must be square with the plan's rows / 4 blocks");
+  if (!h->indptr || (h->nnzb > 0 && (!h->indices || !h->values)))
+    return fail(KPM_ERR_ARG, "block matrix arrays are NULL");
+  if (h->nnzb < 0 || h->nnzb > INT32_MAX / 16)
+    return fail(KPM_ERR_ARG, "nnzb out of range");
+  for (int32_t r : plan->seg_rows)
+    if (r % 4) return fail(KPM_ERR_ARG, "segment boundaries must be multiples of 4");
+  if (!plan->scale_set) return fail(KPM_ERR_ARG, "scale not set (kpm_plan_set_scale)");
+  if (!ring) return fail(KPM_ERR_ARG, "ring is NULL");
+  if (slots < 2) return fail(KPM_ERR_ARG, "ring needs at least two slots");
+  if (slot_stride < int64_t(plan->n_rows) * plan->width)
+    return fail(KPM_ERR_ARG, "slot_stride smaller than n_rows * width");
+  if (dots_mode != KPM_DOTS_DOUBLING && dots_mode != KPM_DOTS_NONE)
+    return fail(KPM_ERR_ARG, "dots_mode must be KPM_DOTS_DOUBLING or KPM_DOTS_NONE");
+  if (dots_mode == KPM_DOTS_DOUBLING && (!d_dots || out_cap <= 0))
+    return fail(KPM_ERR_ARG, "dots output buffer missing");
+  if (dots_mode == KPM_DOTS_DOUBLING && !plan->grid_set)
+    return fail(KPM_ERR_ARG, "dots grid not set (kpm_plan_set_dots_grid)");
+  if (n_steps < 0) return fail(KPM_ERR_ARG, "n_steps must be non-negative");
+  if (dots_mode != KPM_DOTS_NONE &&
+      (plan->step - out_base < 0 || plan->step + n_steps - out_base > out_cap))
+    return fail(KPM_ERR_ARG, "dots slots [step - out_base, + n_steps) outside [0, out_cap)");
+  if (plan->shape.lpr > 16 || plan->shape.waves != 4)
+    return fail(KPM_ERR_UNSUPPORTED,
+                "block-CSR steps need a 4-wave variant with at most 16 lanes per row");
+  StepParams p{};
+  p.ring = reinterpret_cast<char*>(ring);
+  p.slot_bytes = slot_stride * int64_t(elem_bytes(plan->value_type));
+  p.slots = slots;
+  fill_params(plan, p);
+  const int64_t rec = int64_t(plan->nseg) * plan->width * 4;
+  const int64_t half = plan->part_half;
+  int rc = KPM_OK;
+  for (int i = 0; i < n_steps; ++i) {
+    const int n = plan->step;
+    p.part = plan->part + (n & 1) * half;
+    p.prev_part = nullptr;
+    p.prev_dots = nullptr;
+    if (dots_mode != KPM_DOTS_NONE && i > 0) {
+      p.prev_part = plan->part + ((n - 1) & 1) * half;
+      p.prev_dots = d_dots + int64_t(n - 1 - out_base) * rec;
+    }
+    rc = launch_bsr(plan->shape, dots_mode, n, h, p, (hipStream_t)stream);
+    if (rc) return rc;
+    plan->step = n + 1;
+  }
+  if (dots_mode != KPM_DOTS_NONE && n_steps > 0) {
     const int last = plan->step - 1;
     p.part = plan->part + (last & 1) * half;
     p.dots = d_dots + int64_t(last - out_base) * rec;

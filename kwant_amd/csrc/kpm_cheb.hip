@@ -441,6 +441,69 @@ __device__ inline void reduce_sums(const double* __restrict__ part, int nblocks,
                                    const int32_t* __restrict__ qexp,
                                    double* __restrict__ rec_out, int first, int stride);
 
+// Workgroup reduction of the dot accumulators (row groups of a wave, then
+// waves via LDS) and the store of the workgroup's partials.
+template <int VEC, int LPR, int WAVES, int NC, int NQ, int NQA>
+__device__ inline void store_partials(const StepParams& p, int lb, int wave, int grp, int col,
+                                      int tid, double (&as)[NC][VEC], double (&ac)[NC][VEC],
+                                      double (&aq)[NQA][VEC]) {
+  constexpr int W = LPR * VEC;
+#pragma unroll
+  for (int off = LPR; off < 64; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const double os = __shfl_xor(as[c][v], off);
+        const double oc = __shfl_xor(ac[c][v], off);
+        acc2(as[c][v], ac[c][v], os);
+        ac[c][v] += oc;
+      }
+#pragma unroll
+    for (int c = 0; c < NQ; ++c)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) aq[c][v] += __shfl_xor(aq[c][v], off);  // exact
+  }
+  __shared__ double red_s[WAVES][NC][W];
+  __shared__ double red_c[WAVES][NC][W];
+  __shared__ double red_q[WAVES][NQA][W];
+  if (grp == 0) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        red_s[wave][c][col + v] = as[c][v];
+        red_c[wave][c][col + v] = ac[c][v];
+      }
+#pragma unroll
+    for (int c = 0; c < NQ; ++c)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) red_q[wave][c][col + v] = aq[c][v];
+  }
+  __syncthreads();
+  // Workgroup partials are (sum, error) pairs, stored column-major over
+  // workgroups -- part[((c * W + col) * nblocks + blk) * 2 + {0, 1}] -- so that
+  // the reduction of one sum reads contiguous memory.  The grid sums follow
+  // the TwoSum accumulators as c = NC .. NC + NQ - 1, with a zero error.
+  for (int t = tid; t < (NC + NQ) * W; t += WAVES * 64) {
+    const int c = t / W, cc = t % W;
+    double s = 0.0, e = 0.0;
+    if (c < NC) {
+#pragma unroll
+      for (int wv2 = 0; wv2 < WAVES; ++wv2) {
+        acc2(s, e, red_s[wv2][c][cc]);
+        e += red_c[wv2][c][cc];
+      }
+    } else {
+#pragma unroll
+      for (int wv2 = 0; wv2 < WAVES; ++wv2) s += red_q[wv2][c - NC][cc];
+    }
+    double* dst = p.part + ((int64_t(c) * W + cc) * p.nblocks + lb) * 2;
+    dst[0] = s;
+    dst[1] = e;
+  }
+}
+
 // The CSR arrays are separate __restrict__ arguments: nothing the kernel
 // stores can alias them, so for one or two rows per wavefront (LPR >= 32) the
 // compiler reads them with scalar loads, which complete on their own counter
@@ -758,61 +821,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
 
   if (MODE == KPM_DOTS_NONE || MODE == KPM_DOTS_LOCAL) return;
 
-  // ---- workgroup reduction: row groups of a wave, then waves via LDS ----
-#pragma unroll
-  for (int off = LPR; off < 64; off <<= 1) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        const double os = __shfl_xor(as[c][v], off);
-        const double oc = __shfl_xor(ac[c][v], off);
-        acc2(as[c][v], ac[c][v], os);
-        ac[c][v] += oc;
-      }
-#pragma unroll
-    for (int c = 0; c < NQ; ++c)
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) aq[c][v] += __shfl_xor(aq[c][v], off);  // exact
-  }
-  __shared__ double red_s[WAVES][NC][W];
-  __shared__ double red_c[WAVES][NC][W];
-  __shared__ double red_q[WAVES][NQA][W];
-  if (grp == 0) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        red_s[wave][c][col + v] = as[c][v];
-        red_c[wave][c][col + v] = ac[c][v];
-      }
-#pragma unroll
-    for (int c = 0; c < NQ; ++c)
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) red_q[wave][c][col + v] = aq[c][v];
-  }
-  __syncthreads();
-  // Workgroup partials are (sum, error) pairs, stored column-major over
-  // workgroups -- part[((c * W + col) * nblocks + blk) * 2 + {0, 1}] -- so that
-  // the reduction of one sum reads contiguous memory.  The grid sums follow
-  // the TwoSum accumulators as c = NC .. NC + NQ - 1, with a zero error.
-  for (int t = tid; t < (NC + NQ) * W; t += WAVES * 64) {
-    const int c = t / W, cc = t % W;
-    double s = 0.0, e = 0.0;
-    if (c < NC) {
-#pragma unroll
-      for (int wv2 = 0; wv2 < WAVES; ++wv2) {
-        acc2(s, e, red_s[wv2][c][cc]);
-        e += red_c[wv2][c][cc];
-      }
-    } else {
-#pragma unroll
-      for (int wv2 = 0; wv2 < WAVES; ++wv2) s += red_q[wv2][c - NC][cc];
-    }
-    double* dst = p.part + ((int64_t(c) * W + cc) * p.nblocks + lb) * 2;
-    dst[0] = s;
-    dst[1] = e;
-  }
+  store_partials<VEC, LPR, WAVES, NC, NQ, NQA>(p, lb, wave, grp, col, tid, as, ac, aq);
 }
 
 // One wavefront sums the workgroup partials of one (segment, accumulator,
@@ -888,6 +897,144 @@ __global__ __launch_bounds__(256) void cheb_reduce(const double* __restrict__ pa
                                                    double* __restrict__ out) {
   reduce_sums<MODE>(part, nblocks, seg_blk, nseg, W, qexp, out,
                     blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+}
+
+// The Chebyshev step with the matrix in 4 x 4 block-CSR (the norbs x norbs
+// site blocks kwant assembles, kwant/_system.pyx:94-153).  The lane layout is
+// cheb_kernel's per-lane one (LPR lanes x VEC columns per row) with the four
+// rows of a block-row in consecutive lane groups, so the 4 * LPR lanes of a
+// block-row load a neighbour block's four rows of a_n as one contiguous
+// 4 * W complex read and exchange them with shuffles; each lane then applies
+// its row of the 4 x 4 block.  Blocks follow in column order and a block's
+// columns in order, so every row sum is the same FMA sequence as the CSR
+// kernel's on the same (column-sorted) entries -- the padding zeros of a
+// block add exact zeros.  Dots, partials and the epilogue are cheb_kernel's.
+template <typename T, int VEC, int LPR, int WAVES, int MODE>
+__global__ __launch_bounds__(WAVES * 64) void bsr_kernel(StepParams p,
+                                                         const int32_t* __restrict__ bptr,
+                                                         const int32_t* __restrict__ bidx,
+                                                         const void* __restrict__ bval, int n,
+                                                         int iters) {
+  static_assert(LPR <= 16 && 64 % (4 * LPR) == 0, "a block-row needs 4 lane groups");
+  constexpr int W = LPR * VEC;
+  constexpr int RPW = 64 / LPR;
+  constexpr int NC = NComp<MODE>::value;
+  constexpr int NQ = NQuant<MODE>::value;
+  constexpr int NQA = NQ > 0 ? NQ : 1;
+  using IO = LaneIO<T, VEC>;
+
+  const int hb = blockIdx.x;
+  const int per = p.nblocks_launch / p.xcd;
+  const int lb = (hb % p.xcd) * per + hb / p.xcd;
+  if (lb >= p.nblocks) return;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int grp = lane / LPR;
+  const int sub = lane % LPR;
+  const int col = sub * VEC;
+  const int brow = grp & 3;                  // row within the block
+  const int gbase = (grp & ~3) * LPR + sub;  // lane of block row 0, same columns
+
+  if (MODE == KPM_DOTS_DOUBLING && p.prev_part != nullptr)
+    reduce_sums<MODE>(p.prev_part, p.nblocks, p.seg_blk, p.nseg, W, p.im_exp, p.prev_dots,
+                      lb * WAVES + wave, p.nblocks * WAVES);
+  const int seg = p.blk_seg[lb];
+  const int2 rr = p.blk_rows[lb];
+  const int64_t stride = p.blk_stride[lb];
+  const int ipc = p.chunk_iters > 0 ? p.chunk_iters : iters;
+  const double sa = p.scale[2 * seg];
+  const double sb = p.scale[2 * seg + 1];
+  const T mul = T((n == 0 ? 1.0 : 2.0) / sa);
+  const T bsh = T(sb);
+  const T* cur = reinterpret_cast<const T*>(p.ring + int64_t(n % p.slots) * p.slot_bytes);
+  T* nxt = reinterpret_cast<T*>(p.ring + int64_t((n + 1) % p.slots) * p.slot_bytes);
+  const T* prv = reinterpret_cast<const T*>(
+      p.ring + int64_t((n + p.slots - 1) % p.slots) * p.slot_bytes);
+
+  double as[NC][VEC], ac[NC][VEC], aq[NQA][VEC];
+  int qk[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) as[c][v] = ac[c][v] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NQA; ++c) aq[c][v] = 0.0;
+    qk[v] = NQ > 0 && p.im_exp != nullptr ? p.im_exp[seg * W + col + v] : 0;
+  }
+
+  for (int it = 0; it < iters; ++it) {
+    const int64_t r64 = iter_row0(it, rr.x, stride, ipc, WAVES * RPW) + wave * RPW;
+    if (r64 >= rr.y) break;
+    const int row0 = __builtin_amdgcn_readfirstlane(int(r64));
+    const int row = row0 + grp;
+    const bool live = row < rr.y;  // whole block-rows: range ends are multiples of 4
+    const int64_t roff = 2 * (int64_t(live ? row : row0) * W + col);
+    const int br = (live ? row : row0) >> 2;
+    const int k0 = live ? bptr[br] : 0;
+    const int len = live ? bptr[br + 1] - k0 : 0;
+    int lmax = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_xor(lmax, off);
+      lmax = o > lmax ? o : lmax;
+    }
+    CLane<T, VEC> xs = IO::load(cur + roff), xp, y, acc;
+    if (n > 0)
+      xp = IO::load_nt(prv + roff);
+    else
+      zero(xp);
+    zero(acc);
+    for (int t = 0; t < lmax; ++t) {
+      const bool act = t < len;
+      const int k = act ? k0 + t : 0;
+      CLane<T, VEC> xm;
+      T hr[4], hi[4];
+      if (act) {
+        const int j = bidx[k];
+        xm = IO::load(cur + 2 * (int64_t(4 * j + brow) * W + col));
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) load_value<T>(bval, int64_t(k) * 16 + brow * 4 + kk, hr[kk], hi[kk]);
+      } else {
+        zero(xm);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) hr[kk] = hi[kk] = T(0);
+      }
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        CLane<T, VEC> xk;
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          xk.re[v] = __shfl(xm.re[v], gbase + kk * LPR);
+          xk.im[v] = __shfl(xm.im[v], gbase + kk * LPR);
+        }
+        if (act) {
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            acc.re[v] = fma(hr[kk], xk.re[v], acc.re[v]);
+            acc.re[v] = fma(-hi[kk], xk.im[v], acc.re[v]);
+            acc.im[v] = fma(hr[kk], xk.im[v], acc.im[v]);
+            acc.im[v] = fma(hi[kk], xk.re[v], acc.im[v]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      y.re[v] = fma(acc.re[v] - bsh * xs.re[v], mul, -xp.re[v]);
+      y.im[v] = fma(acc.im[v] - bsh * xs.im[v], mul, -xp.im[v]);
+    }
+    if (live) IO::store_nt(nxt + roff, y);
+    if (!live) {
+      zero(xs);
+      zero(y);
+    }
+    CLane<T, VEC> ww;
+    zero(ww);
+    add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, xs, y, ww);
+  }
+  if (MODE == KPM_DOTS_NONE) return;
+  store_partials<VEC, LPR, WAVES, NC, NQ, NQA>(p, lb, wave, grp, col, tid, as, ac, aq);
 }
 
 // y[slot] = op x[slot] for an n_rows x W block (no recursion, no dots).
@@ -1360,6 +1507,45 @@ int launch_reduce(int mode, int width, const StepParams& p, hipStream_t stream) 
   if (mode == KPM_DOTS_DOUBLING) return launch_reduce_mode<KPM_DOTS_DOUBLING>(p, width, stream);
   if (mode == KPM_DOTS_PROJECT) return launch_reduce_mode<KPM_DOTS_PROJECT>(p, width, stream);
   return KPM_OK;
+}
+
+namespace {
+
+struct BsrLauncher {
+  int mode;
+  int iters;
+  int n;
+  const kpm_bsr* h;
+  const StepParams& p;
+  hipStream_t s;
+  template <typename T, int V, int LP>
+  int operator()() {
+    if constexpr (LP > 16) {
+      return fail(KPM_ERR_UNSUPPORTED, "block-CSR steps need at most 16 lanes per row");
+    } else {
+      dim3 grid(p.nblocks_launch), block(kWaves * 64);
+      if (mode == KPM_DOTS_DOUBLING)
+        hipLaunchKernelGGL((bsr_kernel<T, V, LP, kWaves, KPM_DOTS_DOUBLING>), grid, block, 0, s,
+                           p, h->indptr, h->indices, h->values, n, iters);
+      else
+        hipLaunchKernelGGL((bsr_kernel<T, V, LP, kWaves, KPM_DOTS_NONE>), grid, block, 0, s, p,
+                           h->indptr, h->indices, h->values, n, iters);
+      return check_hip(hipGetLastError(), "bsr_kernel launch");
+    }
+  }
+};
+
+}  // namespace
+
+int launch_bsr(const KernelShape& ks, int mode, int n, const kpm_bsr* h, const StepParams& p,
+               hipStream_t stream) {
+  if (ks.waves != kWaves) return fail(KPM_ERR_UNSUPPORTED, "block-CSR steps run 4-wave shapes");
+  if (mode != KPM_DOTS_DOUBLING && mode != KPM_DOTS_NONE)
+    return fail(KPM_ERR_ARG, "block-CSR steps take KPM_DOTS_DOUBLING or KPM_DOTS_NONE");
+  BsrLauncher l{mode, ks.iters, n, h, p, stream};
+  if (h->value_type == KPM_C64) return dispatch_shape<float>(ks.vec, ks.lpr, l);
+  if (h->value_type == KPM_C128) return dispatch_shape<double>(ks.vec, ks.lpr, l);
+  return fail(KPM_ERR_ARG, "unknown value type");
 }
 
 int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x, int64_t x_stride,

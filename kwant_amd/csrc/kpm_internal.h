@@ -68,6 +68,11 @@ struct StepParams {
 int launch_cheb(int value_type, const KernelShape& ks, int mode, bool only_dots, int n,
                 int row_nnz_hint, const StepParams& p, hipStream_t stream);
 int launch_reduce(int mode, int width, const StepParams& p, hipStream_t stream);
+// One block-CSR step (4 x 4 blocks): the kernel shape's lanes per row and
+// complex values per lane, its wavefront iterations (ks.iters), mode
+// KPM_DOTS_DOUBLING or KPM_DOTS_NONE.
+int launch_bsr(const KernelShape& ks, int mode, int n, const kpm_bsr* h, const StepParams& p,
+               hipStream_t stream);
 int launch_bond(int width, const kpm_csr* b, const void* bra, const void* ket, const void* v,
                 const void* w, int per_column, double* out, hipStream_t stream);
 int launch_spmm(int value_type, int width, const kpm_csr* op, const void* x,

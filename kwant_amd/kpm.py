@@ -22,8 +22,10 @@ Extra keyword arguments (all optional): ``precision`` ('complex128' |
 'complex64'), ``max_block_width`` (power of two <= 64), ``variant`` (kernel
 launch shape, see kwant_amd.autotune), ``reorder`` (True | False | 'auto' |
 a permutation: the device row order, see kwant_amd.ordering; the moments do
-not depend on it), ``gemm_precision`` for the Correlator ('fp64' | 'fp32' |
-'fp32x3' | 'sliced3' | 'sliced4').
+not depend on it), ``matrix_format`` ('csr' | 'bsr': the recursion reads the
+Hamiltonian as 4 x 4 site blocks -- for 4-orbital systems, operator=None; the
+moments are bitwise the CSR ones), ``gemm_precision`` for the Correlator
+('fp64' | 'fp32' | 'fp32x3' | 'sliced3' | 'sliced4').
 """
 import math
 import numbers
@@ -423,7 +425,8 @@ class SpectralDensity:
                  num_vectors=10, num_moments=None, energy_resolution=None,
                  vector_factory=None, bounds=None, eps=0.05, rng=None,
                  kernel=None, mean=True, accumulate_vectors=True, *,
-                 precision='complex128', max_block_width=64, variant=-1, reorder='auto'):
+                 precision='complex128', max_block_width=64, variant=-1, reorder='auto',
+                 matrix_format='csr'):
         if num_moments and energy_resolution:
             raise TypeError("either 'num_moments' or 'energy_resolution' "
                             "must be provided.")
@@ -443,10 +446,18 @@ class SpectralDensity:
         self._precision = precision
         self._max_width = max_block_width
         self._variant = variant
-        order = ordering.choose_order(hamiltonian, reorder)
+        if matrix_format not in ('csr', 'bsr'):
+            raise ValueError("matrix_format must be 'csr' or 'bsr'")
+        if matrix_format == 'bsr' and self.operator is not None:
+            raise ValueError("matrix_format='bsr' runs the doubling path (operator=None)")
+        order = None if matrix_format == 'bsr' else ordering.choose_order(hamiltonian, reorder)
         self.hamiltonian, (self._a, self._b) = _rescale(
             hamiltonian, self.eps, self._v0, bounds, precision, order)
         self._dmat = self.hamiltonian._dmat
+        if matrix_format == 'bsr':
+            # the recursion reads 4 x 4 site blocks; the CSR copy serves the
+            # rescaled matvec and the bounds
+            self._dmat = eng.DeviceBSR(hamiltonian, precision)
         self.bounds = (self._b - self._a, self._b + self._a)
 
         if energy_resolution:
