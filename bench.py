@@ -556,7 +556,7 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
     del omega, psi, rec_l, start, block, work
     torch.cuda.empty_cache()
     if full:
-        sig = {}
+        sig, breakdown = {}, {}
         for name, prec in (("fp64", "complex128"), ("sliced4", "complex128"),
                            ("fp32", "complex64")):
             torch.cuda.synchronize()
@@ -568,6 +568,30 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
                                                     for x in (-1.0, -0.5, 0.0, 0.5, 1.0)])
             torch.cuda.synchronize()
             sig[name] = (s_grid, time.perf_counter() - t1)
+            # where the time went: the two streamed expansions (2 M steps
+            # per block), the velocity SpMMs, the Omega x Psi GEMM chunks
+            tm = dict(c.timings)
+            widths = list(c.block_widths)
+            s_v = 16 if prec == "complex128" else 8
+            rec_bytes = sum(3 * w * n * s_v + h.nnz * (s_v + 4) + 4 * (n + 1) for w in widths)
+            rec_steps = 2 * (moments - 1)
+            breakdown[name] = {
+                "block_widths": widths, "total_s": sig[name][1],
+                "expansion_s": tm.get("expansion_s", 0.0),
+                "velocity_spmm_s": tm.get("operator_s", 0.0),
+                "gemm_s": tm.get("gemm_s", 0.0),
+                "host_and_integral_factor_s": sig[name][1] - sum(tm.values()),
+                # one recursion step over every block: algorithmic bytes / time
+                "recursion_step_GBs": rec_bytes * rec_steps / max(tm.get("expansion_s", 0), 1e-12)
+                / 1e9}
+            breakdown[name]["recursion_frac_of_hbm_peak"] = \
+                breakdown[name]["recursion_step_GBs"] / HBM_PEAK_GBS
+            log_fn("sigma_xy %s: %.1f s total, blocks %s: expansions %.2f s (%.1f%% of 8 TB/s), "
+                   "velocity SpMM %.2f s, GEMM %.2f s, host %.2f s"
+                   % (name, sig[name][1], widths, breakdown[name]["expansion_s"],
+                      100 * breakdown[name]["recursion_frac_of_hbm_peak"],
+                      breakdown[name]["velocity_spmm_s"], breakdown[name]["gemm_s"],
+                      breakdown[name]["host_and_integral_factor_s"]))
             del c
             torch.cuda.empty_cache()
         s64 = sig["fp64"][0]
@@ -578,6 +602,7 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
             out["sigma_xy"][name + "_max_rel_dev"] = d
         for name in sig:
             out["sigma_xy"][name + "_seconds"] = sig[name][1]
+        out["sigma_xy"]["breakdown"] = breakdown
         log_fn("sigma_xy (FP64) at mu = -1..1: %s; max rel dev sliced4 %.2e, FP32 %.2e"
                % (np.round(np.real(s64), 4), out["sigma_xy"]["sliced4_max_rel_dev"],
                   out["sigma_xy"]["fp32_max_rel_dev"]))

@@ -488,8 +488,13 @@ class BlockRecursion:
     a_n when ``store_all`` (Correlator expansions)."""
 
     def __init__(self, mat, ab, vectors, width, mode, w=None, store_slots=None,
-                 variant=-1, plan=None):
+                 variant=-1, plan=None, wrap=False):
+        """``wrap``: a ring of ``store_slots`` slots that the caller drains
+        as it goes -- a_k lives in slot k mod slots, and the steps may run
+        past the ring's end (the caller uses each a_k before slots steps
+        overwrite it)."""
         import torch
+        self.wrap = bool(wrap)
         dev = torch_device()
         n = mat.shape[0]
         self.mat, self.width, self.mode, self.n_rows = mat, width, mode, n
@@ -520,7 +525,7 @@ class BlockRecursion:
         import torch
         if n_steps <= 0:
             return np.zeros((0, self.width, 4))
-        if self.n + n_steps >= self.slots and self.slots > 2:
+        if self.n + n_steps >= self.slots and self.slots > 2 and not self.wrap:
             raise ValueError("ring too small for the requested moments")
         mode = self.mode if want_dots else DOTS_NONE
         dots = None

@@ -2,17 +2,20 @@
 
 Mirrors kwant/kpm.py:493-847.  For every block of random vectors r:
 
-* the "right" expansion T_n(H~)|r> (kwant/kpm.py:574) keeps all its
-  Chebyshev vectors in a GPU ring of num_moments slots;
-* Psi_n = op2 T_n(H~)|r> is one batched SpMM over those slots
-  (kwant/kpm.py:708-718);
+* the "right" expansion T_n(H~)|r> (kwant/kpm.py:574) runs through a GPU
+  ring of OMEGA_CHUNK slots, and each run of new vectors becomes
+  Psi_n = op2 T_n(H~)|r> by one batched SpMM (kwant/kpm.py:708-718): Psi
+  (num_moments slots) is the only block kept whole;
 * the "left" expansion T_m(H~) op1|r> (kwant/kpm.py:588-603) starts from
   op1|r> (SpMM) with the same (a, b) -- recomputed through the shrunken
-  bounds exactly as kwant/kpm.py:578-579 does;
-* mu_mn += sum_i conj(Omega_m[i]) Psi_n[i] is a GPU GEMM over
+  bounds exactly as kwant/kpm.py:578-579 does -- and streams through its own
+  ring: every run of OMEGA_CHUNK rows of Omega is multiplied into
+  mu[m0:m1, :] += sum_i conj(Omega_m[i]) Psi_n[i], a GPU GEMM over
   K = N x block width (kwant/kpm.py:685-695), FP64 by default
   (``gemm_precision='fp32'`` runs the complex64 path on the FP32 matrix cores
-  with FP64 accumulation across k-chunks).
+  with FP64 accumulation across k-chunks).  Memory per vector is
+  (num_moments + OMEGA_CHUNK) slots instead of 2 num_moments, so wider
+  blocks fit.
 
 The integral factor (kwant/kpm.py:720-760) is evaluated in its GEMM form,
 factor(x_k) = e+_k^T mu~ T_k + T_k^T mu~ conj(e+_k), and the final
@@ -274,6 +277,58 @@ class Correlator:
                 out[s] = dmat.rows_in(res)
             dst.copy_(torch.from_numpy(out).to(dst.device).to(dst.dtype))
 
+    # Chebyshev vectors per ring of a streamed expansion, and rows of Omega
+    # per GEMM: Psi stays resident (num_moments slots), each expansion runs
+    # through a ring of this many slots that is drained as it fills
+    OMEGA_CHUNK = 256
+
+    def _stream(self, ab, start, width, m_count, consume):
+        """Expand ``start`` (an (N, width) device block) to m_count Chebyshev
+        vectors through a wrap-around ring, handing each run of consecutive
+        vectors to consume(m0, m1, slots) -- slots: the (m1 - m0, N, width)
+        view of the ring holding T_m0 .. T_{m1-1} -- in order."""
+        chunk = max(3, min(m_count, self.OMEGA_CHUNK))
+        rec = eng.BlockRecursion(self._spectrum_R.dmat, ab, start, width, eng.DOTS_NONE,
+                                 store_slots=chunk, wrap=True)
+        consumed, produced = 0, 1
+        while consumed < m_count:
+            target = min(m_count, consumed + chunk)
+            if target > produced:
+                with self._timed('expansion_s'):
+                    rec.step(target - produced, want_dots=False)
+                produced = target
+            m0 = consumed
+            while m0 < target:
+                s0 = m0 % chunk
+                m1 = min(target, m0 + chunk - s0)
+                consume(m0, m1, rec.ring[s0:s0 + (m1 - m0)])
+                m0 = m1
+            consumed = target
+        del rec
+
+    def _timed(self, phase):
+        """A context that adds the GPU time of the enclosed launches to
+        self.timings[phase] (read after the next synchronisation)."""
+        import contextlib
+        import torch
+
+        @contextlib.contextmanager
+        def span():
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            yield
+            e1.record()
+            self._events.append((phase, e0, e1))
+        return span()
+
+    def _collect_timings(self):
+        import torch
+        torch.cuda.synchronize()
+        for phase, e0, e1 in self._events:
+            self.timings[phase] = self.timings.get(phase, 0.0) + e0.elapsed_time(e1) / 1e3
+        self._events = []
+
     def _add_vectors(self, count):
         import torch
         spec = self._spectrum_R
@@ -283,8 +338,12 @@ class Correlator:
         cdt, _ = eng.complex_dtype(dmat.precision)
         n, m_count = dmat.shape[0], self.num_moments
         r0 = self.num_vectors
+        if not hasattr(self, 'timings'):
+            self.timings, self._events = {}, []
         widths = [1] * count if not self.mean else eng.pow2_widths(
             count, self._width_cap(n, m_count, cdt))
+        self.block_widths = getattr(self, 'block_widths', []) + list(widths)
+        gp = _GEMM[self._gemm_precision]
         start = r0
         for width in widths:
             vecs = [factory[r] for r in range(start, start + width)]
@@ -292,31 +351,34 @@ class Correlator:
             host = np.stack([np.asarray(v) for v in vecs], axis=1).astype(np.complex128)
             _check_finite_vectors(host)
             block = torch.from_numpy(np.ascontiguousarray(dmat.rows_in(host))).to(dev).to(cdt)
-            # right expansion: all T_n(H~)|r>
-            rec_r = eng.BlockRecursion(dmat, (self._a, self._b), block, width,
-                                       eng.DOTS_NONE, store_slots=m_count)
-            if m_count > 1:
-                rec_r.step(m_count - 1, want_dots=False)
-            psi = torch.empty_like(rec_r.ring)
-            self._apply('op2', self._op2, rec_r.ring, psi, width, m_count)
-            del rec_r
-            # left expansion from op1|r>
+            # Psi_n = op2 T_n(H~)|r>, resident for the whole block
+            psi = torch.empty((m_count, n, width), dtype=cdt, device=dev)
+
+            def to_psi(m0, m1, slots):
+                with self._timed('operator_s'):
+                    self._apply('op2', self._op2, slots, psi[m0:m1], width, m1 - m0)
+
+            self._stream((self._a, self._b), block, width, m_count, to_psi)
+            # Omega_m = T_m(H~) op1|r>, streamed in chunks against Psi
             start_l = torch.empty_like(block)
-            self._apply('op1', self._op1, block.unsqueeze(0), start_l.unsqueeze(0), width, 1)
-            rec_l = eng.BlockRecursion(dmat, (self._aL, self._bL), start_l, width,
-                                       eng.DOTS_NONE, store_slots=m_count)
-            if m_count > 1:
-                rec_l.step(m_count - 1, want_dots=False)
+            with self._timed('operator_s'):
+                self._apply('op1', self._op1, block.unsqueeze(0), start_l.unsqueeze(0), width, 1)
             mu = torch.zeros((m_count, m_count), dtype=torch.complex128, device=dev)
             k = n * width
-            gp = _GEMM[self._gemm_precision]
-            if isinstance(gp, str):
-                self._gemm_work = eng.corr_gemm_sliced(
-                    rec_l.ring.reshape(m_count, k), psi.reshape(m_count, k), mu,
-                    slices=int(gp[-1]), work=getattr(self, '_gemm_work', None))
-            else:
-                eng.corr_gemm(rec_l.ring.reshape(m_count, k), psi.reshape(m_count, k), mu, gp)
-            del rec_l, psi
+            psi_k = psi.reshape(m_count, k)
+
+            def gemm(m0, m1, slots):
+                a = slots.reshape(m1 - m0, k)
+                with self._timed('gemm_s'):
+                    if isinstance(gp, str):
+                        self._gemm_work = eng.corr_gemm_sliced(
+                            a, psi_k, mu[m0:m1], slices=int(gp[-1]),
+                            work=getattr(self, '_gemm_work', None))
+                    else:
+                        eng.corr_gemm(a, psi_k, mu[m0:m1], gp)
+
+            self._stream((self._aL, self._bL), start_l, width, m_count, gemm)
+            del psi, psi_k
             if self.mean:
                 self._mu_sum = mu if self._mu_sum is None else self._mu_sum + mu
             else:
@@ -326,18 +388,22 @@ class Correlator:
             self.moments_matrix = self._mu_sum.cpu().numpy() / self.num_vectors
         else:
             self.moments_matrix = np.array(self._mu_list)
+        self._collect_timings()
 
     def _width_cap(self, n, m_count, cdt):
-        """Largest power-of-two block width (<= max_block_width) whose two
-        live rings of m_count slots (T_n|r> or Omega, and Psi) take at most
-        half of the free device memory."""
+        """Largest power-of-two block width (<= max_block_width) whose live
+        vectors -- Psi (m_count slots) and one streaming ring of OMEGA_CHUNK
+        slots, plus two single blocks -- take at most 3/4 of the free device
+        memory."""
         import torch
         free, _ = torch.cuda.mem_get_info()
-        per_width = 2 * m_count * n * torch.empty((), dtype=cdt).element_size()
+        slots = m_count + min(m_count, self.OMEGA_CHUNK) + 2
+        per_width = slots * n * torch.empty((), dtype=cdt).element_size()
         width = self._max_width
-        while width > 1 and width * per_width > free // 2:
+        budget = free * 3 // 4
+        while width > 1 and width * per_width > budget:
             width //= 2
-        if per_width > free // 2:
+        if per_width > budget:
             raise MemoryError("one vector's %d Chebyshev vectors of %d sites do not fit in "
                               "device memory" % (m_count, n))
         return width
