@@ -149,11 +149,14 @@ def cpu_baseline_pool(h, a, b, steps, width, max_procs=16):
         per = pool.map(_pool_worker, [(100 + i, steps) for i in range(procs)])
     _POOL_INPUT = None
     t_vs = sum(per) / len(per)
+    affinity = len(os.sched_getaffinity(0))
     return {"value": procs / (t_vs * width), "unit": "steps/s", "cores": procs,
             "kind": "port", "cpu_model": model,
-            "sample": "%d processes x 1 vector x %d steps of the oracle loop, %.1f ms per "
-                      "vector-step each, aggregate scaled to %d-vector block steps"
-                      % (procs, steps, 1e3 * t_vs, width)}
+            "affinity_cores": affinity, "process_cap": max_procs,
+            "sample": "%d processes (the %d cores this process may run on, capped at %d) x "
+                      "1 vector x %d steps of the oracle loop, %.1f ms per vector-step each, "
+                      "aggregate scaled to %d-vector block steps"
+                      % (procs, affinity, max_procs, steps, 1e3 * t_vs, width)}
 
 
 def small_square_bench(log_fn):
@@ -173,28 +176,78 @@ def small_square_bench(log_fn):
     t0 = time.perf_counter()
     kpm.SpectralDensity(h, **kw)
     t_gpu = time.perf_counter() - t0
+    from kwant_amd.surrogate import dos_l1
     bounds = (-4.6, 4.6)
     g = kpm.SpectralDensity(h, bounds=bounds, **kw)
     o = ko.SpectralDensity(h, bounds=bounds, **kw)
     e_g, rho_g = g()
     e_o, rho_o = o()
     err = float(np.max(np.abs(rho_g - rho_o)) / np.max(np.abs(rho_o)))
+    # DOS L1 (relative to int rho) of each GPU precision against the FP64 oracle
+    ref = np.real(o._moments())
+    l1 = {}
+    for prec in ("complex128", "complex64"):
+        m = np.real(kpm.SpectralDensity(h, bounds=bounds, precision=prec, **kw)._moments())
+        l1[prec] = float(dos_l1(m[None] / ref[0], ref[None] / ref[0])[0])
     log_fn("small square case (100x100, 10 vectors, 100 moments): GPU %.3f s, CPU oracle %.3f s "
-           "(wall, bounds by Lanczos / ARPACK); densities with explicit bounds agree to %.1e"
-           % (t_gpu, t_cpu, err))
+           "(wall, bounds by Lanczos / ARPACK); densities with explicit bounds agree to %.1e; "
+           "DOS L1 vs the FP64 oracle: complex128 %.1e, complex64 %.1e"
+           % (t_gpu, t_cpu, err, l1["complex128"], l1["complex64"]))
     return {"workload": "square 100x100, U[-0.5,0.5], 10 vectors, 100 moments, Jackson, "
                         "complex128", "gpu_seconds": t_gpu, "cpu_oracle_seconds": t_cpu,
-            "densities_max_rel_diff_explicit_bounds": err}
+            "densities_max_rel_diff_explicit_bounds": err,
+            "dos_l1_vs_fp64_oracle": l1}
 
 
-def surrogate_bench(epochs, tol, points, check_points, check_sets, log_fn):
+def _stratified_checks(count, seed=11):
+    """Parameter points for judging the surrogate: half uniform over the
+    sweep box, a quarter in the clean limit W < 0.5 and a quarter at strong
+    flux phi > 0.015 (where its fallback concentrates)."""
+    import numpy as np
+    from kwant_amd.surrogate import PARAM_LO, PARAM_HI
+    rng = np.random.RandomState(seed)
+    uni = PARAM_LO + (PARAM_HI - PARAM_LO) * rng.random_sample((count // 2, 3))
+    lo_w = PARAM_LO + (PARAM_HI - PARAM_LO) * rng.random_sample((count // 4, 3))
+    lo_w[:, 0] *= 0.5 / PARAM_HI[0]
+    hi_f = PARAM_LO + (PARAM_HI - PARAM_LO) * rng.random_sample((count - count // 2 - count // 4,
+                                                                 3))
+    hi_f[:, 1] = 0.015 + (PARAM_HI[1] - 0.015) * rng.random_sample(len(hi_f))
+    return np.concatenate([uni, lo_w, hi_f]), {"uniform": slice(0, count // 2),
+                                               "W<0.5": slice(count // 2, count // 2 + count // 4),
+                                               "phi>0.015": slice(count // 2 + count // 4, count)}
+
+
+def _converged(wide, params, sets):
+    """Damped moments from sets x 32 random vectors (seeded sets, the
+    reference's v0 offset skipped), the judge of the surrogate's DOS error."""
+    import numpy as np
+    import torch
+    out = np.zeros((len(params), wide.num_moments))
+    for k in range(sets):
+        rng = np.random.RandomState(1000 + k)
+        rng.random_sample(wide.n)
+        vecs = np.stack([np.exp(2j * np.pi * rng.random_sample(wide.n))
+                         for _ in range(wide.num_vectors)], 1)
+        wide._vectors = torch.from_numpy(vecs).to(wide._vectors.device).to(wide.cdt)
+        out += wide.damped_moments(params) / sets
+    return out
+
+
+def surrogate_bench(epochs, tol, points, check_count, check_sets, log_fn, design="shift"):
     """The moment surrogate on the square-lattice family (512 x 512 sites,
-    W in [0, 4], flux in [0, 0.02], mu in [-1, 1], 512 moments, 32 vectors):
-    train on 4096 + 1024 GPU-KPM runs (the extra ones in the clean limit
-    W < 0.5, whose sharp van Hove and Landau-level features the network
-    needs most data for), sweep ``points`` random parameter points with the
-    spread gate and KPM fallback, and check the result against converged
-    KPM (``check_sets`` x 32 vectors) on ``check_points`` of them."""
+    W in [0, 4], flux in [0, 0.02], mu in [-1, 1], 512 moments, 32 vectors).
+
+    design 'shift': the network maps (W, flux) -> 4 x 256 GELU -> 512
+    tight-window moments and mu enters by the exact shift; trained on 4096 +
+    1024 GPU-KPM runs (the extra ones in the clean limit W < 0.5).
+    design 'literal': the configuration's network (W, flux, mu) -> 4 x 256
+    GELU -> 512 wide-window moments, trained on 4096 GPU-KPM runs of
+    H(W, flux, mu).
+
+    Then: a sweep of ``points`` random points with the spread gate and KPM
+    fallback, timed against plain GPU KPM, and its DOS L1 error against
+    converged KPM (``check_sets`` x 32 vectors) on ``check_count`` stratified
+    points (half uniform, a quarter at W < 0.5, a quarter at flux > 0.015)."""
     import numpy as np
     import torch
     from scipy.stats import qmc
@@ -203,61 +256,76 @@ def surrogate_bench(epochs, tol, points, check_points, check_sets, log_fn):
     out = {}
     wide = SquareFamily(l=512, num_moments=512, num_vectors=32, batch=8, bounds=WIDE_BOUNDS)
     tight = SquareFamily(l=512, num_moments=512, num_vectors=32, batch=8, bounds=TIGHT_BOUNDS)
-    sur = MomentSurrogate(wide, tight, members=5, seed=0, tol=tol)
-    train = qmc.scale(qmc.Sobol(2, scramble=True, seed=8).random(4096), PARAM_LO[:2],
-                      PARAM_HI[:2])
-    clean = qmc.scale(qmc.Sobol(2, scramble=True, seed=9).random(1024), PARAM_LO[:2],
-                      np.array([0.5, PARAM_HI[1]]))
+    literal = design == "literal"
+    layers = (3 if literal else 2, 256, 256, 256, 256, 512)
+    sur = MomentSurrogate(wide, tight, members=5, layers=layers, seed=0, tol=tol)
+    if literal:
+        train = qmc.scale(qmc.Sobol(3, scramble=True, seed=8).random(4096), PARAM_LO, PARAM_HI)
+        batches = [(train[s:s + 512], 100 + s) for s in range(0, 4096, 512)]
+    else:
+        train = qmc.scale(qmc.Sobol(2, scramble=True, seed=8).random(4096), PARAM_LO[:2],
+                          PARAM_HI[:2])
+        clean = qmc.scale(qmc.Sobol(2, scramble=True, seed=9).random(1024), PARAM_LO[:2],
+                          np.array([0.5, PARAM_HI[1]]))
+        batches = [(train[s:s + 512], 100 + s) for s in range(0, 4096, 512)]
+        batches += [(clean[s:s + 512], 9000 + s) for s in range(0, 1024, 512)]
+        train = np.concatenate([train, clean])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     parts = []
-    for s in range(0, 4096, 512):
-        parts.append(sur.training_targets(train[s:s + 512], fresh_seed=100 + s))
-        log_fn("surrogate: training runs %d..%d done" % (s, s + 511))
-    for s in range(0, 1024, 512):
-        parts.append(sur.training_targets(clean[s:s + 512], fresh_seed=9000 + s))
-    train = np.concatenate([train, clean])
+    for prm, seed in batches:
+        parts.append(sur.training_targets(prm, fresh_seed=seed))
+        log_fn("surrogate (%s): training runs %d done" % (design, sum(len(p) for p in parts)))
     targets = np.concatenate(parts)
     t_data = time.perf_counter() - t0
-    log_fn("surrogate: %d training runs in %.1f s" % (len(train), t_data))
+    log_fn("surrogate (%s): %d training runs in %.1f s" % (design, len(train), t_data))
     t0 = time.perf_counter()
     sur.fit(train, targets, epochs=epochs, batch=64, lr=3e-3, ema_decay=0.999)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
-    log_fn("surrogate: %d epochs x %d Adam steps, 5 members (%d kept), in %.1f s"
-           % (epochs, len(train) // 64, sur.active, t_fit))
+    log_fn("surrogate (%s): %d epochs x %d Adam steps, 5 members (%d kept), in %.1f s"
+           % (design, epochs, len(train) // 64, sur.active, t_fit))
     sweep = PARAM_LO + (PARAM_HI - PARAM_LO) * np.random.RandomState(7).random_sample((points, 3))
     sur.predict(sweep[:64])
     moments, fall, t_sweep = sur.sweep(sweep)
-    log_fn("surrogate: sweep of %d points in %.3f s (%.2f%% KPM fallback)"
-           % (points, t_sweep, 100 * fall.mean()))
+    log_fn("surrogate (%s): sweep of %d points in %.3f s (%.2f%% KPM fallback)"
+           % (design, points, t_sweep, 100 * fall.mean()))
     _, spread = sur.predict(sweep)
     spread = spread.cpu().numpy()
     sample = sweep[:64]
     wide.damped_moments(sample[:8])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    kpm = wide.damped_moments(sample)
+    wide.damped_moments(sample)
     t_point = (time.perf_counter() - t0) / len(sample)
     t_plain = t_point * points
     log_fn("plain KPM: %.2f ms per point -> %.1f s for the sweep" % (1e3 * t_point, t_plain))
-    converged = np.zeros((check_points, 512))
-    for k in range(check_sets):
-        rng = np.random.RandomState(1000 + k)
-        rng.random_sample(wide.n)
-        vecs = np.stack([np.exp(2j * np.pi * rng.random_sample(wide.n)) for _ in range(32)], 1)
-        wide._vectors = torch.from_numpy(vecs).to(wide._vectors.device).to(wide.cdt)
-        converged += wide.damped_moments(sweep[:check_points]) / check_sets
-    err = dos_l1(moments[:check_points], converged)
-    err_kpm = dos_l1(kpm[:check_points], converged)
-    log_fn("DOS L1 vs converged KPM (%d vectors): sweep median %.2e p90 %.2e max %.2e; plain "
-           "KPM(32) median %.2e" % (32 * check_sets, np.median(err), np.percentile(err, 90),
-                                   err.max(), np.median(err_kpm)))
+    # accuracy on the stratified points, against converged KPM
+    checks, strata = _stratified_checks(check_count)
+    mom_c, fall_c, _ = sur.sweep(checks)
+    kpm_c = wide.damped_moments(checks)          # plain KPM, the seeded 32 vectors
+    converged = _converged(wide, checks, check_sets)
+    err = dos_l1(mom_c, converged)
+    err_kpm = dos_l1(kpm_c, converged)
+    per = {}
+    for name, sl in strata.items():
+        per[name] = {"points": int(sl.stop - sl.start),
+                     "fallback_fraction": float(fall_c[sl].mean()),
+                     "dos_l1_median": float(np.median(err[sl])),
+                     "dos_l1_max": float(err[sl].max()),
+                     "plain_kpm32_dos_l1_median": float(np.median(err_kpm[sl]))}
+        log_fn("DOS L1 vs converged KPM (%d vectors), %s: sweep median %.2e max %.2e "
+               "(fallback %.1f%%); plain KPM(32) median %.2e"
+               % (32 * check_sets, name, per[name]["dos_l1_median"], per[name]["dos_l1_max"],
+                  100 * per[name]["fallback_fraction"], per[name]["plain_kpm32_dos_l1_median"]))
+    net = ("(W, flux, mu) -> 4x256 GELU -> 512 wide-window moments, trained on 4096 runs"
+           if literal else
+           "(W, flux) -> 4x256 GELU -> 512 + exact shift in mu, trained on 4096 + 1024 "
+           "(W < 0.5) runs")
     out.update({
         "workload": "square 512x512, W in [0,4], flux in [0,0.02], mu in [-1,1], 512 moments, "
-                    "32 vectors; MLP (W, flux) -> 4x256 GELU -> 512 + exact shift in mu; "
-                    "5-member ensemble; 4096 + 1024 (W < 0.5) training runs; %d sweep "
-                    "points" % points,
+                    "32 vectors; MLP %s; 5-member ensemble; %d sweep points" % (net, points),
+        "design": design,
         "speedup_vs_plain_kpm": t_plain / t_sweep,
         "sweep_s": t_sweep, "plain_kpm_sweep_s": t_plain, "plain_kpm_ms_per_point": 1e3 * t_point,
         "fallback_fraction": float(fall.mean()), "spread_tol": tol,
@@ -265,8 +333,9 @@ def surrogate_bench(epochs, tol, points, check_points, check_sets, log_fn):
         "dos_l1_vs_converged_p90": float(np.percentile(err, 90)),
         "dos_l1_vs_converged_max": float(err.max()),
         "plain_kpm32_dos_l1_vs_converged_median": float(np.median(err_kpm)),
-        "converged_reference": "KPM with %d random vectors on %d sweep points" % (32 * check_sets,
-                                                                   check_points),
+        "dos_l1_by_stratum": per,
+        "converged_reference": "KPM with %d random vectors on %d stratified points"
+                               % (32 * check_sets, check_count),
         "training_data_s": t_data, "training_s": t_fit, "epochs": epochs,
         "members_kept": int(sur.active),
         "projected_other_tolerances": {
@@ -277,6 +346,47 @@ def surrogate_bench(epochs, tol, points, check_points, check_sets, log_fn):
             for t in (1.5e-3, 2e-3)},
     })
     return out
+
+
+def surrogate_train_profile(epochs, log_fn):
+    """The surrogate's training and prediction kernels alone, on synthetic
+    targets (smooth random moment curves of the 512-moment shape): the
+    workload of the committed MLP kernel profile.  Kernel timings do not
+    depend on the target values."""
+    import numpy as np
+    import torch
+    from kwant_amd.surrogate import MomentSurrogate, PARAM_LO, PARAM_HI, jackson_factors
+
+    class Family:
+        num_moments = 512
+        a, b = 7.0 / 0.975, 0.0
+        g = jackson_factors(512)
+
+    tight = Family()
+    tight.a = 6.0 / 0.975
+    sur = MomentSurrogate(Family(), tight, members=5, seed=0, tol=1e-3)
+    rng = np.random.RandomState(0)
+    params = PARAM_LO + (PARAM_HI - PARAM_LO) * rng.random_sample((4096, 3))
+    n = np.arange(512)
+    targets = (np.cos(np.outer(params[:, 0], n) * 0.01) * np.exp(-n / 200.0) * tight.g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sur.fit(params, targets.astype(np.float32), epochs=epochs, batch=64, lr=3e-3)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    sur.predict(params)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sur.predict(params)
+    torch.cuda.synchronize()
+    t_pred = time.perf_counter() - t0
+    iters = epochs * (4096 // 64)
+    log_fn("surrogate training: %d epochs x 64 steps, 5 members: %.3f s (%.3f ms per "
+           "iteration); 4096-point prediction %.2f ms" % (epochs, t_fit, 1e3 * t_fit / iters,
+                                                          1e3 * t_pred))
+    return {"workload": "5-member MLP 2 -> 4x256 GELU -> 512, batch 64, synthetic targets",
+            "epochs": epochs, "fit_s": t_fit, "ms_per_iteration": 1e3 * t_fit / iters,
+            "predict_4096_ms": 1e3 * t_pred}
 
 
 def density_bench(mat, host_h, ab, block, width, variant, dos_ms, steps, log_fn):
@@ -643,7 +753,8 @@ def main():
                     help="skip the batched disorder-ensemble measurement")
     ap.add_argument("--no-correlator", action="store_true",
                     help="skip the Kubo-Bastin Gamma_nm measurement")
-    ap.add_argument("--only", choices=["correlator", "correlator-full", "ensemble", "surrogate"],
+    ap.add_argument("--only", choices=["correlator", "correlator-full", "ensemble", "surrogate",
+                                       "surrogate-literal", "surrogate-train"],
                     help="run only one of the secondary measurements (for profiling)")
     ap.add_argument("--no-density", action="store_true",
                     help="skip the site-resolved density step measurement")
@@ -665,7 +776,11 @@ def main():
               "ensemble": lambda: ensemble_bench(256, 512, 16, log, args.ens_variant,
                                                  not args.ens_eager),
               "surrogate": lambda: surrogate_bench(args.sur_epochs, args.sur_tol,
-                                                   args.sur_points, 32, 32, log)}[args.only]
+                                                   args.sur_points, 256, 16, log),
+              "surrogate-literal": lambda: surrogate_bench(args.sur_epochs, args.sur_tol,
+                                                           args.sur_points, 256, 16, log,
+                                                           design="literal"),
+              "surrogate-train": lambda: surrogate_train_profile(22, log)}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return
 
@@ -746,11 +861,20 @@ def main():
         mom128 = moments_from_dots(dots128)
         del r128, m128
         err = float(np.max(np.abs(mom64 - mom128)) / n)
+        # DOS L1 of the complex64 recursion against the complex128 one (which
+        # matches the FP64 oracle to 1e-11 in the tests): mean over vectors,
+        # Jackson damped, relative to int rho
+        from kwant_amd.kpm import jackson_kernel
+        from kwant_amd.surrogate import dos_l1
+        d64 = jackson_kernel(np.real(mom64).mean(0)) / n
+        d128 = jackson_kernel(np.real(mom128).mean(0)) / n
+        l1 = float(dos_l1(d64[None], d128[None])[0])
         check = {"moments_c64_vs_c128_max_err_over_mu0": err,
+                 "dos_l1_c64_vs_c128": l1,
                  "num_moments": 2 * full,
                  "c128_steps_per_s": full / t128}
-        log("c64 vs c128 moments: max |d mu| / mu0 = %.3e; c128 %.0f steps/s"
-            % (err, full / t128))
+        log("c64 vs c128 moments: max |d mu| / mu0 = %.3e, DOS L1 %.1e; c128 %.0f steps/s"
+            % (err, l1, full / t128))
 
     block64 = block.to(torch.complex64)
     del block
@@ -797,7 +921,7 @@ def main():
         assembly = assembly_bench(1000, 60, log)
     surrogate = None
     if not args.no_surrogate:
-        surrogate = surrogate_bench(args.sur_epochs, args.sur_tol, args.sur_points, 32, 32, log)
+        surrogate = surrogate_bench(args.sur_epochs, args.sur_tol, args.sur_points, 256, 16, log)
 
     traffic = read_traffic(args.variant)
     stream = stream_probe(log)

@@ -78,7 +78,11 @@ def save(sd, path):
         rng = None
     if sd._dmat.order is not None:
         arrays["order"] = np.asarray(sd._dmat.order, dtype=np.int64)
+    from . import kpm
+    kernel = {kpm.jackson_kernel: "jackson", kpm.lorentz_kernel: "lorentz"}.get(sd.kernel,
+                                                                              "custom")
     meta = {
+        "kernel": kernel,
         "format": FORMAT, "version": VERSION,
         "n_rows": int(sd._dmat.shape[0]), "a": float(sd._a), "b": float(sd._b),
         "eps": float(sd.eps), "num_moments": int(sd.num_moments), "mean": bool(sd.mean),
@@ -115,6 +119,17 @@ def load(path, hamiltonian, params=None, operator=None, kernel=None, vector_fact
     from . import _engine as eng
     from . import kpm
     meta = read_meta(path)
+    abi = int(eng.load_library().kpm_abi_version())
+    if meta.get("engine_abi") != abi:
+        raise ValueError("%s was saved by engine ABI %r; this library is ABI %d (the saved "
+                         "recursion state is not portable across engine versions)"
+                         % (path, meta.get("engine_abi"), abi))
+    saved_kernel = meta.get("kernel", "jackson")
+    if kernel is None:
+        if saved_kernel == "custom":
+            raise ValueError("%s was made with a custom kernel: pass it again as kernel="
+                             % path)
+        kernel = {"jackson": kpm.jackson_kernel, "lorentz": kpm.lorentz_kernel}[saved_kernel]
     with np.load(path, allow_pickle=False) as z:
         arrays = {k: z[k] for k in z.files}
     h = kpm._as_csr(hamiltonian, params)

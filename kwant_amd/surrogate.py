@@ -331,8 +331,14 @@ class MomentSurrogate:
         import torch
         m = family.num_moments
         layers = tuple(layers)
-        if layers[0] != 2 or layers[-1] != m or train_family.num_moments != m:
-            raise ValueError("layers must map (W, phi) to the family's moments")
+        # layers[0] == 2: (W, phi) -> tight-window moments, mu by the exact
+        # shift; layers[0] == 3: the literal network (W, phi, mu) -> the
+        # wide-window moments, trained on H(W, phi, mu) itself
+        self.literal = layers[0] == 3
+        if layers[0] not in (2, 3) or layers[-1] != m or train_family.num_moments != m:
+            raise ValueError("layers must map (W, phi) or (W, phi, mu) to the family's moments")
+        if self.literal:
+            train_family = family
         self.family, self.train_family = family, train_family
         self.tol = tol
         self.alpha = train_family.a / family.a
@@ -351,23 +357,37 @@ class MomentSurrogate:
         self.history = []
 
     def _inputs(self, params):
-        """Normalised (W, phi) of parameter rows (P, 2) or (P, 3)."""
+        """Normalised (W, phi) -- or (W, phi, mu) for the literal network --
+        of parameter rows (P, 2) or (P, 3)."""
         import torch
-        p = np.atleast_2d(np.asarray(params, dtype=np.float64))[:, :2]
-        x = normalize_params(p, PARAM_LO[:2], PARAM_HI[:2]).astype(np.float32)
+        d = 3 if self.literal else 2
+        p = np.atleast_2d(np.asarray(params, dtype=np.float64))[:, :d]
+        x = normalize_params(p, PARAM_LO[:d], PARAM_HI[:d]).astype(np.float32)
         return torch.from_numpy(x).to(self.mlp.params.device)
 
     def training_targets(self, params, fresh_seed=1):
         """Damped tight-window moments of H(W, phi, 0) for params (P, >= 2)
-        (any shift column is ignored), from the GPU recursion with fresh
-        random vectors per point."""
+        (any shift column is ignored) -- for the literal network the
+        wide-window moments of H(W, phi, mu) -- from the GPU recursion with
+        fresh random vectors per point."""
         p = np.asarray(params, dtype=np.float64)
+        if self.literal:
+            return self.family.damped_moments(p[:, :3], fresh_seed=fresh_seed)
         p = np.concatenate([p[:, :2], np.zeros((len(p), 1))], axis=1)
         return self.train_family.damped_moments(p, fresh_seed=fresh_seed)
 
+    def in_range(self, params):
+        """Points the surrogate answers: (W, phi) inside the training box and,
+        for the shift design, |mu| within the exact shift's range."""
+        p = np.atleast_2d(np.asarray(params, dtype=np.float64)).reshape(-1, 3)
+        ok = np.all((p >= PARAM_LO - 1e-12) & (p <= PARAM_HI + 1e-12), axis=1)
+        if not self.literal:
+            ok &= np.abs(p[:, 2]) <= self.max_shift + 1e-12
+        return ok
+
     def fit(self, params, targets, epochs=400, batch=64, lr=3e-3, lr_min=3e-6, seed=0,
             log=None, graph=True, b1=0.9, b2=0.999, ema_decay=None, ema_start=0.5,
-            warmup=0.02, health=10.0, collapse=0.5):
+            warmup=0.1, health=10.0, collapse=0.5):
         """Train every member on (params, targets) with Adam and a cosine
         learning-rate schedule; members draw their own mini-batch order.
 
@@ -489,7 +509,7 @@ class MomentSurrogate:
         members=True also returns the per-member moments (Z, P, M)."""
         import torch
         p = np.asarray(params, dtype=np.float64).reshape(-1, 3)
-        if np.any(np.abs(p[:, 2]) > self.max_shift + 1e-12):
+        if not self.literal and np.any(np.abs(p[:, 2]) > self.max_shift + 1e-12):
             raise ValueError("shift outside the exact range |mu| <= %.4f" % self.max_shift)
         dev = self.mlp.params.device
         x = self._inputs(p)
@@ -505,9 +525,12 @@ class MomentSurrogate:
         for s in range(0, count, chunk):
             e = min(count, s + chunk)
             y = self.mlp.forward(x[None, s:e])
-            sh = shifted[:, :e - s]
-            eng.moment_shift(y, self.alpha, beta[s:e], sh, in_scale=self._in_scale,
-                             out_scale=self._out_scale)
+            if self.literal:
+                sh = y                       # already the wide-window moments
+            else:
+                sh = shifted[:, :e - s]
+                eng.moment_shift(y, self.alpha, beta[s:e], sh, in_scale=self._in_scale,
+                                 out_scale=self._out_scale)
             if per is not None:
                 per[:, s:e].copy_(sh)
             if self.active == 0:
@@ -531,14 +554,19 @@ class MomentSurrogate:
     def sweep(self, params):
         """Damped moments for every parameter point: the ensemble mean where
         the spread is within ``tol``, plain KPM (``family``) elsewhere.
+        Points outside the surrogate's range (``in_range``) fall back too.
         Returns (moments (P, M) FP64 host array, fallback mask, seconds)."""
         import torch
         params = np.asarray(params, dtype=np.float64).reshape(-1, 3)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        mean, spread = self.predict(params)
-        out = mean.double().cpu().numpy()
-        fall = spread.cpu().numpy() > self.tol
+        inside = self.in_range(params)
+        out = np.zeros((len(params), self.family.num_moments))
+        fall = ~inside
+        if inside.any():
+            mean, spread = self.predict(params[inside])
+            out[inside] = mean.double().cpu().numpy()
+            fall[inside] = spread.cpu().numpy() > self.tol
         if fall.any():
             out[fall] = self.family.damped_moments(params[fall])
         torch.cuda.synchronize()

@@ -283,7 +283,18 @@ def test_surrogate_gain_over_plain_kpm():
     print("median DOS L1 vs exact: untrained %.3e trained %.3e plain KPM(R=%d) %.3e"
           % (l1_untrained, l1_trained, r, l1_kpm))
     assert l1_trained < 0.2 * l1_untrained
-    assert l1_trained < 3 * l1_kpm
+    # the gated sweep (the ensemble where its spread is within tol, plain KPM
+    # elsewhere) is no worse than plain KPM with the same vector count
+    gated, fall_test, _ = sur.sweep(test)
+    l1_gated = np.median(dos_l1(gated, exact))
+    print("gated sweep on the test points: median DOS L1 %.3e (%.0f%% fallback)"
+          % (l1_gated, 100 * fall_test.mean()))
+    assert l1_gated <= l1_kpm
+    # points outside the surrogate's range fall back to plain KPM
+    odd = np.array([[5.0, 0.01, 0.0], [1.0, 0.03, 0.0], [1.0, 0.01, 0.99 + sur.max_shift]])
+    mom_odd, fall_odd, _ = sur.sweep(odd)
+    assert fall_odd.all()
+    assert np.allclose(mom_odd, wide.damped_moments(odd))
     sweep = PARAM_LO + (PARAM_HI - PARAM_LO) * rng.random_sample((1024, 3))
     wide.damped_moments(sweep[:8])
     torch.cuda.synchronize()
@@ -295,6 +306,30 @@ def test_surrogate_gain_over_plain_kpm():
           % (t_kpm, t_sur, 100 * fall.mean()))
     assert mom.shape == (1024, m)
     assert t_sur < t_kpm
+
+
+def test_literal_three_input_network_learns():
+    """The configuration's literal network (W, flux, mu) -> moments, trained
+    on moments of H(W, flux, mu) in the wide window (no shift layer): its
+    DOS error against plain KPM falls well below the untrained one's, and
+    its predictions have the family's shape."""
+    from kwant_amd.surrogate import SquareFamily, MomentSurrogate, dos_l1, PARAM_LO, PARAM_HI
+    m, r, l = 48, 8, 16
+    wide = SquareFamily(l=l, num_moments=m, num_vectors=r, bounds=(-7.0, 7.0), batch=8,
+                        variant=0)
+    sur = MomentSurrogate(wide, wide, members=3, layers=(3, 48, 48, m), seed=2, tol=0.05)
+    assert sur.literal
+    rng = np.random.RandomState(3)
+    train = PARAM_LO + (PARAM_HI - PARAM_LO) * rng.random_sample((256, 3))
+    targets = sur.training_targets(train, fresh_seed=5)
+    test = PARAM_LO + (PARAM_HI - PARAM_LO) * rng.random_sample((16, 3))
+    ref = wide.damped_moments(test)
+    before = np.median(dos_l1(sur.predict(test)[0].double().cpu().numpy(), ref))
+    sur.fit(train, targets, epochs=80, batch=32, lr=3e-3)
+    mean, spread = sur.predict(test)
+    assert tuple(mean.shape) == (16, m) and tuple(spread.shape) == (16,)
+    after = np.median(dos_l1(mean.double().cpu().numpy(), ref))
+    assert after < 0.3 * before, (before, after)
 
 
 def test_fit_graph_replay_matches_eager():

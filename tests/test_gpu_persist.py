@@ -118,3 +118,28 @@ def test_correlator_round_trip(tmp_path):
         back.add_moments(10)
     with pytest.raises(ValueError):
         back.add_vectors(1)
+
+
+def test_round_trip_keeps_the_kernel(tmp_path):
+    """jackson and lorentz kernels come back by name; a custom kernel must be
+    passed again."""
+    from kwant_amd import kpm
+    from oracle import brute
+    h = brute.gaussian_unitary(16, 3)
+    kw = dict(num_moments=40, num_vectors=3, bounds=(-12.0, 12.0), rng=1)
+    sd = kpm.SpectralDensity(h, kernel=kpm.lorentz_kernel, **kw)
+    p = tmp_path / "lorentz.npz"
+    sd.save(p)
+    back = kpm.SpectralDensity.load(p, h)
+    assert back.kernel is kpm.lorentz_kernel
+    assert np.array_equal(back.densities, sd.densities)
+
+    def halved(m):
+        return kpm.jackson_kernel(m) / 2
+    sd = kpm.SpectralDensity(h, kernel=halved, **kw)
+    q = tmp_path / "custom.npz"
+    sd.save(q)
+    with pytest.raises(ValueError, match="custom kernel"):
+        kpm.SpectralDensity.load(q, h)
+    back = kpm.SpectralDensity.load(q, h, kernel=halved)
+    assert np.array_equal(back.densities, sd.densities)

@@ -171,7 +171,8 @@ def test_normalize_params_maps_box_to_unit_cube():
     assert np.allclose(x, [[-1, -1, -1], [1, 1, 1], [0, 0, 0]])
 
 
-@pytest.mark.parametrize("bad", [dict(layers=(3, 8, 512)), dict(layers=(2, 8, 100))])
+@pytest.mark.parametrize("bad", [dict(layers=(4, 8, 512)), dict(layers=(2, 8, 100)),
+                                 dict(layers=(3, 8, 100))])
 def test_surrogate_rejects_bad_layers(bad):
     """Shape validation happens before any GPU work."""
     from kwant_amd.surrogate import MomentSurrogate

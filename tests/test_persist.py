@@ -34,3 +34,13 @@ def test_files_hold_no_pickles(tmp_path):
     _write(p, {"format": persist.FORMAT, "version": persist.VERSION})
     with np.load(p, allow_pickle=False) as z:
         assert all(z[k].dtype != object for k in z.files)
+
+
+def test_load_rejects_another_engine_abi(tmp_path):
+    """The saved recursion state is tied to the engine version that wrote it."""
+    from kwant_amd import kpm
+    p = tmp_path / "old.npz"
+    _write(p, {"format": persist.FORMAT, "version": persist.VERSION, "engine_abi": 1,
+               "n_rows": 4})
+    with pytest.raises(ValueError, match="engine ABI"):
+        kpm.SpectralDensity.load(p, np.eye(4))
