@@ -219,18 +219,21 @@ def _stratified_checks(count, seed=11):
 
 def _converged(wide, params, sets):
     """Damped moments from sets x 32 random vectors (seeded sets, the
-    reference's v0 offset skipped), the judge of the surrogate's DOS error."""
+    reference's v0 offset skipped), the judge of the surrogate's DOS error.
+    Also returns the means of the even and the odd sets: half the DOS L1
+    distance between them estimates the reference's own sampling error."""
     import numpy as np
     import torch
-    out = np.zeros((len(params), wide.num_moments))
+    halves = np.zeros((2, len(params), wide.num_moments))
     for k in range(sets):
         rng = np.random.RandomState(1000 + k)
         rng.random_sample(wide.n)
         vecs = np.stack([np.exp(2j * np.pi * rng.random_sample(wide.n))
                          for _ in range(wide.num_vectors)], 1)
         wide._vectors = torch.from_numpy(vecs).to(wide._vectors.device).to(wide.cdt)
-        out += wide.damped_moments(params) / sets
-    return out
+        halves[k % 2] += wide.damped_moments(params)
+    out = halves.sum(axis=0) / sets
+    return out, halves[0] / ((sets + 1) // 2), halves[1] / max(sets // 2, 1)
 
 
 def surrogate_bench(epochs, tol, points, check_count, check_sets, log_fn, design="shift"):
@@ -280,7 +283,8 @@ def surrogate_bench(epochs, tol, points, check_count, check_sets, log_fn, design
     t_data = time.perf_counter() - t0
     log_fn("surrogate (%s): %d training runs in %.1f s" % (design, len(train), t_data))
     t0 = time.perf_counter()
-    sur.fit(train, targets, epochs=epochs, batch=64, lr=3e-3, ema_decay=0.999)
+    sur.fit(train, targets, epochs=epochs, batch=64, lr=3e-3, ema_decay=0.999,
+            log=lambda msg: log_fn("surrogate (%s) %s" % (design, msg)))
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
     log_fn("surrogate (%s): %d epochs x %d Adam steps, 5 members (%d kept), in %.1f s"
@@ -304,20 +308,30 @@ def surrogate_bench(epochs, tol, points, check_count, check_sets, log_fn, design
     checks, strata = _stratified_checks(check_count)
     mom_c, fall_c, _ = sur.sweep(checks)
     kpm_c = wide.damped_moments(checks)          # plain KPM, the seeded 32 vectors
-    converged = _converged(wide, checks, check_sets)
+    converged, half_a, half_b = _converged(wide, checks, check_sets)
     err = dos_l1(mom_c, converged)
     err_kpm = dos_l1(kpm_c, converged)
+    ref_noise = 0.5 * dos_l1(half_a, half_b)
+    fall_c = np.asarray(fall_c, dtype=bool)
     per = {}
     for name, sl in strata.items():
+        e, f = err[sl], fall_c[sl]
+        acc = e[~f]
         per[name] = {"points": int(sl.stop - sl.start),
-                     "fallback_fraction": float(fall_c[sl].mean()),
-                     "dos_l1_median": float(np.median(err[sl])),
-                     "dos_l1_max": float(err[sl].max()),
-                     "plain_kpm32_dos_l1_median": float(np.median(err_kpm[sl]))}
+                     "fallback_fraction": float(f.mean()),
+                     "dos_l1_median": float(np.median(e)),
+                     "dos_l1_max": float(e.max()),
+                     "surrogate_points": int(acc.size),
+                     "surrogate_dos_l1_max": float(acc.max()) if acc.size else None,
+                     "plain_kpm32_dos_l1_median": float(np.median(err_kpm[sl])),
+                     "reference_noise_median": float(np.median(ref_noise[sl]))}
         log_fn("DOS L1 vs converged KPM (%d vectors), %s: sweep median %.2e max %.2e "
-               "(fallback %.1f%%); plain KPM(32) median %.2e"
+               "(fallback %.1f%%), max over the %d surrogate points %s; plain KPM(32) "
+               "median %.2e; the reference's own noise ~%.1e"
                % (32 * check_sets, name, per[name]["dos_l1_median"], per[name]["dos_l1_max"],
-                  100 * per[name]["fallback_fraction"], per[name]["plain_kpm32_dos_l1_median"]))
+                  100 * per[name]["fallback_fraction"], acc.size,
+                  "%.2e" % acc.max() if acc.size else "-",
+                  per[name]["plain_kpm32_dos_l1_median"], per[name]["reference_noise_median"]))
     net = ("(W, flux, mu) -> 4x256 GELU -> 512 wide-window moments, trained on 4096 runs"
            if literal else
            "(W, flux) -> 4x256 GELU -> 512 + exact shift in mu, trained on 4096 + 1024 "
@@ -332,6 +346,8 @@ def surrogate_bench(epochs, tol, points, check_count, check_sets, log_fn, design
         "dos_l1_vs_converged_median": float(np.median(err)),
         "dos_l1_vs_converged_p90": float(np.percentile(err, 90)),
         "dos_l1_vs_converged_max": float(err.max()),
+        "surrogate_points_dos_l1_max": float(err[~fall_c].max()) if (~fall_c).any() else None,
+        "reference_noise_median": float(np.median(ref_noise)),
         "plain_kpm32_dos_l1_vs_converged_median": float(np.median(err_kpm)),
         "dos_l1_by_stratum": per,
         "converged_reference": "KPM with %d random vectors on %d stratified points"
@@ -617,7 +633,7 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
            "gemm": {}}
     ref = None
     work = None
-    for name, prec, peak in (("fp64", eng.GEMM_FP64, "fp64"), ("fp32x3", eng.GEMM_FP32X3, "fp32"),
+    for name, prec, peak in (("fp64", eng.GEMM_FP64, "fp64"),
                              ("fp32", eng.GEMM_FP32, "fp32"), ("sliced3", 3, "bf16"),
                              ("sliced4", 4, "bf16")):
         sliced = name.startswith("sliced")

@@ -35,8 +35,12 @@ from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _chec
 from collections.abc import Iterable
 import math
 
-_GEMM = {'fp64': eng.GEMM_FP64, 'fp32': eng.GEMM_FP32, 'fp32x3': eng.GEMM_FP32X3,
-         'sliced3': 'sliced3', 'sliced4': 'sliced4'}
+# The split-precision mode is 'sliced4' (exact 8-bit slices on the BF16
+# matrix cores).  A hi/lo split into FP32 (KPM_GEMM_FP32X3 in the C ABI) is
+# not offered: the FP32 matrix cores round every product and their
+# accumulator to 24 bits, so it is no more accurate than plain FP32.
+_GEMM = {'fp64': eng.GEMM_FP64, 'fp32': eng.GEMM_FP32, 'sliced3': 'sliced3',
+         'sliced4': 'sliced4'}
 
 
 def _normalize_operator(op, params):
@@ -130,8 +134,7 @@ class Correlator:
         if gp is None:
             gp = 'fp64' if precision in ('complex128', 'c128') else 'fp32'
         if gp not in _GEMM:
-            raise ValueError("gemm_precision must be 'fp64', 'fp32', 'fp32x3', 'sliced3' "
-                             "or 'sliced4'")
+            raise ValueError("gemm_precision must be 'fp64', 'fp32', 'sliced3' or 'sliced4'")
         if gp.startswith('sliced') and precision not in ('complex128', 'c128'):
             raise ValueError("the sliced GEMM takes complex128 expansions")
         self._gemm_precision = gp

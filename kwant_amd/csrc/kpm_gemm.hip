@@ -56,18 +56,23 @@ __global__ __launch_bounds__(256) void corr_gemm_fma(const void* Av, int64_t lda
                                                      const void* Bv, int64_t ldb,
                                                      double* out, int64_t ldo, int m_rows,
                                                      int n_rows, int64_t k, int64_t kps,
-                                                     int k_chunk, int accumulate) {
+                                                     int k_chunk, int accumulate, int nblocks,
+                                                     int splits) {
   using T = typename GemmIn<PREC>::T;
   constexpr bool SPLIT = PREC == KPM_GEMM_FP32X3;
   constexpr bool F32 = PREC != KPM_GEMM_FP64;
   using F = typename std::conditional<F32, float, double>::type;
 
+  int m_tile, n_tile, split;
+  gemm_tile_of(blockIdx.x, nblocks, (m_rows + TM - 1) / TM, (n_rows + TN - 1) / TN, &m_tile,
+               &n_tile, &split);
+  if (split >= splits) return;  // grid padding
   const T* A = reinterpret_cast<const T*>(Av);
   const T* B = reinterpret_cast<const T*>(Bv);
   const int tid = threadIdx.x;
   const int tx = tid % 16, ty = tid / 16;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-  const int64_t kb = int64_t(blockIdx.z) * kps;
+  const int m0 = m_tile * TM, n0 = n_tile * TN;
+  const int64_t kb = int64_t(split) * kps;
   const int64_t ke = kb + kps < k ? kb + kps : k;
 
   __shared__ F a_re[KT][TM], a_im[KT][TM], b_re[KT][TN], b_im[KT][TN];
@@ -168,7 +173,7 @@ __global__ __launch_bounds__(256) void corr_gemm_fma(const void* Av, int64_t lda
     }
   }
 
-  double* o = out + int64_t(blockIdx.z) * (accumulate ? 0 : int64_t(m_rows) * ldo * 2);
+  double* o = out + int64_t(split) * (accumulate ? 0 : int64_t(m_rows) * ldo * 2);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -247,7 +252,8 @@ __global__ __launch_bounds__(256) void corr_gemm_mfma(const void* Av, int64_t ld
                                                       const void* Bv, int64_t ldb,
                                                       double* out, int64_t ldo, int m_rows,
                                                       int n_rows, int64_t k, int64_t kps,
-                                                      int k_chunk, int accumulate) {
+                                                      int k_chunk, int accumulate, int nblocks,
+                                                      int splits) {
   using T = typename GemmIn<PREC>::T;
   constexpr bool SPLIT = PREC == KPM_GEMM_FP32X3;
   constexpr bool F32 = PREC != KPM_GEMM_FP64;
@@ -255,14 +261,18 @@ __global__ __launch_bounds__(256) void corr_gemm_mfma(const void* Av, int64_t ld
   using MF = Mfma<F>;
   using acc_t = typename MF::acc_t;
 
+  int m_tile, n_tile, split;
+  gemm_tile_of(blockIdx.x, nblocks, (m_rows + MB - 1) / MB, (n_rows + MB - 1) / MB, &m_tile,
+               &n_tile, &split);
+  if (split >= splits) return;  // grid padding
   const T* A = reinterpret_cast<const T*>(Av);
   const T* B = reinterpret_cast<const T*>(Bv);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * MB, n0 = blockIdx.x * MB;
-  const int64_t kb = int64_t(blockIdx.z) * kps;
+  const int m0 = m_tile * MB, n0 = n_tile * MB;
+  const int64_t kb = int64_t(split) * kps;
   const int64_t ke = kb + kps < k ? kb + kps : k;
 
   __shared__ F as_[MB * MLD], bs_[MB * MLD];
@@ -381,7 +391,7 @@ __global__ __launch_bounds__(256) void corr_gemm_mfma(const void* Av, int64_t ld
     }
   }
 
-  double* o = out + int64_t(blockIdx.z) * (accumulate ? 0 : int64_t(m_rows) * ldo * 2);
+  double* o = out + int64_t(split) * (accumulate ? 0 : int64_t(m_rows) * ldo * 2);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -426,16 +436,19 @@ int launch_fma(const void* A, int64_t lda, const void* B, int64_t ldb, double* m
   kps = (kps + KT - 1) / KT * KT;
   slices = (k + kps - 1) / kps;
   if (k_chunk <= 0) k_chunk = 256;
-  dim3 grid((n_rows + TN - 1) / TN, (m_rows + TM - 1) / TM, unsigned(slices));
+  // a 1-D grid, a multiple of the XCD count: the kernels map workgroups to
+  // (split, tile) in XCD-local groups (gemm_tile_of)
+  const int64_t blocks = (int64_t(tiles) * slices + kNumXcd - 1) / kNumXcd * kNumXcd;
+  const dim3 grid(static_cast<unsigned>(blocks));
   static_assert(TM == MB && TN == MB && KT == MKC, "tile shapes of the two kernels differ");
   auto kern = use_mfma ? corr_gemm_mfma<PREC> : corr_gemm_fma<PREC>;
   if (slices == 1) {
     hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, A, lda, B, ldb, mu, ldc, m_rows, n_rows,
-                       k, kps, k_chunk, 1);
+                       k, kps, k_chunk, 1, int(blocks), int(slices));
     return check_hip(hipGetLastError(), "corr_gemm launch");
   }
   hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, A, lda, B, ldb, work, int64_t(n_rows),
-                     m_rows, n_rows, k, kps, k_chunk, 0);
+                     m_rows, n_rows, k, kps, k_chunk, 0, int(blocks), int(slices));
   int rc = check_hip(hipGetLastError(), "corr_gemm launch");
   if (rc) return rc;
   const int64_t total = int64_t(m_rows) * n_rows;

@@ -14,6 +14,36 @@ namespace kpm {
 // of rows (its own L2 then sees the neighbour rows of its gathers).
 constexpr int kNumXcd = 8;
 
+// Workgroup -> (k split, output tile) of the moment-matrix GEMMs.
+// Workgroups are dealt round-robin to the XCDs, so hardware block hb is
+// remapped to logical block (hb mod 8) * (blocks / 8) + hb / 8: each XCD owns
+// a contiguous run of logical blocks, split-major (blocks must be a multiple
+// of 8).  Within a split the tiles are ordered in groups of 4 row tiles x 8
+// column tiles when the tile grid allows, so the 32 workgroups an XCD holds
+// at once read 4 tiles' rows of A and 8 tiles' rows of B over the same k
+// range: each operand chunk comes from HBM once per XCD and is reused from
+// its L2 by the group.  Logical blocks past mt x nt x splits give split >=
+// the split count (grid padding).
+constexpr int kGemmGroupM = 4, kGemmGroupN = 8;
+
+__device__ inline void gemm_tile_of(int hb, int nblocks, int mt, int nt, int* m_tile,
+                                    int* n_tile, int* split) {
+  const int per = nblocks / kNumXcd;
+  const int lb = (hb % kNumXcd) * per + hb / kNumXcd;
+  const int tiles = mt * nt;
+  *split = lb / tiles;
+  const int t = lb - *split * tiles;
+  if (mt % kGemmGroupM == 0 && nt % kGemmGroupN == 0) {
+    const int g = t / (kGemmGroupM * kGemmGroupN), w = t - g * (kGemmGroupM * kGemmGroupN);
+    const int groups_n = nt / kGemmGroupN;
+    *m_tile = (g / groups_n) * kGemmGroupM + w / kGemmGroupN;
+    *n_tile = (g % groups_n) * kGemmGroupN + w % kGemmGroupN;
+  } else {
+    *m_tile = t / nt;
+    *n_tile = t % nt;
+  }
+}
+
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);

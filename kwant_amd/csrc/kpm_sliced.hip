@@ -123,13 +123,17 @@ template <int S>
 __global__ __launch_bounds__(256) void sliced_gemm_kernel(
     const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int m_rows, int n_rows,
     int64_t kp2, int64_t kps, const int* __restrict__ ea, const int* __restrict__ eb,
-    double* __restrict__ out, int64_t ldo, int accumulate) {
+    double* __restrict__ out, int64_t ldo, int accumulate, int nblocks, int zs) {
   __shared__ __bf16 la[S][SB * SLD];
   __shared__ __bf16 lb[S][SB * SLD];
+  const int mt = (m_rows + SB - 1) / SB, nt = (n_rows + SB - 1) / SB;
+  int m_tile, n_tile, split;
+  gemm_tile_of(blockIdx.x, nblocks, mt, nt, &m_tile, &n_tile, &split);
+  if (split >= zs) return;  // padding of the grid to a multiple of 8
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * SB, n0 = blockIdx.x * SB;
-  const int64_t kb = int64_t(blockIdx.z) * kps;
+  const int m0 = m_tile * SB, n0 = n_tile * SB;
+  const int64_t kb = int64_t(split) * kps;
   const int64_t ke = kb + kps < kp2 ? kb + kps : kp2;
 
   f4a acc_r[S][2][2], acc_i[S][2][2];
@@ -233,7 +237,7 @@ __global__ __launch_bounds__(256) void sliced_gemm_kernel(
     }
   }
 
-  double* o = out + int64_t(blockIdx.z) * int64_t(m_rows) * ldo * 2;
+  double* o = out + int64_t(split) * int64_t(m_rows) * ldo * 2;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -300,9 +304,13 @@ template <int S>
 void launch_sliced_gemm(const __bf16* As, const __bf16* Bs, int m_rows, int n_rows,
                         const SlicedPlan& p, const int* ea, const int* eb, double* part,
                         int accumulate, hipStream_t s) {
-  dim3 grid((n_rows + SB - 1) / SB, (m_rows + SB - 1) / SB, unsigned(p.zs));
-  hipLaunchKernelGGL(sliced_gemm_kernel<S>, grid, dim3(256), 0, s, As, Bs, m_rows, n_rows, p.kp2,
-                     p.kps, ea, eb, part, int64_t(n_rows), accumulate);
+  const int64_t tiles = int64_t((n_rows + SB - 1) / SB) * ((m_rows + SB - 1) / SB);
+  // a multiple of the XCD count (the kernel's remap needs it; the padding
+  // workgroups return at once)
+  const int64_t blocks = (tiles * p.zs + kNumXcd - 1) / kNumXcd * kNumXcd;
+  hipLaunchKernelGGL(sliced_gemm_kernel<S>, dim3(unsigned(blocks)), dim3(256), 0, s, As, Bs,
+                     m_rows, n_rows, p.kp2, p.kps, ea, eb, part, int64_t(n_rows), accumulate,
+                     int(blocks), int(p.zs));
 }
 
 }  // namespace

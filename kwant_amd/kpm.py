@@ -25,7 +25,7 @@ a permutation: the device row order, see kwant_amd.ordering; the moments do
 not depend on it), ``matrix_format`` ('csr' | 'bsr': the recursion reads the
 Hamiltonian as 4 x 4 site blocks -- for 4-orbital systems, operator=None; the
 moments are bitwise the CSR ones), ``gemm_precision`` for the Correlator
-('fp64' | 'fp32' | 'fp32x3' | 'sliced3' | 'sliced4').
+('fp64' | 'fp32' | 'sliced3' | 'sliced4'; 'sliced4' is the split-precision mode).
 """
 import math
 import numbers
