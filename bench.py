@@ -586,7 +586,7 @@ def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
     return out
 
 
-MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "int8": 5000.0}
 
 
 def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
@@ -635,8 +635,10 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
     work = None
     for name, prec, peak in (("fp64", eng.GEMM_FP64, "fp64"),
                              ("fp32", eng.GEMM_FP32, "fp32"), ("sliced3", 3, "bf16"),
-                             ("sliced4", 4, "bf16")):
-        sliced = name.startswith("sliced")
+                             ("sliced4", 4, "bf16"), ("isliced4", 4, "int8"),
+                             ("isliced5", 5, "int8")):
+        sliced = "sliced" in name
+        kind = "int8" if name.startswith("i") else "bf16"
         if sliced:
             a_op, b_op = omega.reshape(moments, k), psi.reshape(moments, k)
         elif prec == eng.GEMM_FP32:
@@ -649,7 +651,7 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
         def run():
             nonlocal work
             if sliced:
-                work = eng.corr_gemm_sliced(a_op, b_op, mu, slices=prec, work=work)
+                work = eng.corr_gemm_sliced(a_op, b_op, mu, slices=prec, work=work, kind=kind)
             else:
                 eng.corr_gemm(a_op, b_op, mu, prec)
         run()
@@ -667,9 +669,9 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
         if ref is None:
             ref = gam
         rel = float(np.linalg.norm(gam - ref) / np.linalg.norm(ref))
-        # MFMA work actually issued: 3 products for FP32X3, one per kept
-        # slice pair (S (S + 1) / 2) for the sliced GEMM
-        mult = {"fp32x3": 3, "sliced3": 6, "sliced4": 10}.get(name, 1)
+        # MFMA work actually issued: one product per kept slice pair
+        # (S (S + 1) / 2) for the sliced GEMMs
+        mult = prec * (prec + 1) // 2 if sliced else 1
         tflops = flops * mult / sec / 1e12
         out["gemm"][name] = {"seconds": sec, "algorithmic_TFLOPs": flops / sec / 1e12,
                              "mfma_TFLOPs": tflops, "mfma_peak_TFLOPs": MFMA_PEAK_TFLOPS[peak],

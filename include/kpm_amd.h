@@ -308,6 +308,21 @@ int kpm_corr_gemm_sliced(const void* A, int64_t lda, const void* B, int64_t ldb,
                          int64_t k, int32_t slices, void* work, int64_t work_bytes,
                          kpm_stream_t stream);
 
+/* The same product (kwant/kpm.py:685-695) on the INT8 matrix cores
+ * (v_mfma_i32_32x32x32_i8), after Ootomo, Ozaki and Yokota (2024): each real
+ * component is cut into `slices` (1..6) integer slices of <= 7 bits
+ * (|q| <= 64) against its row's largest exponent; slice products are summed
+ * exactly in int32 over at most 65536 k' per workgroup and folded into FP64
+ * once.  The error is about 2^(-7 slices) of max|A_m| max|B_n| per term
+ * (slices = 5: below the 4-slice BF16 form's).  work: device buffer of at
+ * least kpm_corr_gemm_islice_work() bytes.  Asynchronous. */
+int kpm_corr_gemm_islice_work(int32_t m_rows, int32_t n_rows, int64_t k, int32_t slices,
+                              int64_t* bytes);
+int kpm_corr_gemm_islice(const void* A, int64_t lda, const void* B, int64_t ldb,
+                         double* mu, int64_t ldc, int32_t m_rows, int32_t n_rows,
+                         int64_t k, int32_t slices, void* work, int64_t work_bytes,
+                         kpm_stream_t stream);
+
 /* Exact moments after a uniform on-site shift (no counterpart in kwant/kpm.py;
  * it acts on the moments kwant/kpm.py:436-477 produces).  nu holds moments
  * <T_k(t)> in a window t = (E - b') / a'; the moments in the window

@@ -40,7 +40,7 @@ EXPORTS = (
     "kpm_plan_set_dots_grid", "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps",
     "kpm_cheb_steps_static", "kpm_bsr_steps", "kpm_block_dots",
     "kpm_bond_records", "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
-    "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
+    "kpm_corr_gemm_islice_work", "kpm_corr_gemm_islice", "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
     "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
     "kpm_mlp_ensemble", "kpm_mlp_adam",
 )
@@ -139,6 +139,10 @@ def load_library(path=LIB_PATH):
         "kpm_corr_gemm_sliced_work": (ctypes.c_int, [i32, i32, i64, i32,
                                                      ctypes.POINTER(ctypes.c_int64)]),
         "kpm_corr_gemm_sliced": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
+                                                vp, i64, vp]),
+        "kpm_corr_gemm_islice_work": (ctypes.c_int, [i32, i32, i64, i32,
+                                                     ctypes.POINTER(ctypes.c_int64)]),
+        "kpm_corr_gemm_islice": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
                                                 vp, i64, vp]),
         "kpm_moment_shift": (ctypes.c_int, [vp, i64, i64, i32, i64, i32, vp, vp,
                                             ctypes.c_double, vp, vp, i64, i64, vp]),
@@ -608,9 +612,11 @@ def corr_gemm(a, b, mu, precision, k_chunk=256, work=None):
         "kpm_corr_gemm")
 
 
-def corr_gemm_sliced(a, b, mu, slices=4, work=None):
-    """mu += conj(a) @ b.T from complex128 a (m x k) and b (n x k) on the
-    BF16 matrix cores with ``slices`` exact 8-bit slices per component."""
+def corr_gemm_sliced(a, b, mu, slices=4, work=None, kind="bf16"):
+    """mu += conj(a) @ b.T from complex128 a (m x k) and b (n x k) with
+    ``slices`` exact integer slices per component: on the BF16 matrix cores
+    (kind 'bf16': slices of 8 bits, 1..4) or the INT8 matrix cores (kind
+    'int8': slices of 7 bits, 1..6)."""
     import torch
     m, k = a.shape
     n = b.shape[0]
@@ -620,16 +626,21 @@ def corr_gemm_sliced(a, b, mu, slices=4, work=None):
         raise ValueError("corr_gemm_sliced takes complex128 operands and output")
     if a.stride(1) != 1 or b.stride(1) != 1 or mu.stride(1) != 1:
         raise ValueError("operands must be contiguous along k")
+    if kind not in ("bf16", "int8"):
+        raise ValueError("kind must be 'bf16' or 'int8'")
     lib = load_library()
+    work_fn, gemm_fn, name = ((lib.kpm_corr_gemm_sliced_work, lib.kpm_corr_gemm_sliced,
+                               "kpm_corr_gemm_sliced") if kind == "bf16" else
+                              (lib.kpm_corr_gemm_islice_work, lib.kpm_corr_gemm_islice,
+                               "kpm_corr_gemm_islice"))
     need = ctypes.c_int64()
-    check(lib.kpm_corr_gemm_sliced_work(m, n, k, int(slices), ctypes.byref(need)),
-          "kpm_corr_gemm_sliced_work")
+    check(work_fn(m, n, k, int(slices), ctypes.byref(need)), name + "_work")
     if work is None or work.numel() * work.element_size() < need.value:
         work = torch.empty(need.value, dtype=torch.uint8, device=mu.device)
-    check(lib.kpm_corr_gemm_sliced(a.data_ptr(), a.stride(0), b.data_ptr(), b.stride(0),
-                                   mu.data_ptr(), mu.stride(0), m, n, k, int(slices),
-                                   work.data_ptr(), work.numel() * work.element_size(),
-                                   stream_handle()), "kpm_corr_gemm_sliced")
+    check(gemm_fn(a.data_ptr(), a.stride(0), b.data_ptr(), b.stride(0),
+                  mu.data_ptr(), mu.stride(0), m, n, k, int(slices),
+                  work.data_ptr(), work.numel() * work.element_size(),
+                  stream_handle()), name)
     return work
 
 

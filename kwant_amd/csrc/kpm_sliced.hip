@@ -313,6 +313,259 @@ void launch_sliced_gemm(const __bf16* As, const __bf16* Bs, int m_rows, int n_ro
                      int(blocks), int(p.zs));
 }
 
+// ---------------------------------------------------------------------------
+// The same product on the INT8 matrix cores (v_mfma_i32_32x32x32_i8), after
+// Ootomo, Ozaki and Yokota, "DGEMM on integer matrix multiplication unit"
+// (Int. J. High Perform. Comput. Appl., 2024): slices of at most 7 bits and
+// exact int32 accumulation.
+//     v = x 2^(6 - e_r);  q_1 = rint(v);  v = (v - q_1) 128;  q_2 = rint(v); ...
+//     x = 2^(e_r - 6) (q_1 + q_2 2^-7 + ... + q_S 2^(7 - 7S)) + t,  |t| <= 2^(e_r - 7S)
+// |v| < 64 before each rint, so |q_i| <= 64 fits int8 and every product is at
+// most 2^12 in magnitude.  A level l = i + j (0-based slices) has at most S
+// pairs, so its sum over 65536 k' is below S 2^28 < 2^31 for S <= 6: the
+// int32 accumulator is exact over that interval.  Every 65536 k' (and at the
+// end of the workgroup's k range) each level is folded into the FP64 partial
+// output as sum_l acc_l 2^(-7l), scaled by 2^(e_m + e_n - 12).  Against the
+// BF16 form an int8 slice carries one bit less, but takes half the bytes,
+// multiplies at twice the rate and needs no FP32 fold every 256 k'.
+
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef int i16v __attribute__((ext_vector_type(16)));
+
+constexpr int IB = 64;               // output tile (rows and cols)
+constexpr int IKS = 64;              // real k' per LDS stage (two MFMA k-steps)
+constexpr int ILD = IKS + 16;        // LDS row stride in bytes (16-byte aligned rows)
+constexpr int64_t IFOLD = 65536;     // real k' per exact int32 interval
+constexpr int kMaxISlices = 6;
+// word 3 of a raw (stride 0) buffer descriptor on this GPU family: 32-bit
+// data format, no swizzle
+constexpr int kBufferWord3 = 0x00020000;
+
+// S int8 slices of every component of the panel, 4 components per thread,
+// rows padded to kp2 with zeros: out[(s * rows + r) * kp2 + k'].
+__global__ __launch_bounds__(256) void islice_kernel(const double* __restrict__ X, int64_t ld,
+                                                     int rows, int64_t k0, int64_t kn,
+                                                     const int* __restrict__ e, int S,
+                                                     int8_t* __restrict__ out, int64_t kp2) {
+  const int r = blockIdx.y;
+  const int64_t c = 4 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
+  if (c >= kp2 || r >= rows) return;
+  const double* x = X + 2 * (int64_t(r) * ld + k0);
+  double v[4];
+  if (c + 4 <= 2 * kn) {
+    // (re, im) pairs are 16-byte aligned and c is a multiple of 4
+    const d2v p0 = *reinterpret_cast<const d2v*>(x + c);
+    const d2v p1 = *reinterpret_cast<const d2v*>(x + c + 2);
+    v[0] = p0.x;
+    v[1] = p0.y;
+    v[2] = p1.x;
+    v[3] = p1.y;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = c + j < 2 * kn ? x[c + j] : 0.0;
+  }
+  const int sh = 6 - e[r];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = ldexp(v[j], sh);
+  for (int s = 0; s < S; ++s) {
+    uint32_t packed = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double q = rint(v[j]);
+      packed |= (uint32_t(int(q)) & 0xFFu) << (8 * j);
+      v[j] = (v[j] - q) * 128.0;
+    }
+    *reinterpret_cast<uint32_t*>(out + (int64_t(s) * rows + r) * kp2 + c) = packed;
+  }
+}
+
+// (b1, -b0, b3, -b2, ...) on packed int8: the B'' = (im, -re) pairing.  The
+// even bytes are negated as 0x100 - b in 16-bit lanes (no borrow crosses a
+// lane; the low byte of the difference is -b mod 256) and moved up one byte.
+__device__ inline i4v swap_negate_i8(i4v b) {
+  i4v w;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const uint32_t u = uint32_t(b[p]);
+    const uint32_t neg = 0x01000100u - (u & 0x00FF00FFu);
+    w[p] = int(((u >> 8) & 0x00FF00FFu) | ((neg & 0x00FF00FFu) << 8));
+  }
+  return w;
+}
+
+static_assert(IKS == 64, "the staging loads 4 x 16 B per row and stage");
+
+template <int S>
+__device__ __forceinline__ void islice_gemm_body(
+    const int8_t* __restrict__ As, const int8_t* __restrict__ Bs, int m_rows, int n_rows,
+    int64_t kp2, int64_t kps, const int* __restrict__ ea, const int* __restrict__ eb,
+    double* __restrict__ out, int64_t ldo, int slot0, int nblocks, int zs) {
+  __shared__ __attribute__((aligned(16))) int8_t la[S][IB * ILD];
+  __shared__ __attribute__((aligned(16))) int8_t lb[S][IB * ILD];
+  const int mt = (m_rows + IB - 1) / IB, nt = (n_rows + IB - 1) / IB;
+  int m_tile, n_tile, split;
+  gemm_tile_of(blockIdx.x, nblocks, mt, nt, &m_tile, &n_tile, &split);
+  if (split >= zs) return;  // padding of the grid to a multiple of 8
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = m_tile * IB, n0 = n_tile * IB;
+  // kps <= IFOLD (islice_plan), so the int32 level sums are exact over the
+  // whole range and are folded once, at the end
+  const int64_t kb = int64_t(split) * kps;
+  const int64_t ke = kb + kps < kp2 ? kb + kps : kp2;
+
+  // one 32 x 32 tile per wave: 16 int32 per lane and level, Re and Im
+  i16v acc_r[S], acc_i[S];
+#pragma unroll
+  for (int l = 0; l < S; ++l)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc_r[l][r] = acc_i[l][r] = 0;
+
+  // staging: per slice and operand 64 rows x 64 B = 256 threads x 16 B; the
+  // next stage's global loads are in flight during this stage's products.
+  // Buffer loads: one descriptor per slice and operand covers the tile's
+  // valid rows (num_records = rows x kp2 < 2^27), so a lane needs a single
+  // 32-bit offset, the stage's k' goes in the scalar offset, and rows past
+  // the matrix read as zeros (out of the descriptor's range).
+  const int srow = tid >> 2, scol = (tid & 3) * 16;
+  const int rows_a = m_rows - m0 < IB ? m_rows - m0 : IB;
+  const int rows_b = n_rows - n0 < IB ? n_rows - n0 : IB;
+  const int voff = srow * int(kp2) + scol;
+  __amdgpu_buffer_rsrc_t ra[S], rb[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    ra[s] = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t*>(As + (int64_t(s) * m_rows + m0) * kp2), short(0),
+        rows_a * int(kp2), kBufferWord3);
+    rb[s] = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t*>(Bs + (int64_t(s) * n_rows + n0) * kp2), short(0),
+        rows_b * int(kp2), kBufferWord3);
+  }
+  i4v va[S], vb[S];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      va[s] = __builtin_bit_cast(i4v, __builtin_amdgcn_raw_buffer_load_b128(ra[s], voff, int(k0), 0));
+      vb[s] = __builtin_bit_cast(i4v, __builtin_amdgcn_raw_buffer_load_b128(rb[s], voff, int(k0), 0));
+    }
+  };
+
+  if (kb < ke) fetch(kb);
+  const int fr = lane & 31, fc = 16 * (lane >> 5);
+  for (int64_t k0 = kb; k0 < ke; k0 += IKS) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      *reinterpret_cast<i4v*>(&la[s][srow * ILD + scol]) = va[s];
+      *reinterpret_cast<i4v*>(&lb[s][srow * ILD + scol]) = vb[s];
+    }
+    __syncthreads();
+    if (k0 + IKS < ke) fetch(k0 + IKS);
+#pragma unroll
+    for (int kk = 0; kk < IKS; kk += 32) {
+      // A and B fragments hold the same k' bytes lane for lane, so the
+      // products pair a_re with b_re and a_im with b_im, and after the swap
+      // a_re with b_im and a_im with -b_re.
+      i4v a[S];
+#pragma unroll
+      for (int q = 0; q < S; ++q)
+        a[q] = *reinterpret_cast<const i4v*>(&la[q][(wm * 32 + fr) * ILD + kk + fc]);
+#pragma unroll
+      for (int sb = 0; sb < S; ++sb) {
+        const i4v b = *reinterpret_cast<const i4v*>(&lb[sb][(wn * 32 + fr) * ILD + kk + fc]);
+        const i4v bs = swap_negate_i8(b);
+#pragma unroll
+        for (int sa = 0; sa + sb < S; ++sa) {
+          acc_r[sa + sb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[sa], b, acc_r[sa + sb], 0, 0, 0);
+          acc_i[sa + sb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[sa], bs, acc_i[sa + sb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // Fold the exact level sums into this (panel, split)'s FP64 partial output
+  // slot; each element belongs to one lane of one workgroup.  32 x 32 MFMA
+  // layout: lane l holds column l mod 32 and rows 8 (r / 4) + 4 (l / 32) + r mod 4.
+  double* o = out + int64_t(slot0 + split) * int64_t(m_rows) * ldo * 2;
+  const int n = n0 + wn * 32 + (lane & 31);
+  if (n >= n_rows) return;
+  const int en = eb[n] - 12;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (m >= m_rows) continue;
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int l = S - 1; l >= 0; --l) {
+      sr += ldexp(double(acc_r[l][r]), -7 * l);
+      si += ldexp(double(acc_i[l][r]), -7 * l);
+    }
+    const int sc = ea[m] + en;
+    double* p = o + 2 * (int64_t(m) * ldo + n);
+    p[0] = ldexp(sr, sc);
+    p[1] = ldexp(si, sc);
+  }
+}
+
+// Two waves per SIMD: up to 5 slices, the accumulators (32 registers per
+// slice) and the staging fit in 256 registers (measured with 4 slices: 0.143
+// against 0.196 s per 1024 x 1024 x 2^21 product at one wave); 6 slices do
+// not, and run at one wave per SIMD.
+template <int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+islice_gemm_kernel(const int8_t* __restrict__ As, const int8_t* __restrict__ Bs, int m_rows,
+                   int n_rows, int64_t kp2, int64_t kps, const int* __restrict__ ea,
+                   const int* __restrict__ eb, double* __restrict__ out, int64_t ldo, int slot0,
+                   int nblocks, int zs) {
+  islice_gemm_body<S>(As, Bs, m_rows, n_rows, kp2, kps, ea, eb, out, ldo, slot0, nblocks, zs);
+}
+
+__global__ __launch_bounds__(256) void islice_gemm_kernel_wide(
+    const int8_t* __restrict__ As, const int8_t* __restrict__ Bs, int m_rows, int n_rows,
+    int64_t kp2, int64_t kps, const int* __restrict__ ea, const int* __restrict__ eb,
+    double* __restrict__ out, int64_t ldo, int slot0, int nblocks, int zs) {
+  islice_gemm_body<kMaxISlices>(As, Bs, m_rows, n_rows, kp2, kps, ea, eb, out, ldo, slot0,
+                                nblocks, zs);
+}
+
+SlicedPlan islice_plan(int m_rows, int n_rows, int64_t k, int slices) {
+  SlicedPlan p;
+  p.panel = k < (int64_t(1) << 20) ? k : (int64_t(1) << 20);
+  p.kp2 = (2 * p.panel + IKS - 1) / IKS * IKS;
+  const int64_t tiles = int64_t((m_rows + IB - 1) / IB) * ((n_rows + IB - 1) / IB);
+  int64_t zs = (2048 + tiles - 1) / tiles;
+  const int64_t by_k = (p.kp2 + 16383) / 16384;
+  if (zs > by_k) zs = by_k;
+  const int64_t exact = (p.kp2 + IFOLD - 1) / IFOLD;  // at most IFOLD k' per split
+  if (zs < exact) zs = exact;
+  if (zs < 1) zs = 1;
+  p.kps = ((p.kp2 + zs - 1) / zs + IKS - 1) / IKS * IKS;
+  if (p.kps > IFOLD) p.kps = IFOLD;
+  p.zs = (p.kp2 + p.kps - 1) / p.kps;
+  const int64_t panels = (k + p.panel - 1) / p.panel;
+  p.slice_bytes = int64_t(slices) * (m_rows + n_rows) * p.kp2;
+  p.exp_bytes = int64_t(m_rows + n_rows) * 4;
+  // one FP64 partial slot per (panel, split)
+  p.part_bytes = panels * p.zs * int64_t(m_rows) * n_rows * 2 * 8;
+  return p;
+}
+
+template <int S>
+void launch_islice_gemm(const int8_t* As, const int8_t* Bs, int m_rows, int n_rows,
+                        const SlicedPlan& p, const int* ea, const int* eb, double* part,
+                        int slot0, hipStream_t s) {
+  const int64_t tiles = int64_t((n_rows + IB - 1) / IB) * ((m_rows + IB - 1) / IB);
+  const int64_t blocks = (tiles * p.zs + kNumXcd - 1) / kNumXcd * kNumXcd;
+  if constexpr (S == kMaxISlices)
+    hipLaunchKernelGGL(islice_gemm_kernel_wide, dim3(unsigned(blocks)), dim3(256), 0, s, As, Bs,
+                       m_rows, n_rows, p.kp2, p.kps, ea, eb, part, int64_t(n_rows), slot0,
+                       int(blocks), int(p.zs));
+  else
+    hipLaunchKernelGGL(islice_gemm_kernel<S>, dim3(unsigned(blocks)), dim3(256), 0, s, As, Bs,
+                       m_rows, n_rows, p.kp2, p.kps, ea, eb, part, int64_t(n_rows), slot0,
+                       int(blocks), int(p.zs));
+}
+
 }  // namespace
 
 }  // namespace kpm
@@ -384,6 +637,75 @@ int kpm_corr_gemm_sliced(const void* A, int64_t lda, const void* B, int64_t ldb,
   hipLaunchKernelGGL(add_partials, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, part,
                      int(p.zs), mu, ldc, m_rows, n_rows);
   return check_hip(hipGetLastError(), "sliced GEMM partial sum");
+}
+
+int kpm_corr_gemm_islice_work(int32_t m_rows, int32_t n_rows, int64_t k, int32_t slices,
+                              int64_t* bytes) {
+  if (!bytes || m_rows <= 0 || n_rows <= 0 || k <= 0 || slices < 1 || slices > kMaxISlices)
+    return fail(KPM_ERR_ARG, "bad int8-sliced GEMM shape");
+  if (m_rows > 65535 || n_rows > 65535)
+    return fail(KPM_ERR_ARG, "at most 65535 rows per operand");
+  const SlicedPlan p = islice_plan(m_rows, n_rows, k, slices);
+  *bytes = p.slice_bytes + p.exp_bytes + p.part_bytes + 256;
+  return KPM_OK;
+}
+
+int kpm_corr_gemm_islice(const void* A, int64_t lda, const void* B, int64_t ldb, double* mu,
+                         int64_t ldc, int32_t m_rows, int32_t n_rows, int64_t k,
+                         int32_t slices, void* work, int64_t work_bytes, kpm_stream_t stream) {
+  if (!A || !B || !mu || !work) return fail(KPM_ERR_ARG, "NULL buffer");
+  if (m_rows <= 0 || n_rows <= 0 || k <= 0) return fail(KPM_ERR_ARG, "empty GEMM");
+  if (m_rows > 65535 || n_rows > 65535)
+    return fail(KPM_ERR_ARG, "at most 65535 rows per operand");
+  if (slices < 1 || slices > kMaxISlices) return fail(KPM_ERR_ARG, "slices must be 1..6");
+  if (lda < k || ldb < k || ldc < n_rows) return fail(KPM_ERR_ARG, "leading dimension too small");
+  const SlicedPlan p = islice_plan(m_rows, n_rows, k, slices);
+  if (work_bytes < p.slice_bytes + p.exp_bytes + p.part_bytes + 256)
+    return fail(KPM_ERR_ARG, "work buffer too small (kpm_corr_gemm_islice_work)");
+  hipStream_t s = (hipStream_t)stream;
+  char* w = static_cast<char*>(work);
+  double* part = reinterpret_cast<double*>(w);
+  int* ea = reinterpret_cast<int*>(w + p.part_bytes);
+  int* eb = ea + m_rows;
+  // kp2 is a multiple of 64, so both slice arrays start 64-byte aligned
+  int8_t* As = reinterpret_cast<int8_t*>(w + ((p.part_bytes + p.exp_bytes + 255) / 256) * 256);
+  int8_t* Bs = As + int64_t(slices) * m_rows * p.kp2;
+  const double* a = static_cast<const double*>(A);
+  const double* b = static_cast<const double*>(B);
+  int panel_no = 0;
+  for (int64_t k0 = 0; k0 < k; k0 += p.panel, ++panel_no) {
+    const int64_t kn = k - k0 < p.panel ? k - k0 : p.panel;
+    const unsigned spans = unsigned((2 * kn + kExpSpan - 1) / kExpSpan);
+    hipLaunchKernelGGL(exponent_clear, dim3((m_rows + 255) / 256), dim3(256), 0, s, ea, m_rows);
+    hipLaunchKernelGGL(exponent_clear, dim3((n_rows + 255) / 256), dim3(256), 0, s, eb, n_rows);
+    hipLaunchKernelGGL(row_exponent_kernel, dim3(spans, m_rows), dim3(256), 0, s, a, lda,
+                       m_rows, k0, kn, ea);
+    hipLaunchKernelGGL(row_exponent_kernel, dim3(spans, n_rows), dim3(256), 0, s, b, ldb,
+                       n_rows, k0, kn, eb);
+    hipLaunchKernelGGL(exponent_decode, dim3((m_rows + 255) / 256), dim3(256), 0, s, ea, m_rows);
+    hipLaunchKernelGGL(exponent_decode, dim3((n_rows + 255) / 256), dim3(256), 0, s, eb, n_rows);
+    const unsigned sg = unsigned((p.kp2 / 4 + 255) / 256);
+    hipLaunchKernelGGL(islice_kernel, dim3(sg, m_rows), dim3(256), 0, s, a, lda, m_rows, k0, kn,
+                       ea, int(slices), As, p.kp2);
+    hipLaunchKernelGGL(islice_kernel, dim3(sg, n_rows), dim3(256), 0, s, b, ldb, n_rows, k0, kn,
+                       eb, int(slices), Bs, p.kp2);
+    const int slot0 = panel_no * int(p.zs);
+    switch (slices) {
+      case 1: launch_islice_gemm<1>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      case 2: launch_islice_gemm<2>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      case 3: launch_islice_gemm<3>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      case 4: launch_islice_gemm<4>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      case 5: launch_islice_gemm<5>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      default: launch_islice_gemm<6>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+    }
+    int rc = check_hip(hipGetLastError(), "int8-sliced GEMM launch");
+    if (rc) return rc;
+  }
+  // the (panel, split) slots in order: deterministic
+  const int64_t total = int64_t(m_rows) * n_rows;
+  hipLaunchKernelGGL(add_partials, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, part,
+                     panel_no * int(p.zs), mu, ldc, m_rows, n_rows);
+  return check_hip(hipGetLastError(), "int8-sliced GEMM partial sum");
 }
 
 }  // extern "C"

@@ -226,6 +226,76 @@ def test_sliced_gemm_panels_and_accumulate():
     assert np.max(np.abs(mu.cpu().numpy() - want)) < 1e-8 * np.sqrt(k) * 16
 
 
+@pytest.mark.gpu
+def test_int8_sliced_gemm_exact_on_small_integers():
+    """Components that are integers of at most 6 bits (times a power of two
+    per row) fit in one 7-bit slice: every slice count gives the exact
+    product bitwise, on ragged tile shapes."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(1)
+    for m, n, k in ((70, 33, 5000), (64, 64, 32), (1, 129, 70001)):
+        a = (rng.randint(-63, 64, (m, k)) + 1j * rng.randint(-63, 64, (m, k))) \
+            * 2.0 ** rng.randint(-20, 20, (m, 1))
+        b = (rng.randint(-63, 64, (n, k)) + 1j * rng.randint(-63, 64, (n, k))) \
+            * 2.0 ** rng.randint(-20, 20, (n, 1))
+        want = _cgemm_ref(a, b)
+        for slices in (1, 3, 6):
+            mu = torch.zeros((m, n), dtype=torch.complex128, device="cuda")
+            eng.corr_gemm_sliced(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mu,
+                                 slices=slices, kind="int8")
+            assert np.array_equal(mu.cpu().numpy(), want), (m, n, k, slices)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slices,tol", [(2, 1e-3), (3, 1e-5), (4, 1e-7), (5, 1e-9), (6, 1e-11)])
+def test_int8_sliced_gemm_accuracy(slices, tol):
+    """Random complex128 operands with row scales spread over 2^+-30: the
+    error against the FP64 product falls by ~2^-7 per slice (the bound is
+    about (slices + 1) 2^(-7 slices) of the entry's natural scale)."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(10 + slices)
+    m, n, k = 130, 96, 70000
+    a = (rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))) \
+        * 2.0 ** rng.uniform(-30, 30, (m, 1))
+    b = (rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))) \
+        * 2.0 ** rng.uniform(-30, 30, (n, 1))
+    want = _cgemm_ref(a, b)
+    mu = torch.zeros((m, n), dtype=torch.complex128, device="cuda")
+    eng.corr_gemm_sliced(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mu,
+                         slices=slices, kind="int8")
+    got = mu.cpu().numpy()
+    scale = np.sqrt(k) * np.abs(a).max(1)[:, None] * np.abs(b).max(1)[None, :]
+    err = np.max(np.abs(got - want) / scale)
+    assert err < tol, err
+    # a slice more must gain accuracy (a dropped level or a wrong level
+    # scale would not)
+    if slices < 6:
+        mu2 = torch.zeros_like(mu)
+        eng.corr_gemm_sliced(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mu2,
+                             slices=slices + 1, kind="int8")
+        err2 = np.max(np.abs(mu2.cpu().numpy() - want) / scale)
+        assert err2 < err / 16, (err, err2)
+
+
+@pytest.mark.gpu
+def test_int8_sliced_gemm_panels_and_accumulate():
+    """k longer than one panel (2^20) and accumulation into an existing mu."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(19)
+    m, n, k = 17, 5, (1 << 20) + 4099
+    a = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
+    b = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    mu0 = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    mu = torch.from_numpy(mu0.copy()).cuda()
+    eng.corr_gemm_sliced(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mu, slices=5,
+                         kind="int8")
+    want = mu0 + _cgemm_ref(a, b)
+    assert np.max(np.abs(mu.cpu().numpy() - want)) < 1e-9 * np.sqrt(k) * 16
+
+
 def test_conductivity_reordered_equals_caller_order():
     """A shuffled square flake: conductivity with the rows stored in RCM
     order (velocity operators permuted alike) equals the caller-order run."""
@@ -344,6 +414,11 @@ def test_gamma_gemm_full_shape_sliced_vs_fp64():
     natural = np.sqrt(2.0 * k)
     err = float((mus - mu64).abs().max()) / natural
     assert err < 1e-7, err
+    # the INT8 form with 5 slices is closer still
+    work = eng.corr_gemm_sliced(a, b, mus.zero_(), slices=5, kind="int8")
+    del work
+    err8 = float((mus - mu64).abs().max()) / natural
+    assert err8 < err / 3 and err8 < 5e-8, (err, err8)
     rows = [0, 1, 517, 1023]
     ha = a[rows].cpu().numpy()
     hb = b[rows].cpu().numpy()
