@@ -685,8 +685,8 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
     torch.cuda.empty_cache()
     if full:
         sig, breakdown = {}, {}
-        for name, prec in (("fp64", "complex128"), ("sliced4", "complex128"),
-                           ("fp32", "complex64")):
+        for name, prec in (("fp64", "complex128"), ("isliced5", "complex128"),
+                           ("sliced4", "complex128"), ("fp32", "complex64")):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             c = conductivity(h, alpha='x', beta='y', positions=pos, num_moments=moments,
@@ -720,20 +720,23 @@ def correlator_bench(cells, moments, vectors, width, log_fn, full=True):
                       100 * breakdown[name]["recursion_frac_of_hbm_peak"],
                       breakdown[name]["velocity_spmm_s"], breakdown[name]["gemm_s"],
                       breakdown[name]["host_and_integral_factor_s"]))
+            # the cached device blocks are kept for the next precision: after
+            # an empty_cache() the next run spends ~4 s in fresh device
+            # allocations of its ~200 GB (measured)
             del c
-            torch.cuda.empty_cache()
         s64 = sig["fp64"][0]
         out["sigma_xy"] = {"mu": [-1.0, -0.5, 0.0, 0.5, 1.0],
                            "fp64": [float(np.real(v)) for v in s64]}
-        for name in ("sliced4", "fp32"):
+        for name in ("isliced5", "sliced4", "fp32"):
             d = float(np.max(np.abs(sig[name][0] - s64)) / np.max(np.abs(s64)))
             out["sigma_xy"][name + "_max_rel_dev"] = d
         for name in sig:
             out["sigma_xy"][name + "_seconds"] = sig[name][1]
         out["sigma_xy"]["breakdown"] = breakdown
-        log_fn("sigma_xy (FP64) at mu = -1..1: %s; max rel dev sliced4 %.2e, FP32 %.2e"
-               % (np.round(np.real(s64), 4), out["sigma_xy"]["sliced4_max_rel_dev"],
-                  out["sigma_xy"]["fp32_max_rel_dev"]))
+        log_fn("sigma_xy (FP64) at mu = -1..1: %s; max rel dev isliced5 %.2e, sliced4 %.2e, "
+               "FP32 %.2e" % (np.round(np.real(s64), 4), out["sigma_xy"]["isliced5_max_rel_dev"],
+                              out["sigma_xy"]["sliced4_max_rel_dev"],
+                              out["sigma_xy"]["fp32_max_rel_dev"]))
     return out
 
 

@@ -26,6 +26,7 @@ from scipy.integrate import simpson
 from scipy.sparse import coo_matrix, csr_matrix
 
 from . import _engine as eng
+from . import autotune
 from . import ordering
 from .operator import operator_layout
 from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _check_finite_vectors,
@@ -35,12 +36,15 @@ from .kpm import (SpectralDensity, _VectorFactory, RandomVectors, _as_csr, _chec
 from collections.abc import Iterable
 import math
 
-# The split-precision mode is 'sliced4' (exact 8-bit slices on the BF16
-# matrix cores).  A hi/lo split into FP32 (KPM_GEMM_FP32X3 in the C ABI) is
-# not offered: the FP32 matrix cores round every product and their
-# accumulator to 24 bits, so it is no more accurate than plain FP32.
-_GEMM = {'fp64': eng.GEMM_FP64, 'fp32': eng.GEMM_FP32, 'sliced3': 'sliced3',
-         'sliced4': 'sliced4'}
+# The split-precision modes cut each FP64 component into exact integer
+# slices: 'isliced4' / 'isliced5' (7-bit slices on the INT8 matrix cores,
+# exact int32 sums; 5 slices are within ~1e-9 of FP64 and the fastest form)
+# and 'sliced3' / 'sliced4' (8-bit slices on the BF16 matrix cores).  A hi/lo
+# split into FP32 (KPM_GEMM_FP32X3 in the C ABI) is not offered: the FP32
+# matrix cores round every product and their accumulator to 24 bits, so it is
+# no more accurate than plain FP32.
+_GEMM = {'fp64': eng.GEMM_FP64, 'fp32': eng.GEMM_FP32, 'sliced3': ('bf16', 3),
+         'sliced4': ('bf16', 4), 'isliced4': ('int8', 4), 'isliced5': ('int8', 5)}
 
 
 def _normalize_operator(op, params):
@@ -134,12 +138,12 @@ class Correlator:
         if gp is None:
             gp = 'fp64' if precision in ('complex128', 'c128') else 'fp32'
         if gp not in _GEMM:
-            raise ValueError("gemm_precision must be 'fp64', 'fp32', 'sliced3' or 'sliced4'")
-        if gp.startswith('sliced') and precision not in ('complex128', 'c128'):
+            raise ValueError("gemm_precision must be one of " + ", ".join(sorted(_GEMM)))
+        if 'sliced' in gp and precision not in ('complex128', 'c128'):
             raise ValueError("the sliced GEMM takes complex128 expansions")
         self._gemm_precision = gp
         self._max_width = kwargs.pop('max_block_width', 16)
-        kwargs.pop('variant', None)
+        self._variant = int(kwargs.pop('variant', -1))
         self._op1 = _normalize_operator(operator1, params)
         self._op2 = _normalize_operator(operator2, params)
 
@@ -292,7 +296,7 @@ class Correlator:
         view of the ring holding T_m0 .. T_{m1-1} -- in order."""
         chunk = max(3, min(m_count, self.OMEGA_CHUNK))
         rec = eng.BlockRecursion(self._spectrum_R.dmat, ab, start, width, eng.DOTS_NONE,
-                                 store_slots=chunk, wrap=True)
+                                 store_slots=chunk, wrap=True, plan=self._plan_for(width))
         consumed, produced = 0, 1
         while consumed < m_count:
             target = min(m_count, consumed + chunk)
@@ -308,6 +312,21 @@ class Correlator:
                 m0 = m1
             consumed = target
         del rec
+
+    def _plan_for(self, width):
+        """The launch plan of the expansions at this block width: one per
+        width, shared by every expansion (creating a plan synchronises the
+        device), with the autotuned launch shape unless ``variant`` was
+        given."""
+        plans = self.__dict__.setdefault('_plans', {})
+        plan = plans.get(width)
+        if plan is None:
+            dmat = self._spectrum_R.dmat
+            _, vt = eng.complex_dtype(dmat.precision)
+            variant = self._variant if self._variant >= 0 else autotune.pick_variant(
+                dmat, (self._a, self._b), width, eng.DOTS_NONE)
+            plan = plans[width] = eng.Plan(dmat.shape[0], width, vt, variant=variant)
+        return plan
 
     def _timed(self, phase):
         """A context that adds the GPU time of the enclosed launches to
@@ -347,21 +366,35 @@ class Correlator:
             count, self._width_cap(n, m_count, cdt))
         self.block_widths = getattr(self, 'block_widths', []) + list(widths)
         gp = _GEMM[self._gemm_precision]
-        start = r0
-        for width in widths:
-            vecs = [factory[r] for r in range(start, start + width)]
-            start += width
+
+        def draw(first, width):
+            # the factory's vectors in index order (the reference's RNG order)
+            vecs = [factory[r] for r in range(first, first + width)]
             host = np.stack([np.asarray(v) for v in vecs], axis=1).astype(np.complex128)
             _check_finite_vectors(host)
-            block = torch.from_numpy(np.ascontiguousarray(dmat.rows_in(host))).to(dev).to(cdt)
+            return np.ascontiguousarray(dmat.rows_in(host))
+
+        # Psi storage for the widest block, reused by every block (views);
+        # one launch plan per block width, shared by all expansions
+        psi_store = torch.empty(m_count * n * max(widths, default=1), dtype=cdt, device=dev)
+        start = r0
+        host_next = draw(start, widths[0]) if widths else None
+        for bi, width in enumerate(widths):
+            host = host_next
+            start += width
+            block = torch.from_numpy(host).to(dev).to(cdt)
             # Psi_n = op2 T_n(H~)|r>, resident for the whole block
-            psi = torch.empty((m_count, n, width), dtype=cdt, device=dev)
+            psi = psi_store[:m_count * n * width].view(m_count, n, width)
 
             def to_psi(m0, m1, slots):
                 with self._timed('operator_s'):
                     self._apply('op2', self._op2, slots, psi[m0:m1], width, m1 - m0)
 
             self._stream((self._a, self._b), block, width, m_count, to_psi)
+            # the next block's vectors are drawn on the host while the GPU
+            # runs this block's expansions
+            if bi + 1 < len(widths):
+                host_next = draw(start, widths[bi + 1])
             # Omega_m = T_m(H~) op1|r>, streamed in chunks against Psi
             start_l = torch.empty_like(block)
             with self._timed('operator_s'):
@@ -373,9 +406,9 @@ class Correlator:
             def gemm(m0, m1, slots):
                 a = slots.reshape(m1 - m0, k)
                 with self._timed('gemm_s'):
-                    if isinstance(gp, str):
+                    if isinstance(gp, tuple):
                         self._gemm_work = eng.corr_gemm_sliced(
-                            a, psi_k, mu[m0:m1], slices=int(gp[-1]),
+                            a, psi_k, mu[m0:m1], slices=gp[1], kind=gp[0],
                             work=getattr(self, '_gemm_work', None))
                     else:
                         eng.corr_gemm(a, psi_k, mu[m0:m1], gp)

@@ -280,6 +280,7 @@ struct SlicedPlan {
   int64_t kp2;        // padded real k' per panel row
   int64_t zs;         // k-splits per panel
   int64_t kps;        // real k' per split (multiple of SK)
+  int64_t ldk;        // row pitch of the int8 slice arrays in bytes
   int64_t slice_bytes, exp_bytes, part_bytes;
 };
 
@@ -287,6 +288,7 @@ SlicedPlan sliced_plan(int m_rows, int n_rows, int64_t k, int slices) {
   SlicedPlan p;
   p.panel = k < (int64_t(1) << 20) ? k : (int64_t(1) << 20);
   p.kp2 = (2 * p.panel + SK - 1) / SK * SK;
+  p.ldk = p.kp2;
   const int64_t tiles = int64_t((m_rows + SB - 1) / SB) * ((n_rows + SB - 1) / SB);
   int64_t zs = (2048 + tiles - 1) / tiles;
   const int64_t by_k = (p.kp2 + 4 * FOLD - 1) / (4 * FOLD);
@@ -340,13 +342,18 @@ constexpr int kMaxISlices = 6;
 // word 3 of a raw (stride 0) buffer descriptor on this GPU family: 32-bit
 // data format, no swizzle
 constexpr int kBufferWord3 = 0x00020000;
+// Slice rows are kp2 bytes (a power of two at full panels) plus this much
+// padding apart, so that the rows a workgroup reads at one k' do not all
+// fall on the same L2 set and channel.
+constexpr int kIPad = 256;
 
 // S int8 slices of every component of the panel, 4 components per thread,
-// rows padded to kp2 with zeros: out[(s * rows + r) * kp2 + k'].
+// rows padded to kp2 with zeros: out[(s * rows + r) * ldk + k'].
 __global__ __launch_bounds__(256) void islice_kernel(const double* __restrict__ X, int64_t ld,
                                                      int rows, int64_t k0, int64_t kn,
                                                      const int* __restrict__ e, int S,
-                                                     int8_t* __restrict__ out, int64_t kp2) {
+                                                     int8_t* __restrict__ out, int64_t kp2,
+                                                     int64_t ldk) {
   const int r = blockIdx.y;
   const int64_t c = 4 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
   if (c >= kp2 || r >= rows) return;
@@ -375,7 +382,7 @@ __global__ __launch_bounds__(256) void islice_kernel(const double* __restrict__ 
       packed |= (uint32_t(int(q)) & 0xFFu) << (8 * j);
       v[j] = (v[j] - q) * 128.0;
     }
-    *reinterpret_cast<uint32_t*>(out + (int64_t(s) * rows + r) * kp2 + c) = packed;
+    *reinterpret_cast<uint32_t*>(out + (int64_t(s) * rows + r) * ldk + c) = packed;
   }
 }
 
@@ -398,8 +405,9 @@ static_assert(IKS == 64, "the staging loads 4 x 16 B per row and stage");
 template <int S>
 __device__ __forceinline__ void islice_gemm_body(
     const int8_t* __restrict__ As, const int8_t* __restrict__ Bs, int m_rows, int n_rows,
-    int64_t kp2, int64_t kps, const int* __restrict__ ea, const int* __restrict__ eb,
-    double* __restrict__ out, int64_t ldo, int slot0, int nblocks, int zs) {
+    int64_t kp2, int64_t ldk, int64_t kps, const int* __restrict__ ea,
+    const int* __restrict__ eb, double* __restrict__ out, int64_t ldo, int slot0, int nblocks,
+    int zs) {
   __shared__ __attribute__((aligned(16))) int8_t la[S][IB * ILD];
   __shared__ __attribute__((aligned(16))) int8_t lb[S][IB * ILD];
   const int mt = (m_rows + IB - 1) / IB, nt = (n_rows + IB - 1) / IB;
@@ -424,22 +432,22 @@ __device__ __forceinline__ void islice_gemm_body(
   // staging: per slice and operand 64 rows x 64 B = 256 threads x 16 B; the
   // next stage's global loads are in flight during this stage's products.
   // Buffer loads: one descriptor per slice and operand covers the tile's
-  // valid rows (num_records = rows x kp2 < 2^27), so a lane needs a single
+  // valid rows (num_records = rows x ldk < 2^28), so a lane needs a single
   // 32-bit offset, the stage's k' goes in the scalar offset, and rows past
   // the matrix read as zeros (out of the descriptor's range).
   const int srow = tid >> 2, scol = (tid & 3) * 16;
   const int rows_a = m_rows - m0 < IB ? m_rows - m0 : IB;
   const int rows_b = n_rows - n0 < IB ? n_rows - n0 : IB;
-  const int voff = srow * int(kp2) + scol;
+  const int voff = srow * int(ldk) + scol;
   __amdgpu_buffer_rsrc_t ra[S], rb[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     ra[s] = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<int8_t*>(As + (int64_t(s) * m_rows + m0) * kp2), short(0),
-        rows_a * int(kp2), kBufferWord3);
+        const_cast<int8_t*>(As + (int64_t(s) * m_rows + m0) * ldk), short(0),
+        rows_a * int(ldk), kBufferWord3);
     rb[s] = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<int8_t*>(Bs + (int64_t(s) * n_rows + n0) * kp2), short(0),
-        rows_b * int(kp2), kBufferWord3);
+        const_cast<int8_t*>(Bs + (int64_t(s) * n_rows + n0) * ldk), short(0),
+        rows_b * int(ldk), kBufferWord3);
   }
   i4v va[S], vb[S];
   auto fetch = [&](int64_t k0) {
@@ -514,17 +522,18 @@ __device__ __forceinline__ void islice_gemm_body(
 template <int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 islice_gemm_kernel(const int8_t* __restrict__ As, const int8_t* __restrict__ Bs, int m_rows,
-                   int n_rows, int64_t kp2, int64_t kps, const int* __restrict__ ea,
+                   int n_rows, int64_t kp2, int64_t ldk, int64_t kps, const int* __restrict__ ea,
                    const int* __restrict__ eb, double* __restrict__ out, int64_t ldo, int slot0,
                    int nblocks, int zs) {
-  islice_gemm_body<S>(As, Bs, m_rows, n_rows, kp2, kps, ea, eb, out, ldo, slot0, nblocks, zs);
+  islice_gemm_body<S>(As, Bs, m_rows, n_rows, kp2, ldk, kps, ea, eb, out, ldo, slot0, nblocks,
+                      zs);
 }
 
 __global__ __launch_bounds__(256) void islice_gemm_kernel_wide(
     const int8_t* __restrict__ As, const int8_t* __restrict__ Bs, int m_rows, int n_rows,
-    int64_t kp2, int64_t kps, const int* __restrict__ ea, const int* __restrict__ eb,
+    int64_t kp2, int64_t ldk, int64_t kps, const int* __restrict__ ea, const int* __restrict__ eb,
     double* __restrict__ out, int64_t ldo, int slot0, int nblocks, int zs) {
-  islice_gemm_body<kMaxISlices>(As, Bs, m_rows, n_rows, kp2, kps, ea, eb, out, ldo, slot0,
+  islice_gemm_body<kMaxISlices>(As, Bs, m_rows, n_rows, kp2, ldk, kps, ea, eb, out, ldo, slot0,
                                 nblocks, zs);
 }
 
@@ -543,7 +552,8 @@ SlicedPlan islice_plan(int m_rows, int n_rows, int64_t k, int slices) {
   if (p.kps > IFOLD) p.kps = IFOLD;
   p.zs = (p.kp2 + p.kps - 1) / p.kps;
   const int64_t panels = (k + p.panel - 1) / p.panel;
-  p.slice_bytes = int64_t(slices) * (m_rows + n_rows) * p.kp2;
+  p.ldk = p.kp2 + kIPad;
+  p.slice_bytes = int64_t(slices) * (m_rows + n_rows) * p.ldk;
   p.exp_bytes = int64_t(m_rows + n_rows) * 4;
   // one FP64 partial slot per (panel, split)
   p.part_bytes = panels * p.zs * int64_t(m_rows) * n_rows * 2 * 8;
@@ -558,11 +568,11 @@ void launch_islice_gemm(const int8_t* As, const int8_t* Bs, int m_rows, int n_ro
   const int64_t blocks = (tiles * p.zs + kNumXcd - 1) / kNumXcd * kNumXcd;
   if constexpr (S == kMaxISlices)
     hipLaunchKernelGGL(islice_gemm_kernel_wide, dim3(unsigned(blocks)), dim3(256), 0, s, As, Bs,
-                       m_rows, n_rows, p.kp2, p.kps, ea, eb, part, int64_t(n_rows), slot0,
+                       m_rows, n_rows, p.kp2, p.ldk, p.kps, ea, eb, part, int64_t(n_rows), slot0,
                        int(blocks), int(p.zs));
   else
     hipLaunchKernelGGL(islice_gemm_kernel<S>, dim3(unsigned(blocks)), dim3(256), 0, s, As, Bs,
-                       m_rows, n_rows, p.kp2, p.kps, ea, eb, part, int64_t(n_rows), slot0,
+                       m_rows, n_rows, p.kp2, p.ldk, p.kps, ea, eb, part, int64_t(n_rows), slot0,
                        int(blocks), int(p.zs));
 }
 
@@ -667,9 +677,10 @@ int kpm_corr_gemm_islice(const void* A, int64_t lda, const void* B, int64_t ldb,
   double* part = reinterpret_cast<double*>(w);
   int* ea = reinterpret_cast<int*>(w + p.part_bytes);
   int* eb = ea + m_rows;
-  // kp2 is a multiple of 64, so both slice arrays start 64-byte aligned
+  // the row pitch ldk is a multiple of 64, so both slice arrays start
+  // 64-byte aligned
   int8_t* As = reinterpret_cast<int8_t*>(w + ((p.part_bytes + p.exp_bytes + 255) / 256) * 256);
-  int8_t* Bs = As + int64_t(slices) * m_rows * p.kp2;
+  int8_t* Bs = As + int64_t(slices) * m_rows * p.ldk;
   const double* a = static_cast<const double*>(A);
   const double* b = static_cast<const double*>(B);
   int panel_no = 0;
@@ -686,9 +697,9 @@ int kpm_corr_gemm_islice(const void* A, int64_t lda, const void* B, int64_t ldb,
     hipLaunchKernelGGL(exponent_decode, dim3((n_rows + 255) / 256), dim3(256), 0, s, eb, n_rows);
     const unsigned sg = unsigned((p.kp2 / 4 + 255) / 256);
     hipLaunchKernelGGL(islice_kernel, dim3(sg, m_rows), dim3(256), 0, s, a, lda, m_rows, k0, kn,
-                       ea, int(slices), As, p.kp2);
+                       ea, int(slices), As, p.kp2, p.ldk);
     hipLaunchKernelGGL(islice_kernel, dim3(sg, n_rows), dim3(256), 0, s, b, ldb, n_rows, k0, kn,
-                       eb, int(slices), Bs, p.kp2);
+                       eb, int(slices), Bs, p.kp2, p.ldk);
     const int slot0 = panel_no * int(p.zs);
     switch (slices) {
       case 1: launch_islice_gemm<1>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;

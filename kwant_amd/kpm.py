@@ -25,7 +25,8 @@ a permutation: the device row order, see kwant_amd.ordering; the moments do
 not depend on it), ``matrix_format`` ('csr' | 'bsr': the recursion reads the
 Hamiltonian as 4 x 4 site blocks -- for 4-orbital systems, operator=None; the
 moments are bitwise the CSR ones), ``gemm_precision`` for the Correlator
-('fp64' | 'fp32' | 'sliced3' | 'sliced4'; 'sliced4' is the split-precision mode).
+('fp64' | 'fp32' | 'isliced4' | 'isliced5' | 'sliced3' | 'sliced4': the 'isliced'
+modes are the split-precision GEMM on the INT8 matrix cores, 'sliced' on BF16).
 """
 import math
 import numbers

@@ -55,7 +55,7 @@ def test_correlator_matches_oracle():
         assert abs(gpu(mu, 0.1) - ora(mu, 0.1)) < 1e-9 * (1 + abs(ora(mu, 0.1)))
 
 
-@pytest.mark.parametrize("gemm", ["fp64", "sliced4"])
+@pytest.mark.parametrize("gemm", ["fp64", "sliced4", "isliced5"])
 def test_correlator_streamed_rings_match_oracle(monkeypatch, gemm):
     """Both expansions stream through rings shorter than the moment count
     (runs wrap around the ring's end; Omega reaches the GEMM in chunks)."""
@@ -70,7 +70,8 @@ def test_correlator_streamed_rings_match_oracle(monkeypatch, gemm):
     ora = ko.Correlator(h, operator1=op1, operator2=op2, **kw)
     assert gpu.block_widths == [4, 2]
     scale = np.abs(ora.moments_matrix).max()
-    # 4 slices keep 32 bits of every component against its row's largest
+    # 4 BF16 slices keep 32 bits of every component against its row's
+    # largest, 5 INT8 slices 35 bits
     tol = 1e-10 if gemm == "fp64" else 1e-8
     assert rel(gpu.moments_matrix, ora.moments_matrix, scale) < tol
     assert set(gpu.timings) == {"expansion_s", "operator_s", "gemm_s"}
@@ -388,6 +389,10 @@ def test_conductivity_dirac_4band_matches_oracle():
     sl4 = kpm().conductivity(h, 'x', 'y', positions=pos, gemm_precision='sliced4', **kw)
     d_sl4 = rel([sl4(mu, 0.05) for mu in mus], want, scale)
     assert d_sl4 < 5e-6, d_sl4
+    # 5 INT8 slices: 3 bits more per component
+    is5 = kpm().conductivity(h, 'x', 'y', positions=pos, gemm_precision='isliced5', **kw)
+    d_is5 = rel([is5(mu, 0.05) for mu in mus], want, scale)
+    assert d_is5 < 2e-6 and d_is5 < d_sl4, (d_is5, d_sl4)
     f32 = kpm().conductivity(h, 'x', 'y', positions=pos, precision='complex64', **kw)
     d_f32 = rel([f32(mu, 0.05) for mu in mus], want, scale)
     assert d_f32 < 2e-3, d_f32
