@@ -30,6 +30,8 @@
 // lane, so a row of the block is read as one coalesced 512-1024 byte access
 // when W*sizeof(complex) allows.
 
+#include <type_traits>
+
 #include "kpm_internal.h"
 
 namespace kpm {
@@ -275,11 +277,32 @@ __device__ inline void spmm_rows_scalar(const int32_t* __restrict__ indptr,
 // FP64 dot contributions of one row (lane columns) to the accumulators:
 // TwoSum pairs (as, ac) and, in the doubling mode, the exact grid sums aq of
 // Im <a_{n+1}|a_n> with the columns' grid exponents qk.
+//
+// fg (complex64 vectors, doubling mode, grid set): all three sums are exact
+// sums on the single grid 2^-k = 2^(E-51): each term is rounded to the grid
+// (3 FP64 operations instead of a 6-operation TwoSum or the two-level grid)
+// and the integer sums are order-independent.  The rounding of each term,
+// at most 2^-52 of <a_0|a_0>, is far below what the FP32 vectors carry
+// (their recursion is good to ~1e-7), and the sums stay the same bits for
+// every launch shape.  as holds grid units until finish_grid_dots.
 template <typename T, int VEC, int MODE, int NC, int NQ>
 __device__ inline void add_dots(double (&as)[NC][VEC], double (&ac)[NC][VEC],
-                                double (&aq)[NQ][VEC], const int (&qk)[VEC],
+                                double (&aq)[NQ][VEC], const int (&qk)[VEC], bool fg,
                                 const CLane<T, VEC>& xs, const CLane<T, VEC>& y,
                                 const CLane<T, VEC>& ww) {
+  if constexpr (MODE == KPM_DOTS_DOUBLING && std::is_same<T, float>::value) {
+    if (fg) {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const double xr = xs.re[v], xi = xs.im[v];
+        const double yr = y.re[v], yi = y.im[v];
+        as[0][v] += rint(ldexp(fma(xr, xr, xi * xi), qk[v]));
+        as[1][v] += rint(ldexp(fma(yr, xr, yi * xi), qk[v]));
+        aq[0][v] += rint(ldexp(fma(yr, xi, -(yi * xr)), qk[v]));
+      }
+      return;
+    }
+  }
   if constexpr (MODE == KPM_DOTS_DOUBLING) {
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
@@ -300,6 +323,21 @@ __device__ inline void add_dots(double (&as)[NC][VEC], double (&ac)[NC][VEC],
       acc2(as[1][v], ac[1][v], fma(wr, xi, -(wi * xr)));
       acc2(as[2][v], ac[2][v], fma(wr, yr, wi * yi));
       acc2(as[3][v], ac[3][v], fma(wr, yi, -(wi * yr)));
+    }
+  }
+}
+
+// The single-grid sums (fg) back from grid units to values (exact scaling),
+// so that the workgroup reduction adds them as (sum, 0) pairs: sums of grid
+// multiples below 2^52 units are exact in every order.
+template <typename T, int MODE, int NC, int VEC>
+__device__ inline void finish_grid_dots(bool fg, double (&as)[NC][VEC], const int (&qk)[VEC]) {
+  if constexpr (MODE == KPM_DOTS_DOUBLING && std::is_same<T, float>::value) {
+    if (fg) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) as[c][v] = ldexp(as[c][v], -qk[v]);
     }
   }
 }
@@ -576,6 +614,8 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
     for (int c = 0; c < NQA; ++c) aq[c][v] = 0.0;
     qk[v] = NQ > 0 && p.im_exp != nullptr ? p.im_exp[seg * W + col + v] : 0;
   }
+  const bool fg = MODE == KPM_DOTS_DOUBLING && std::is_same<T, float>::value &&
+                  p.im_exp != nullptr;
 
   if constexpr (RPW <= 2) {
     // In each chunk a wave owns a contiguous range of ipc * RPW rows and walks
@@ -628,7 +668,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
           zero(xs[u]);
           zero(y[u]);
         }
-        add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, xs[u], y[u], ww[u]);
+        add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, fg, xs[u], y[u], ww[u]);
         if constexpr (MODE == KPM_DOTS_LOCAL)
           write_local<T, VEC, LPR>(p, r0 + u * RPW + grp, live[u], col, only_dots ? xs[u] : y[u],
                                    ww[u]);
@@ -764,7 +804,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         zero(xs);
         zero(y);
       }
-      add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, xs, y, ww);
+      add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, fg, xs, y, ww);
       if constexpr (MODE == KPM_DOTS_LOCAL)
         write_local<T, VEC, LPR>(p, row, live, col, only_dots ? xs : y, ww);
 #pragma unroll
@@ -813,7 +853,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
         zero(xs);
         zero(y);
       }
-      add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, xs, y, ww);
+      add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, fg, xs, y, ww);
       if constexpr (MODE == KPM_DOTS_LOCAL)
         write_local<T, VEC, LPR>(p, row, live, col, only_dots ? xs : y, ww);
     }
@@ -821,6 +861,7 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
 
   if (MODE == KPM_DOTS_NONE || MODE == KPM_DOTS_LOCAL) return;
 
+  finish_grid_dots<T, MODE>(fg, as, qk);
   store_partials<VEC, LPR, WAVES, NC, NQ, NQA>(p, lb, wave, grp, col, tid, as, ac, aq);
 }
 
@@ -962,6 +1003,8 @@ __global__ __launch_bounds__(WAVES * 64) void bsr_kernel(StepParams p,
     for (int c = 0; c < NQA; ++c) aq[c][v] = 0.0;
     qk[v] = NQ > 0 && p.im_exp != nullptr ? p.im_exp[seg * W + col + v] : 0;
   }
+  const bool fg = MODE == KPM_DOTS_DOUBLING && std::is_same<T, float>::value &&
+                  p.im_exp != nullptr;
 
   for (int it = 0; it < iters; ++it) {
     const int64_t r64 = iter_row0(it, rr.x, stride, ipc, WAVES * RPW) + wave * RPW;
@@ -1031,9 +1074,10 @@ __global__ __launch_bounds__(WAVES * 64) void bsr_kernel(StepParams p,
     }
     CLane<T, VEC> ww;
     zero(ww);
-    add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, xs, y, ww);
+    add_dots<T, VEC, MODE, NC, NQA>(as, ac, aq, qk, fg, xs, y, ww);
   }
   if (MODE == KPM_DOTS_NONE) return;
+  finish_grid_dots<T, MODE>(fg, as, qk);
   store_partials<VEC, LPR, WAVES, NC, NQ, NQA>(p, lb, wave, grp, col, tid, as, ac, aq);
 }
 
