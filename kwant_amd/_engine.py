@@ -492,11 +492,13 @@ class BlockRecursion:
     a_n when ``store_all`` (Correlator expansions)."""
 
     def __init__(self, mat, ab, vectors, width, mode, w=None, store_slots=None,
-                 variant=-1, plan=None, wrap=False):
+                 variant=-1, plan=None, wrap=False, slot_pad=0):
         """``wrap``: a ring of ``store_slots`` slots that the caller drains
         as it goes -- a_k lives in slot k mod slots, and the steps may run
         past the ring's end (the caller uses each a_k before slots steps
-        overwrite it)."""
+        overwrite it).  ``slot_pad``: elements of padding after each slot, so
+        that slots read together as the rows of a GEMM are not a power of two
+        apart (the ring is then a strided (slots, N, W) view)."""
         import torch
         self.wrap = bool(wrap)
         dev = torch_device()
@@ -512,7 +514,12 @@ class BlockRecursion:
         self.slots = 2 if store_slots is None else int(store_slots)
         if self.slots < 2:
             raise ValueError("a ring needs at least two slots")
-        self.ring = torch.zeros((self.slots, n, width), dtype=self.cdt, device=dev)
+        self.pitch = n * width + int(slot_pad)
+        if slot_pad:
+            flat = torch.zeros((self.slots, self.pitch), dtype=self.cdt, device=dev)
+            self.ring = flat[:, :n * width].view(self.slots, n, width)
+        else:
+            self.ring = torch.zeros((self.slots, n, width), dtype=self.cdt, device=dev)
         self.ring[0, :, :vectors.shape[1]] = vectors
         if mode == DOTS_DOUBLING:
             self.plan.set_dots_grid(self.ring[0])
@@ -522,7 +529,7 @@ class BlockRecursion:
 
     @property
     def slot_stride(self):
-        return self.n_rows * self.width
+        return self.pitch
 
     def step(self, n_steps, want_dots=True):
         """Advance ``n_steps``; returns the (n_steps, W, 4) FP64 dots on host."""

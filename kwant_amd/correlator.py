@@ -270,8 +270,8 @@ class Correlator:
             dst.copy_(src)
         elif kind == 'matrix':
             self._device_op(key, kind, payload).spmm(
-                src, dst, width, n_slots=n_slots, x_stride=src[0].numel(),
-                y_stride=dst[0].numel())
+                src, dst, width, n_slots=n_slots, x_stride=src.stride(0),
+                y_stride=dst.stride(0))
         else:
             dmat = self._spectrum_R.dmat
             host = src.cpu().numpy().astype(np.complex128)
@@ -288,6 +288,10 @@ class Correlator:
     # per GEMM: Psi stays resident (num_moments slots), each expansion runs
     # through a ring of this many slots that is drained as it fills
     OMEGA_CHUNK = 256
+    # elements of padding after every ring slot and Psi row: the GEMM reads
+    # 64 of them at the same k, and rows a power of two apart (N x W
+    # elements) all fall on one L2 set and channel
+    SLOT_PAD = 32
 
     def _stream(self, ab, start, width, m_count, consume):
         """Expand ``start`` (an (N, width) device block) to m_count Chebyshev
@@ -296,7 +300,8 @@ class Correlator:
         view of the ring holding T_m0 .. T_{m1-1} -- in order."""
         chunk = max(3, min(m_count, self.OMEGA_CHUNK))
         rec = eng.BlockRecursion(self._spectrum_R.dmat, ab, start, width, eng.DOTS_NONE,
-                                 store_slots=chunk, wrap=True, plan=self._plan_for(width))
+                                 store_slots=chunk, wrap=True, plan=self._plan_for(width),
+                                 slot_pad=self.SLOT_PAD)
         consumed, produced = 0, 1
         while consumed < m_count:
             target = min(m_count, consumed + chunk)
@@ -374,9 +379,12 @@ class Correlator:
             _check_finite_vectors(host)
             return np.ascontiguousarray(dmat.rows_in(host))
 
-        # Psi storage for the widest block, reused by every block (views);
-        # one launch plan per block width, shared by all expansions
-        psi_store = torch.empty(m_count * n * max(widths, default=1), dtype=cdt, device=dev)
+        # Psi storage for the widest block, reused by every block (views with
+        # padded rows); one launch plan per block width, shared by all
+        # expansions
+        pad = self.SLOT_PAD
+        psi_store = torch.empty(m_count * (n * max(widths, default=1) + pad), dtype=cdt,
+                                device=dev)
         start = r0
         host_next = draw(start, widths[0]) if widths else None
         for bi, width in enumerate(widths):
@@ -384,7 +392,9 @@ class Correlator:
             start += width
             block = torch.from_numpy(host).to(dev).to(cdt)
             # Psi_n = op2 T_n(H~)|r>, resident for the whole block
-            psi = psi_store[:m_count * n * width].view(m_count, n, width)
+            k = n * width
+            psi_rows = psi_store[:m_count * (k + pad)].view(m_count, k + pad)[:, :k]
+            psi = psi_rows.view(m_count, n, width)
 
             def to_psi(m0, m1, slots):
                 with self._timed('operator_s'):
@@ -400,11 +410,11 @@ class Correlator:
             with self._timed('operator_s'):
                 self._apply('op1', self._op1, block.unsqueeze(0), start_l.unsqueeze(0), width, 1)
             mu = torch.zeros((m_count, m_count), dtype=torch.complex128, device=dev)
-            k = n * width
-            psi_k = psi.reshape(m_count, k)
+            psi_k = psi_rows
 
             def gemm(m0, m1, slots):
-                a = slots.reshape(m1 - m0, k)
+                # the ring slots as GEMM rows: k elements each, padded pitch
+                a = slots.as_strided((m1 - m0, k), (slots.stride(0), 1))
                 with self._timed('gemm_s'):
                     if isinstance(gp, tuple):
                         self._gemm_work = eng.corr_gemm_sliced(
@@ -414,7 +424,7 @@ class Correlator:
                         eng.corr_gemm(a, psi_k, mu[m0:m1], gp)
 
             self._stream((self._aL, self._bL), start_l, width, m_count, gemm)
-            del psi, psi_k
+            del psi, psi_k, psi_rows
             if self.mean:
                 self._mu_sum = mu if self._mu_sum is None else self._mu_sum + mu
             else:
@@ -433,6 +443,8 @@ class Correlator:
         memory."""
         import torch
         free, _ = torch.cuda.mem_get_info()
+        # memory that torch's allocator holds but does not use is free for us
+        free += torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
         slots = m_count + min(m_count, self.OMEGA_CHUNK) + 2
         per_width = slots * n * torch.empty((), dtype=cdt).element_size()
         width = self._max_width

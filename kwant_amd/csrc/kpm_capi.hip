@@ -289,17 +289,19 @@ int kpm_plan_set_dots_grid(kpm_plan* plan, const void* a0, kpm_stream_t stream) 
   if (!plan || !a0) return fail(KPM_ERR_ARG, "NULL argument");
   if (!plan->scale_set) return fail(KPM_ERR_ARG, "scale not set (kpm_plan_set_scale)");
   hipStream_t s = (hipStream_t)stream;
-  // <a_0|a_0> per segment and column: a sum of non-negative terms, so the
-  // TwoSum result is its correctly rounded value whatever the launch shape
+  // <a_0|a_0> per segment and column, as Re <w|x> of the projection mode with
+  // w = x = a_0 (record slot 0): a TwoSum of non-negative terms, so its
+  // correctly rounded value whatever the launch shape
   StepParams p{};
   p.ring = reinterpret_cast<char*>(const_cast<void*>(a0));
   p.slot_bytes = 0;
   p.slots = 1;
+  p.w = a0;
   p.dots = plan->grid_dots;
   fill_params(plan, p);
   p.im_exp = nullptr;
-  int rc = launch_cheb(plan->value_type, plan->shape, KPM_DOTS_DOUBLING, true, 0, 0, p, s);
-  if (!rc) rc = launch_reduce(KPM_DOTS_DOUBLING, plan->width, p, s);
+  int rc = launch_cheb(plan->value_type, plan->shape, KPM_DOTS_PROJECT, true, 0, 0, p, s);
+  if (!rc) rc = launch_reduce(KPM_DOTS_PROJECT, plan->width, p, s);
   const int64_t cols = int64_t(plan->nseg) * plan->width;
   std::vector<double> dots(static_cast<size_t>(cols) * 4);
   std::vector<int32_t> kexp(static_cast<size_t>(cols));
@@ -493,6 +495,8 @@ int kpm_block_dots(kpm_plan* plan, const void* x, const void* w, double* d_out,
   if ((dots_mode == KPM_DOTS_PROJECT || local) && !w)
     return fail(KPM_ERR_ARG, "projection vector missing");
   if (!plan->scale_set) return fail(KPM_ERR_ARG, "scale not set (kpm_plan_set_scale)");
+  if (dots_mode == KPM_DOTS_DOUBLING && !plan->grid_set)
+    return fail(KPM_ERR_ARG, "dots grid not set (kpm_plan_set_dots_grid)");
   StepParams p{};
   p.ring = reinterpret_cast<char*>(const_cast<void*>(x));
   p.slot_bytes = 0;

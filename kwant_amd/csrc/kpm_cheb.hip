@@ -614,8 +614,9 @@ __global__ __launch_bounds__(WAVES * 64) void cheb_kernel(StepParams p,
     for (int c = 0; c < NQA; ++c) aq[c][v] = 0.0;
     qk[v] = NQ > 0 && p.im_exp != nullptr ? p.im_exp[seg * W + col + v] : 0;
   }
-  const bool fg = MODE == KPM_DOTS_DOUBLING && std::is_same<T, float>::value &&
-                  p.im_exp != nullptr;
+  // complex64 doubling steps always run with the grid set (the C entry points
+  // require it; <a_0|a_0> itself is taken in the projection mode)
+  constexpr bool fg = MODE == KPM_DOTS_DOUBLING && std::is_same<T, float>::value;
 
   if constexpr (RPW <= 2) {
     // In each chunk a wave owns a contiguous range of ipc * RPW rows and walks
@@ -1003,8 +1004,9 @@ __global__ __launch_bounds__(WAVES * 64) void bsr_kernel(StepParams p,
     for (int c = 0; c < NQA; ++c) aq[c][v] = 0.0;
     qk[v] = NQ > 0 && p.im_exp != nullptr ? p.im_exp[seg * W + col + v] : 0;
   }
-  const bool fg = MODE == KPM_DOTS_DOUBLING && std::is_same<T, float>::value &&
-                  p.im_exp != nullptr;
+  // complex64 doubling steps always run with the grid set (the C entry points
+  // require it; <a_0|a_0> itself is taken in the projection mode)
+  constexpr bool fg = MODE == KPM_DOTS_DOUBLING && std::is_same<T, float>::value;
 
   for (int it = 0; it < iters; ++it) {
     const int64_t r64 = iter_row0(it, rr.x, stride, ipc, WAVES * RPW) + wave * RPW;

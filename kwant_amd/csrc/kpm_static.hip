@@ -243,16 +243,18 @@ extern "C" int kpm_cheb_steps_static(const kpm_csr* h, double a, double b, int32
   if (rc) return rc;
 
   if (dots_mode == KPM_DOTS_DOUBLING && first_step == 0) {
-    // the grid of the exact sums from <a_0|a_0> (ring slot 0), on the device
+    // the grid of the exact sums from <a_0|a_0> (ring slot 0), on the device:
+    // Re <w|x> of the projection mode with w = x = a_0 (record slot 0)
     StepParams q = p;
     q.ring = reinterpret_cast<char*>(ring);
     q.slot_bytes = 0;
     q.slots = 1;
+    q.w = ring;
     q.part = part;
     q.im_exp = nullptr;
     q.dots = reinterpret_cast<double*>(base + o_grid);
-    rc = launch_cheb(h->value_type, shape, KPM_DOTS_DOUBLING, true, 0, 0, q, s);
-    if (!rc) rc = launch_reduce(KPM_DOTS_DOUBLING, width, q, s);
+    rc = launch_cheb(h->value_type, shape, KPM_DOTS_PROJECT, true, 0, 0, q, s);
+    if (!rc) rc = launch_reduce(KPM_DOTS_PROJECT, width, q, s);
     if (rc) return rc;
     hipLaunchKernelGGL(static_grid_kernel, dim3((width + 63) / 64), dim3(64), 0, s, q.dots,
                        int(width), grid_exp);
