@@ -323,6 +323,27 @@ int kpm_corr_gemm_islice(const void* A, int64_t lda, const void* B, int64_t ldb,
                          int64_t k, int32_t slices, void* work, int64_t work_bytes,
                          kpm_stream_t stream);
 
+/* The same product with B cut into slices once and reused: the Correlator
+ * multiplies Psi (kwant/kpm.py:708-718) by every chunk of Omega
+ * (kwant/kpm.py:685-695).  kpm_islice_prep_rows slices rows
+ * [row0, row0 + rows) of an n_rows x k operand (B points at row row0) into
+ * `prep` (kpm_islice_prep_bytes bytes); rows may be filled in any order and
+ * call, each row independently (its own exponent per k-panel).  Then
+ * kpm_corr_gemm_islice_prep multiplies an m_rows x k A by all n_rows prepared
+ * rows: mu (m_rows x n_rows) += conj(A) B^T, with a work buffer of
+ * kpm_corr_gemm_islice_prep_work bytes.  Results are bitwise those of
+ * kpm_corr_gemm_islice on the same operands.  Asynchronous. */
+int kpm_islice_prep_bytes(int32_t n_rows, int64_t k, int32_t slices, int64_t* bytes);
+int kpm_islice_prep_rows(const void* B, int64_t ldb, int32_t row0, int32_t rows,
+                         int32_t n_rows, int64_t k, int32_t slices, void* prep,
+                         int64_t prep_bytes, kpm_stream_t stream);
+int kpm_corr_gemm_islice_prep_work(int32_t m_rows, int32_t n_rows, int64_t k,
+                                   int32_t slices, int64_t* bytes);
+int kpm_corr_gemm_islice_prep(const void* A, int64_t lda, int32_t m_rows, const void* prep,
+                              int64_t prep_bytes, int32_t n_rows, int64_t k, int32_t slices,
+                              double* mu, int64_t ldc, void* work, int64_t work_bytes,
+                              kpm_stream_t stream);
+
 /* Exact moments after a uniform on-site shift (no counterpart in kwant/kpm.py;
  * it acts on the moments kwant/kpm.py:436-477 produces).  nu holds moments
  * <T_k(t)> in a window t = (E - b') / a'; the moments in the window

@@ -40,7 +40,9 @@ EXPORTS = (
     "kpm_plan_set_dots_grid", "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps",
     "kpm_cheb_steps_static", "kpm_bsr_steps", "kpm_block_dots",
     "kpm_bond_records", "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
-    "kpm_corr_gemm_islice_work", "kpm_corr_gemm_islice", "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
+    "kpm_corr_gemm_islice_work", "kpm_corr_gemm_islice", "kpm_islice_prep_bytes",
+    "kpm_islice_prep_rows", "kpm_corr_gemm_islice_prep_work", "kpm_corr_gemm_islice_prep",
+    "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
     "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
     "kpm_mlp_ensemble", "kpm_mlp_adam",
 )
@@ -144,6 +146,12 @@ def load_library(path=LIB_PATH):
                                                      ctypes.POINTER(ctypes.c_int64)]),
         "kpm_corr_gemm_islice": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, i32, i32, i64, i32,
                                                 vp, i64, vp]),
+        "kpm_islice_prep_bytes": (ctypes.c_int, [i32, i64, i32, ctypes.POINTER(ctypes.c_int64)]),
+        "kpm_islice_prep_rows": (ctypes.c_int, [vp, i64, i32, i32, i32, i64, i32, vp, i64, vp]),
+        "kpm_corr_gemm_islice_prep_work": (ctypes.c_int, [i32, i32, i64, i32,
+                                                          ctypes.POINTER(ctypes.c_int64)]),
+        "kpm_corr_gemm_islice_prep": (ctypes.c_int, [vp, i64, i32, vp, i64, i32, i64, i32, vp,
+                                                     i64, vp, i64, vp]),
         "kpm_moment_shift": (ctypes.c_int, [vp, i64, i64, i32, i64, i32, vp, vp,
                                             ctypes.c_double, vp, vp, i64, i64, vp]),
         "kpm_cubic4_nnz": (ctypes.c_int, [i32, ctypes.POINTER(ctypes.c_int64)]),
@@ -649,6 +657,55 @@ def corr_gemm_sliced(a, b, mu, slices=4, work=None, kind="bf16"):
                   work.data_ptr(), work.numel() * work.element_size(),
                   stream_handle()), name)
     return work
+
+
+class IsliceOperand:
+    """An n_rows x k complex128 operand cut once into ``slices`` int8 slices
+    per component (kpm_islice_prep_rows), for many products
+    mu += conj(a) @ b.T on the INT8 matrix cores (``gemm``)."""
+
+    def __init__(self, n_rows, k, slices, device=None):
+        import torch
+        self.n_rows, self.k, self.slices = int(n_rows), int(k), int(slices)
+        need = ctypes.c_int64()
+        check(load_library().kpm_islice_prep_bytes(self.n_rows, self.k, self.slices,
+                                                   ctypes.byref(need)), "kpm_islice_prep_bytes")
+        self.data = torch.empty(need.value, dtype=torch.uint8,
+                                device=device if device is not None else torch_device())
+        self.work = None
+
+    def fill_rows(self, rows, row0):
+        """Slice ``rows`` (r x k complex128, unit stride along k) into rows
+        row0 .. row0 + r - 1."""
+        import torch
+        if rows.dtype != torch.complex128 or rows.stride(1) != 1 or rows.shape[1] != self.k:
+            raise ValueError("rows must be r x k complex128 with unit stride along k")
+        check(load_library().kpm_islice_prep_rows(
+            rows.data_ptr(), rows.stride(0), int(row0), rows.shape[0], self.n_rows, self.k,
+            self.slices, self.data.data_ptr(), self.data.numel(), stream_handle()),
+            "kpm_islice_prep_rows")
+
+    def gemm(self, a, mu):
+        """mu (m x n_rows) += conj(a) @ b.T, a: m x k complex128."""
+        import torch
+        m, k = a.shape
+        if k != self.k or tuple(mu.shape) != (m, self.n_rows):
+            raise ValueError("IsliceOperand.gemm shape mismatch")
+        if a.dtype != torch.complex128 or mu.dtype != torch.complex128:
+            raise ValueError("IsliceOperand.gemm takes complex128 operands and output")
+        if a.stride(1) != 1 or mu.stride(1) != 1:
+            raise ValueError("operands must be contiguous along k")
+        lib = load_library()
+        need = ctypes.c_int64()
+        check(lib.kpm_corr_gemm_islice_prep_work(m, self.n_rows, k, self.slices,
+                                                 ctypes.byref(need)),
+              "kpm_corr_gemm_islice_prep_work")
+        if self.work is None or self.work.numel() < need.value:
+            self.work = torch.empty(need.value, dtype=torch.uint8, device=mu.device)
+        check(lib.kpm_corr_gemm_islice_prep(
+            a.data_ptr(), a.stride(0), m, self.data.data_ptr(), self.data.numel(), self.n_rows,
+            k, self.slices, mu.data_ptr(), mu.stride(0), self.work.data_ptr(),
+            self.work.numel(), stream_handle()), "kpm_corr_gemm_islice_prep")
 
 
 def moment_shift(nu, alpha, beta, out, in_scale=None, out_scale=None):

@@ -381,9 +381,13 @@ class Correlator:
 
         # Psi storage for the widest block, reused by every block (views with
         # padded rows); one launch plan per block width, shared by all
-        # expansions
+        # expansions.  With the INT8 GEMM, Psi is kept only as its int8
+        # slices (prepared once per block, reused by every chunk of Omega):
+        # each run of Psi rows passes through a buffer of OMEGA_CHUNK rows.
         pad = self.SLOT_PAD
-        psi_store = torch.empty(m_count * (n * max(widths, default=1) + pad), dtype=cdt,
+        prepared = isinstance(gp, tuple) and gp[0] == 'int8'
+        psi_slots = min(m_count, self.OMEGA_CHUNK) if prepared else m_count
+        psi_store = torch.empty(psi_slots * (n * max(widths, default=1) + pad), dtype=cdt,
                                 device=dev)
         start = r0
         host_next = draw(start, widths[0]) if widths else None
@@ -393,12 +397,23 @@ class Correlator:
             block = torch.from_numpy(host).to(dev).to(cdt)
             # Psi_n = op2 T_n(H~)|r>, resident for the whole block
             k = n * width
-            psi_rows = psi_store[:m_count * (k + pad)].view(m_count, k + pad)[:, :k]
-            psi = psi_rows.view(m_count, n, width)
+            psi_rows = psi_store[:psi_slots * (k + pad)].view(psi_slots, k + pad)[:, :k]
+            psi = psi_rows.view(psi_slots, n, width)
+            if prepared:
+                prep = getattr(self, '_psi_prep', None)
+                if prep is None or prep.k != k:
+                    self._psi_prep = prep = None
+                    prep = self._psi_prep = eng.IsliceOperand(m_count, k, gp[1])
 
             def to_psi(m0, m1, slots):
-                with self._timed('operator_s'):
-                    self._apply('op2', self._op2, slots, psi[m0:m1], width, m1 - m0)
+                if prepared:
+                    with self._timed('operator_s'):
+                        self._apply('op2', self._op2, slots, psi[:m1 - m0], width, m1 - m0)
+                    with self._timed('gemm_s'):
+                        prep.fill_rows(psi_rows[:m1 - m0], m0)
+                else:
+                    with self._timed('operator_s'):
+                        self._apply('op2', self._op2, slots, psi[m0:m1], width, m1 - m0)
 
             self._stream((self._a, self._b), block, width, m_count, to_psi)
             # the next block's vectors are drawn on the host while the GPU
@@ -416,7 +431,9 @@ class Correlator:
                 # the ring slots as GEMM rows: k elements each, padded pitch
                 a = slots.as_strided((m1 - m0, k), (slots.stride(0), 1))
                 with self._timed('gemm_s'):
-                    if isinstance(gp, tuple):
+                    if prepared:
+                        prep.gemm(a, mu[m0:m1])
+                    elif isinstance(gp, tuple):
                         self._gemm_work = eng.corr_gemm_sliced(
                             a, psi_k, mu[m0:m1], slices=gp[1], kind=gp[0],
                             work=getattr(self, '_gemm_work', None))

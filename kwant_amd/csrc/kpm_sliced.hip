@@ -348,12 +348,13 @@ constexpr int kBufferWord3 = 0x00020000;
 constexpr int kIPad = 256;
 
 // S int8 slices of every component of the panel, 4 components per thread,
-// rows padded to kp2 with zeros: out[(s * rows + r) * ldk + k'].
+// rows padded to kp2 with zeros: out[(s * slice_rows + r) * ldk + k'] (the
+// rows may be a part of a larger sliced operand of slice_rows rows).
 __global__ __launch_bounds__(256) void islice_kernel(const double* __restrict__ X, int64_t ld,
                                                      int rows, int64_t k0, int64_t kn,
                                                      const int* __restrict__ e, int S,
                                                      int8_t* __restrict__ out, int64_t kp2,
-                                                     int64_t ldk) {
+                                                     int64_t ldk, int64_t slice_rows) {
   const int r = blockIdx.y;
   const int64_t c = 4 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
   if (c >= kp2 || r >= rows) return;
@@ -382,7 +383,7 @@ __global__ __launch_bounds__(256) void islice_kernel(const double* __restrict__ 
       packed |= (uint32_t(int(q)) & 0xFFu) << (8 * j);
       v[j] = (v[j] - q) * 128.0;
     }
-    *reinterpret_cast<uint32_t*>(out + (int64_t(s) * rows + r) * ldk + c) = packed;
+    *reinterpret_cast<uint32_t*>(out + (int64_t(s) * slice_rows + r) * ldk + c) = packed;
   }
 }
 
@@ -697,9 +698,9 @@ int kpm_corr_gemm_islice(const void* A, int64_t lda, const void* B, int64_t ldb,
     hipLaunchKernelGGL(exponent_decode, dim3((n_rows + 255) / 256), dim3(256), 0, s, eb, n_rows);
     const unsigned sg = unsigned((p.kp2 / 4 + 255) / 256);
     hipLaunchKernelGGL(islice_kernel, dim3(sg, m_rows), dim3(256), 0, s, a, lda, m_rows, k0, kn,
-                       ea, int(slices), As, p.kp2, p.ldk);
+                       ea, int(slices), As, p.kp2, p.ldk, int64_t(m_rows));
     hipLaunchKernelGGL(islice_kernel, dim3(sg, n_rows), dim3(256), 0, s, b, ldb, n_rows, k0, kn,
-                       eb, int(slices), Bs, p.kp2, p.ldk);
+                       eb, int(slices), Bs, p.kp2, p.ldk, int64_t(n_rows));
     const int slot0 = panel_no * int(p.zs);
     switch (slices) {
       case 1: launch_islice_gemm<1>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
@@ -713,6 +714,129 @@ int kpm_corr_gemm_islice(const void* A, int64_t lda, const void* B, int64_t ldb,
     if (rc) return rc;
   }
   // the (panel, split) slots in order: deterministic
+  const int64_t total = int64_t(m_rows) * n_rows;
+  hipLaunchKernelGGL(add_partials, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, part,
+                     panel_no * int(p.zs), mu, ldc, m_rows, n_rows);
+  return check_hip(hipGetLastError(), "int8-sliced GEMM partial sum");
+}
+
+// A prepared (pre-sliced) B operand: per panel, the rows' exponents (a
+// 256-byte aligned block of n_rows int32) and then S x n_rows slice rows of
+// ldk bytes.  The layout depends on (n_rows, k, slices) only.
+static int64_t prep_panel_bytes(int n_rows, const SlicedPlan& p, int slices) {
+  const int64_t exp_block = (int64_t(n_rows) * 4 + 255) / 256 * 256;
+  return exp_block + int64_t(slices) * n_rows * p.ldk;
+}
+
+int kpm_islice_prep_bytes(int32_t n_rows, int64_t k, int32_t slices, int64_t* bytes) {
+  if (!bytes || n_rows <= 0 || k <= 0 || slices < 1 || slices > kMaxISlices)
+    return fail(KPM_ERR_ARG, "bad prepared-operand shape");
+  if (n_rows > 65535) return fail(KPM_ERR_ARG, "at most 65535 rows per operand");
+  const SlicedPlan p = islice_plan(1, n_rows, k, slices);
+  const int64_t panels = (k + p.panel - 1) / p.panel;
+  *bytes = panels * prep_panel_bytes(n_rows, p, slices);
+  return KPM_OK;
+}
+
+int kpm_islice_prep_rows(const void* B, int64_t ldb, int32_t row0, int32_t rows, int32_t n_rows,
+                         int64_t k, int32_t slices, void* prep, int64_t prep_bytes,
+                         kpm_stream_t stream) {
+  if (!B || !prep) return fail(KPM_ERR_ARG, "NULL buffer");
+  if (n_rows <= 0 || n_rows > 65535 || k <= 0 || slices < 1 || slices > kMaxISlices)
+    return fail(KPM_ERR_ARG, "bad prepared-operand shape");
+  if (rows <= 0 || row0 < 0 || row0 + rows > n_rows)
+    return fail(KPM_ERR_ARG, "rows outside the prepared operand");
+  if (ldb < k) return fail(KPM_ERR_ARG, "leading dimension too small");
+  const SlicedPlan p = islice_plan(1, n_rows, k, slices);
+  const int64_t panels = (k + p.panel - 1) / p.panel;
+  const int64_t pb = prep_panel_bytes(n_rows, p, slices);
+  if (prep_bytes < panels * pb)
+    return fail(KPM_ERR_ARG, "prepared operand too small (kpm_islice_prep_bytes)");
+  hipStream_t s = (hipStream_t)stream;
+  const double* b = static_cast<const double*>(B);
+  const int64_t exp_block = (int64_t(n_rows) * 4 + 255) / 256 * 256;
+  int panel_no = 0;
+  for (int64_t k0 = 0; k0 < k; k0 += p.panel, ++panel_no) {
+    const int64_t kn = k - k0 < p.panel ? k - k0 : p.panel;
+    char* base = static_cast<char*>(prep) + panel_no * pb;
+    int* e = reinterpret_cast<int*>(base) + row0;
+    int8_t* out = reinterpret_cast<int8_t*>(base + exp_block) + int64_t(row0) * p.ldk;
+    const unsigned spans = unsigned((2 * kn + kExpSpan - 1) / kExpSpan);
+    hipLaunchKernelGGL(exponent_clear, dim3((rows + 255) / 256), dim3(256), 0, s, e, rows);
+    hipLaunchKernelGGL(row_exponent_kernel, dim3(spans, rows), dim3(256), 0, s, b, ldb, rows, k0,
+                       kn, e);
+    hipLaunchKernelGGL(exponent_decode, dim3((rows + 255) / 256), dim3(256), 0, s, e, rows);
+    const unsigned sg = unsigned((p.kp2 / 4 + 255) / 256);
+    hipLaunchKernelGGL(islice_kernel, dim3(sg, rows), dim3(256), 0, s, b, ldb, rows, k0, kn, e,
+                       int(slices), out, p.kp2, p.ldk, int64_t(n_rows));
+    int rc = check_hip(hipGetLastError(), "operand slicing launch");
+    if (rc) return rc;
+  }
+  return KPM_OK;
+}
+
+int kpm_corr_gemm_islice_prep_work(int32_t m_rows, int32_t n_rows, int64_t k, int32_t slices,
+                                   int64_t* bytes) {
+  if (!bytes || m_rows <= 0 || n_rows <= 0 || k <= 0 || slices < 1 || slices > kMaxISlices)
+    return fail(KPM_ERR_ARG, "bad int8-sliced GEMM shape");
+  if (m_rows > 65535 || n_rows > 65535)
+    return fail(KPM_ERR_ARG, "at most 65535 rows per operand");
+  const SlicedPlan p = islice_plan(m_rows, n_rows, k, slices);
+  *bytes = p.part_bytes + int64_t(m_rows) * 4 + int64_t(slices) * m_rows * p.ldk + 256;
+  return KPM_OK;
+}
+
+int kpm_corr_gemm_islice_prep(const void* A, int64_t lda, int32_t m_rows, const void* prep,
+                              int64_t prep_bytes, int32_t n_rows, int64_t k, int32_t slices,
+                              double* mu, int64_t ldc, void* work, int64_t work_bytes,
+                              kpm_stream_t stream) {
+  if (!A || !prep || !mu || !work) return fail(KPM_ERR_ARG, "NULL buffer");
+  if (m_rows <= 0 || n_rows <= 0 || k <= 0) return fail(KPM_ERR_ARG, "empty GEMM");
+  if (m_rows > 65535 || n_rows > 65535)
+    return fail(KPM_ERR_ARG, "at most 65535 rows per operand");
+  if (slices < 1 || slices > kMaxISlices) return fail(KPM_ERR_ARG, "slices must be 1..6");
+  if (lda < k || ldc < n_rows) return fail(KPM_ERR_ARG, "leading dimension too small");
+  const SlicedPlan p = islice_plan(m_rows, n_rows, k, slices);
+  const int64_t panels = (k + p.panel - 1) / p.panel;
+  const int64_t pb = prep_panel_bytes(n_rows, p, slices);
+  if (prep_bytes < panels * pb)
+    return fail(KPM_ERR_ARG, "prepared operand too small (kpm_islice_prep_bytes)");
+  if (work_bytes < p.part_bytes + int64_t(m_rows) * 4 + int64_t(slices) * m_rows * p.ldk + 256)
+    return fail(KPM_ERR_ARG, "work buffer too small (kpm_corr_gemm_islice_prep_work)");
+  hipStream_t s = (hipStream_t)stream;
+  char* w = static_cast<char*>(work);
+  double* part = reinterpret_cast<double*>(w);
+  int* ea = reinterpret_cast<int*>(w + p.part_bytes);
+  int8_t* As = reinterpret_cast<int8_t*>(
+      w + ((p.part_bytes + int64_t(m_rows) * 4 + 255) / 256) * 256);
+  const double* a = static_cast<const double*>(A);
+  const int64_t exp_block = (int64_t(n_rows) * 4 + 255) / 256 * 256;
+  int panel_no = 0;
+  for (int64_t k0 = 0; k0 < k; k0 += p.panel, ++panel_no) {
+    const int64_t kn = k - k0 < p.panel ? k - k0 : p.panel;
+    const char* base = static_cast<const char*>(prep) + panel_no * pb;
+    const int* eb = reinterpret_cast<const int*>(base);
+    const int8_t* Bs = reinterpret_cast<const int8_t*>(base + exp_block);
+    const unsigned spans = unsigned((2 * kn + kExpSpan - 1) / kExpSpan);
+    hipLaunchKernelGGL(exponent_clear, dim3((m_rows + 255) / 256), dim3(256), 0, s, ea, m_rows);
+    hipLaunchKernelGGL(row_exponent_kernel, dim3(spans, m_rows), dim3(256), 0, s, a, lda,
+                       m_rows, k0, kn, ea);
+    hipLaunchKernelGGL(exponent_decode, dim3((m_rows + 255) / 256), dim3(256), 0, s, ea, m_rows);
+    const unsigned sg = unsigned((p.kp2 / 4 + 255) / 256);
+    hipLaunchKernelGGL(islice_kernel, dim3(sg, m_rows), dim3(256), 0, s, a, lda, m_rows, k0, kn,
+                       ea, int(slices), As, p.kp2, p.ldk, int64_t(m_rows));
+    const int slot0 = panel_no * int(p.zs);
+    switch (slices) {
+      case 1: launch_islice_gemm<1>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      case 2: launch_islice_gemm<2>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      case 3: launch_islice_gemm<3>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      case 4: launch_islice_gemm<4>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      case 5: launch_islice_gemm<5>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+      default: launch_islice_gemm<6>(As, Bs, m_rows, n_rows, p, ea, eb, part, slot0, s); break;
+    }
+    int rc = check_hip(hipGetLastError(), "int8-sliced GEMM launch");
+    if (rc) return rc;
+  }
   const int64_t total = int64_t(m_rows) * n_rows;
   hipLaunchKernelGGL(add_partials, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, part,
                      panel_no * int(p.zs), mu, ldc, m_rows, n_rows);

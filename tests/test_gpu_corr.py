@@ -281,6 +281,31 @@ def test_int8_sliced_gemm_accuracy(slices, tol):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("m,n,k", [(70, 33, 5001), (17, 5, (1 << 20) + 4099)])
+def test_int8_prepared_operand_equals_direct(m, n, k):
+    """B sliced once (its rows filled in two calls, out of order) and
+    multiplied by two different A: bitwise the direct INT8 product, and
+    accumulated into an existing mu."""
+    import torch
+    from kwant_amd import _engine as eng
+    rng = np.random.RandomState(m)
+    b = torch.from_numpy(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))).cuda()
+    prep = eng.IsliceOperand(n, k, 5)
+    cut = n // 2
+    prep.fill_rows(b[cut:], cut)
+    prep.fill_rows(b[:cut], 0)
+    for seed in (1, 2):
+        a = torch.from_numpy(rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
+                             ).cuda() * (10.0 ** seed)
+        mu0 = torch.from_numpy(rng.standard_normal((m, n)) + 0j).cuda()
+        want = mu0.clone()
+        eng.corr_gemm_sliced(a, b, want, slices=5, kind="int8")
+        got = mu0.clone()
+        prep.gemm(a, got)
+        assert torch.equal(got, want), seed
+
+
+@pytest.mark.gpu
 def test_int8_sliced_gemm_panels_and_accumulate():
     """k longer than one panel (2^20) and accumulation into an existing mu."""
     import torch

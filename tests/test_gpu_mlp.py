@@ -395,7 +395,7 @@ def test_failed_member_is_left_out():
     tight = Fam()
     tight.a = 6.1
     sur = MomentSurrogate(Fam(), tight, members=4, layers=(2, 16, 32), seed=2)
-    sur.fit(params, targets, epochs=30, batch=32, lr=1e-2)
+    sur.fit(params, targets, epochs=30, batch=32, lr=1e-2, health=10.0)
     assert sur.active == 4
     # spoil member 1, then re-run the health check on the training data
     sur.mlp.weight(1)[1].mul_(50.0)
