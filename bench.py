@@ -7,16 +7,22 @@ t = 1, Anderson disorder U[-0.25, 0.25], 64 random-phase vectors in one
 N x 64 block, complex64 recursion with FP64 dot accumulation; a full
 expansion is 2048 moments = 1024 Chebyshev steps.
 
-A "step" is one launch of the fused recursion kernel over the whole block:
+A "step" is one Chebyshev step over the whole block:
 a_{n+1} = 2 (H a_n - b a_n)/a - a_{n-1} plus the two FP64 dot products per
-column (two moments per vector).  The timed region holds only kernel
+column (two moments per vector).  Two kernels run it: the one-step kernel (one
+launch per step, natural row order) and the two-step kernel (one launch per two
+steps over compact tiles of the flake: each tile's a_{n+1} and its halo's are
+formed in LDS, so a_{n+1} never makes a round trip through HBM).  Both are
+timed; the faster is the headline.  The timed region holds only kernel
 launches on the current HIP stream, bracketed by events; inputs are resident
 in HBM before it starts.
 
-Algorithmic bytes per step (the roofline numerator):
+Algorithmic bytes per step (the roofline numerator, per Chebyshev step of
+the path, whichever kernel runs it):
     B = 3 R N s_v + nnz (s_m + 4) + 4 (N + 1)
 with s_v = s_m = 8 bytes (complex64): read a_n once, read a_{n-1}, write
-a_{n+1}, read the CSR once.
+a_{n+1}, read the CSR once.  The two-step kernel's own bytes per launch
+(halo rows recomputed by every tile that needs them) are reported beside it.
 
 Also reported: complex64 moments against the complex128 GPU path on the same
 vectors (the complex128 path itself matches the CPU oracle to 1e-11 in
@@ -76,8 +82,9 @@ def stream_probe(log_fn):
 def read_traffic(variant):
     """Bytes per launch fetched past L2 plus written, measured with
     rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration) for this launch variant, if committed; else None."""
-    key = "traffic_bytes_per_launch variant %d:" % variant
+    calibration) for this launch shape ("variant V" or "tiled V"), if
+    committed; else None."""
+    key = "traffic_bytes_per_launch %s:" % variant
     try:
         for line in open(TRAFFIC_FILE):
             if line.startswith(key):
@@ -85,6 +92,94 @@ def read_traffic(variant):
     except OSError:
         pass
     return None
+
+
+def two_step_bench(h, pos, ab, block, width, warmup, steps, tile_rows, variant,
+                   one_step_variant, log_fn):
+    """The same expansion with two Chebyshev steps per launch: the flake's
+    rows in a tiled order (kwant_amd.ordering.tile_order, tiles of about
+    ``tile_rows`` rows), the vectors permuted alike.  Every launch shape whose
+    LDS holds a tile and its halo is timed briefly and the fastest runs the
+    timed steps; its dots are checked bitwise against one-step launches on the
+    same rows first."""
+    import numpy as np
+    import torch
+    from kwant_amd import _engine as eng
+    from kwant_amd.ordering import tile_order
+    n, nnz = h.shape[0], h.nnz
+    t0 = time.time()
+    tiling = tile_order(pos, tile_rows)
+    mat = eng.DeviceCSR(h, "complex64", order=tiling.order)
+    blk = block[torch.from_numpy(tiling.order).to(block.device)]
+    log_fn("two-step: %d tiles of <= %d rows (%.1fs)"
+           % (tiling.n_tiles, int(np.max(np.diff(tiling.tile_begin))), time.time() - t0))
+    shapes = [variant] if variant >= 0 else list(range(eng.tiled_variant_count(width,
+                                                                                eng.KPM_C64)))
+    tried = {}
+    best = None
+    for v in shapes:
+        try:
+            tp = eng.TiledPlan(mat, width, tiling.tile_begin, variant=v)
+        except RuntimeError as err:
+            log_fn("two-step shape %d: %s" % (v, err))
+            continue
+        rec = eng.BlockRecursion(mat, ab, blk, width, eng.DOTS_DOUBLING, tiles=tp)
+        rec.step(9)
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        rec.step(32)
+        end.record()
+        torch.cuda.synchronize()
+        tried[v] = start.elapsed_time(end) / 32
+        if best is None or tried[v] < tried[best[0]]:
+            best = (v, tp)
+        del rec
+    if best is None:
+        return None
+    v, tp = best
+    info = tp.info()
+    # bitwise against one-step launches on the same rows
+    check = 64
+    ref = eng.BlockRecursion(mat, ab, blk, width, eng.DOTS_DOUBLING,
+                             variant=max(one_step_variant, 0))
+    d_ref = ref.step(check)
+    del ref
+    rec = eng.BlockRecursion(mat, ab, blk, width, eng.DOTS_DOUBLING, tiles=tp)
+    d_two = rec.step(check)
+    bitwise = bool(np.array_equal(d_ref, d_two))
+    del rec
+    rec = eng.BlockRecursion(mat, ab, blk, width, eng.DOTS_DOUBLING, tiles=tp)
+    pairs = steps // 2
+    dots = torch.empty((warmup + 1 + 2 * pairs, 1, width, 4), dtype=torch.float64,
+                       device=block.device)
+    rec.step(warmup + 1 if warmup % 2 == 0 else warmup)   # odd: step 0 alone, then pairs
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    tp.steps(rec.plan, mat, rec.ring, rec.slot_stride, rec.slots, pairs, eng.DOTS_DOUBLING,
+             dots=dots, out_base=0)
+    end.record()
+    torch.cuda.synchronize()
+    ms_launch = start.elapsed_time(end) / pairs
+    assert rec.plan.get_step() == rec.n + 2 * pairs
+    kb = tp.kernel_bytes(nnz, 8, 8)
+    gw, waves, _ = eng.tiled_variant_info(width, eng.KPM_C64, v)
+    out = {"ms_per_step": ms_launch / 2, "ms_per_launch": ms_launch, "steps": 2 * pairs,
+           "variant": v, "group_width": gw, "waves": waves, "tile_rows": tile_rows,
+           "tiles": info["tiles"], "max_tile_rows": info["max_tile_rows"],
+           "halo_rows": info["halo_rows"], "gather_rows": info["gather_rows"],
+           "workgroups": info["workgroups"], "lds_bytes": info["lds_bytes"],
+           "kernel_bytes_per_launch": kb,
+           "kernel_GBs": kb / (ms_launch * 1e-3) / 1e9,
+           "kernel_frac_of_hbm_peak": kb / (ms_launch * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "shapes_ms_per_step": {str(k): t for k, t in tried.items()},
+           "dots_bitwise_vs_one_step": bitwise, "checked_steps": check}
+    log_fn("two-step shape %d: %.4f ms/step (%.4f ms per launch), kernel bytes %.3f GB per "
+           "launch = %.0f GB/s; dots bitwise vs one-step: %s"
+           % (v, ms_launch / 2, ms_launch, kb / 1e9, out["kernel_GBs"], bitwise))
+    del rec, dots, mat, blk
+    return out
 
 
 def moments_from_dots(dots):
@@ -785,6 +880,10 @@ def main():
     ap.add_argument("--cells", type=int, default=1000, help="unit cells per side")
     ap.add_argument("--width", type=int, default=64, help="vectors per block")
     ap.add_argument("--variant", type=int, default=-1, help="kernel launch shape")
+    ap.add_argument("--tile-rows", type=int, default=512,
+                    help="rows per tile of the two-step kernel (0: one-step kernel only)")
+    ap.add_argument("--tile-variant", type=int, default=-1,
+                    help="two-step launch shape (default: the fastest that fits)")
     ap.add_argument("--ens-variant", type=int, default=-1,
                     help="kernel launch shape of the ensemble (default: autotuned)")
     ap.add_argument("--ens-eager", action="store_true",
@@ -840,7 +939,7 @@ def main():
     from kwant_amd.lattices import honeycomb, honeycomb_bounds
 
     t0 = time.time()
-    h, _ = honeycomb(args.cells, disorder=0.5, seed=2)
+    h, pos = honeycomb(args.cells, disorder=0.5, seed=2)
     n, nnz, width = h.shape[0], h.nnz, args.width
     lmin, lmax = honeycomb_bounds(1.0, 0.5)
     eps = 0.05
@@ -943,13 +1042,27 @@ def main():
                                                                      args.variant)
     log("%.4f ms/step, %.1f steps/s, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
         % (ms_step, steps_s, achieved, 100 * achieved / HBM_PEAK_GBS))
+    one_step = {"ms_per_step": ms_step, "steps_per_s": steps_s, "achieved_GBs": achieved,
+                "frac": achieved / HBM_PEAK_GBS, "variant": args.variant}
 
     del rec, plan, dots
     torch.cuda.empty_cache()
+    two_step = None
+    if args.tile_rows > 0:
+        two_step = two_step_bench(h, pos, (a, b), block64, width, args.warmup, m_steps,
+                                  args.tile_rows, args.tile_variant, args.variant, log)
+        torch.cuda.empty_cache()
+    headline_two = two_step is not None and two_step["ms_per_step"] < ms_step
+    if headline_two:
+        ms_step = two_step["ms_per_step"]
+        steps_s = 1000.0 / ms_step
+        achieved = bstep / (ms_step * 1e-3) / 1e9
+        log("headline: two-step kernel, %.4f ms/step, %.1f steps/s (%.1f%% of 8 TB/s)"
+            % (ms_step, steps_s, 100 * achieved / HBM_PEAK_GBS))
     site_density = None
     if not args.no_density:
-        site_density = density_bench(m64, h, (a, b), block64, width, args.variant, ms_step,
-                                     256, log)
+        site_density = density_bench(m64, h, (a, b), block64, width, args.variant,
+                                     one_step["ms_per_step"], 256, log)
     del m64, block64
     torch.cuda.empty_cache()
     correlator = None
@@ -966,8 +1079,17 @@ def main():
     if not args.no_surrogate:
         surrogate = surrogate_bench(args.sur_epochs, args.sur_tol, args.sur_points, 256, 16, log)
 
-    traffic = read_traffic(args.variant)
+    traffic = read_traffic("tiled %d" % two_step["variant"] if headline_two
+                           else "variant %d" % args.variant)
     stream = stream_probe(log)
+    if headline_two:
+        kernel = {"two_step": True, "group_width": two_step["group_width"],
+                  "waves": two_step["waves"], "tile_rows": args.tile_rows,
+                  "tiled_variant": two_step["variant"], "tiles": two_step["tiles"]}
+    else:
+        kernel = {"two_step": False, "lanes_per_row": lpr, "complex_per_lane": vec,
+                  "waves": waves, "grid_rounds": rounds, "row_slots": unroll,
+                  "interleave_chunk": chunk, "spare_per_cu": spare, "variant": args.variant}
     line = {
         "metric": "KPM Chebyshev steps/s (% of HBM peak); surrogate speedup at 1e-3 DOS error",
         "value": steps_s,
@@ -985,16 +1107,15 @@ def main():
                                "U[-0.25,0.25], R=%d random-phase vectors in one block, "
                                "2048 moments per expansion, complex64 recursion + FP64 dots"
                                % (args.cells, args.cells, n, nnz, width),
-                   "N": n, "nnz": nnz, "block_width": width,
-                   "kernel": {"lanes_per_row": lpr, "complex_per_lane": vec, "waves": waves,
-                              "grid_rounds": rounds, "row_slots": unroll,
-                              "interleave_chunk": chunk, "spare_per_cu": spare,
-                              "variant": args.variant}},
+                   "N": n, "nnz": nnz, "block_width": width, "kernel": kernel},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_step": bstep,
+                     "steps_per_launch": 2 if headline_two else 1,
                      "measured_stream_GBs": stream,
                      "frac_of_measured_add": achieved / stream["add"]},
+        "one_step": one_step,
+        "two_step": two_step,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "small_square": small,

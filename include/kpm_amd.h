@@ -199,6 +199,55 @@ int kpm_bsr_steps(kpm_plan* plan, const kpm_bsr* h, void* ring, int64_t slot_str
                   int32_t slots, double* d_dots, int32_t out_base, int32_t out_cap,
                   int32_t n_steps, int32_t dots_mode, kpm_stream_t stream);
 
+/* Two Chebyshev steps per launch over tiles of consecutive rows (one
+ * segment).  The same recursion as kpm_cheb_steps (kwant/kpm.py:464-473 with
+ * the rescaled matvec of kwant/kpm.py:1182-1183), a_{n+1} and a_{n+2} from
+ * one launch: a workgroup computes a_{n+1} of its tile and of the tile's halo
+ * (the rows outside the tile that its entries reach) into LDS, then a_{n+2}
+ * of the tile from LDS, so the vectors cross HBM four times per two steps
+ * instead of six.  Every row of a_{n+1}, a_{n+2} and every dot is bitwise
+ * what two kpm_cheb_steps launches give.  Tiles pay off when the row order
+ * makes them compact patches of the lattice (few halo rows per tile row).
+ *
+ * kpm_tiled_create builds the tiling of a CSR matrix (host arrays indptr and
+ * indices, the same matrix later passed to kpm_cheb_steps2 in device memory):
+ * tile t owns rows [tile_row_begin[t], tile_row_begin[t+1]) (host array of
+ * n_tiles + 1), at most the variant's max_tile_rows each.  It computes each
+ * tile's halo and the LDS slot of every entry's column, and fails with
+ * KPM_ERR_UNSUPPORTED when a tile and its halo do not fit in LDS.  variant
+ * selects the launch shape (kpm_tiled_variant_info: columns per workgroup,
+ * waves per workgroup, the largest tile); -1 picks the default.  The tiling
+ * holds its own partials workspace; use one tiling per plan at a time.
+ * Replaces the per-vector loop of kwant/kpm.py:436-477 for the plain DOS
+ * path, like kpm_cheb_steps. */
+typedef struct kpm_tiled kpm_tiled;
+int kpm_tiled_variant_count(int32_t width, int32_t value_type, int32_t* count);
+int kpm_tiled_variant_info(int32_t width, int32_t value_type, int32_t variant,
+                           int32_t* group_width, int32_t* waves, int32_t* max_tile_rows);
+int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t nnz,
+                     const int32_t* h_indptr, const int32_t* h_indices, int32_t n_tiles,
+                     const int32_t* h_tile_row_begin, int32_t variant, kpm_tiled** out);
+int kpm_tiled_destroy(kpm_tiled* tiles);
+/* Tiles, the largest tile, the largest halo, all halo rows and their matrix
+ * entries, the rows of a_n the tiles gather (each tile's rows, its halo and
+ * the halo's own neighbours, summed over tiles), the persistent workgroups of
+ * a launch and the LDS bytes each uses. */
+int kpm_tiled_info(const kpm_tiled* tiles, int32_t* n_tiles, int32_t* max_tile_rows,
+                   int32_t* max_halo_rows, int64_t* halo_rows, int64_t* halo_entries,
+                   int64_t* gather_rows, int32_t* workgroups, int32_t* lds_bytes);
+
+/* n_pairs launches of two steps each, from the plan's step counter n >= 1
+ * (run step 0, and an odd last step, with kpm_cheb_steps), advancing it by
+ * 2 n_pairs.  The ring needs at least four slots (a_{n-1} .. a_{n+2} are read
+ * and written by one launch; the halo rows of one tile are other tiles' rows,
+ * so nothing is updated in place).  dots_mode KPM_DOTS_DOUBLING (records as
+ * kpm_cheb_steps, two per launch) or KPM_DOTS_NONE.  Plan: one segment, the
+ * tiling's row count, width and value type, scale (and, for the doubling
+ * mode, dots grid) set. */
+int kpm_cheb_steps2(kpm_plan* plan, kpm_tiled* tiles, const kpm_csr* h, void* ring,
+                    int64_t slot_stride, int32_t slots, double* d_dots, int32_t out_base,
+                    int32_t out_cap, int32_t n_pairs, int32_t dots_mode, kpm_stream_t stream);
+
 /* The steps of kpm_cheb_steps for one system (one segment) without a plan
  * and without allocating: no counterpart in the reference; for callers that
  * cannot call hipMalloc (code under HIP graph capture, tight serving loops).

@@ -38,7 +38,9 @@ EXPORTS = (
     "kpm_last_error", "kpm_abi_version", "kpm_device_info", "kpm_variant_count",
     "kpm_variant_info", "kpm_plan_create", "kpm_plan_destroy", "kpm_plan_set_scale",
     "kpm_plan_set_dots_grid", "kpm_plan_set_step", "kpm_plan_get_step", "kpm_cheb_steps",
-    "kpm_cheb_steps_static", "kpm_bsr_steps", "kpm_block_dots",
+    "kpm_cheb_steps_static", "kpm_bsr_steps", "kpm_tiled_variant_count",
+    "kpm_tiled_variant_info", "kpm_tiled_create", "kpm_tiled_destroy", "kpm_tiled_info",
+    "kpm_cheb_steps2", "kpm_block_dots",
     "kpm_bond_records", "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
     "kpm_corr_gemm_islice_work", "kpm_corr_gemm_islice", "kpm_islice_prep_bytes",
     "kpm_islice_prep_rows", "kpm_corr_gemm_islice_prep_work", "kpm_corr_gemm_islice_prep",
@@ -132,6 +134,15 @@ def load_library(path=LIB_PATH):
                                                  vp, i32, i32, vp, i32, i32, i32, vp]),
         "kpm_bsr_steps": (ctypes.c_int, [vp, ctypes.POINTER(KpmBsr), vp, i64, i32, vp, i32,
                                          i32, i32, i32, vp]),
+        "kpm_tiled_variant_count": (ctypes.c_int, [i32, i32, pi32]),
+        "kpm_tiled_variant_info": (ctypes.c_int, [i32, i32, i32, pi32, pi32, pi32]),
+        "kpm_tiled_create": (ctypes.c_int, [i32, i32, i32, i64, pi32, pi32, i32, pi32, i32,
+                                            ctypes.POINTER(vp)]),
+        "kpm_tiled_destroy": (ctypes.c_int, [vp]),
+        "kpm_tiled_info": (ctypes.c_int, [vp, pi32, pi32, pi32] +
+                           [ctypes.POINTER(ctypes.c_int64)] * 3 + [pi32, pi32]),
+        "kpm_cheb_steps2": (ctypes.c_int, [vp, vp, ctypes.POINTER(KpmCsr), vp, i64, i32, vp, i32,
+                                           i32, i32, i32, vp]),
         "kpm_block_dots": (ctypes.c_int, [vp, vp, vp, vp, i32, vp]),
         "kpm_bond_records": (ctypes.c_int, [ctypes.POINTER(KpmCsr), vp, vp, vp, vp, i32, i32,
                                             vp, vp]),
@@ -464,6 +475,87 @@ class Plan:
             mode, stream_handle()), "kpm_block_dots")
 
 
+def tiled_variant_count(width, value_type):
+    """Number of two-step (tiled) launch shapes for (width, value type)."""
+    n = ctypes.c_int32()
+    check(load_library().kpm_tiled_variant_count(width, value_type, ctypes.byref(n)),
+          "kpm_tiled_variant_count")
+    return n.value
+
+
+def tiled_variant_info(width, value_type, variant):
+    """(columns per workgroup, waves per workgroup, suggested tile rows)."""
+    vals = [ctypes.c_int32() for _ in range(3)]
+    check(load_library().kpm_tiled_variant_info(width, value_type, variant,
+                                                *[ctypes.byref(v) for v in vals]),
+          "kpm_tiled_variant_info")
+    return tuple(v.value for v in vals)
+
+
+class TiledPlan:
+    """Tiles of consecutive device rows of ``mat`` (tile t owns rows
+    tile_begin[t] .. tile_begin[t+1]-1) for the two-step launches
+    (kpm_cheb_steps2): the library computes each tile's halo and the LDS slot
+    of every entry from the matrix pattern (read back from the device)."""
+
+    def __init__(self, mat, width, tile_begin, variant=-1):
+        lib = load_library()
+        tb = np.ascontiguousarray(tile_begin, dtype=np.int32)
+        indptr = np.ascontiguousarray(mat.indptr.cpu().numpy(), dtype=np.int32)
+        indices = np.ascontiguousarray(mat.indices.cpu().numpy(), dtype=np.int32)
+        if indices.size == 0:
+            indices = np.zeros(1, dtype=np.int32)
+        pi32 = ctypes.POINTER(ctypes.c_int32)
+        handle = ctypes.c_void_p()
+        check(lib.kpm_tiled_create(int(mat.shape[0]), int(width), mat.value_type, mat.nnz,
+                                   indptr.ctypes.data_as(pi32), indices.ctypes.data_as(pi32),
+                                   len(tb) - 1, tb.ctypes.data_as(pi32), int(variant),
+                                   ctypes.byref(handle)), "kpm_tiled_create")
+        self.handle = handle
+        self.width, self.n_rows, self.value_type = int(width), int(mat.shape[0]), mat.value_type
+        self.tile_begin = tb
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            try:
+                _lib.kpm_tiled_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def info(self):
+        """Tiles, largest tile, largest halo, all halo rows and their entries,
+        the gathered rows of a_n, workgroups, LDS bytes."""
+        vals = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(),
+                ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()]
+        check(load_library().kpm_tiled_info(self.handle, *[ctypes.byref(v) for v in vals]),
+              "kpm_tiled_info")
+        keys = ("tiles", "max_tile_rows", "max_halo_rows", "halo_rows", "halo_entries",
+                "gather_rows", "workgroups", "lds_bytes")
+        return dict(zip(keys, (v.value for v in vals)))
+
+    def kernel_bytes(self, nnz, s_v, s_m):
+        """Bytes one two-step launch reads and writes if nothing is shared
+        between tiles: a_{n-1} of every tile's rows and halo, a_n of the rows
+        each tile gathers, a_{n+1} and a_{n+2} written, the CSR of the tile
+        and halo rows (phase A) and of the tile rows with their local indices
+        (phase B), the halo ids."""
+        i = self.info()
+        n, w = self.n_rows, self.width
+        rows_a = n + i["halo_rows"]
+        vec = w * s_v * (rows_a + i["gather_rows"] + 2 * n)
+        csr = 4 * rows_a + (nnz + i["halo_entries"]) * (4 + s_m) + 4 * n + nnz * (4 + s_m)
+        return vec + csr + 4 * i["halo_rows"]
+
+    def steps(self, plan, mat, ring, slot_stride, slots, n_pairs, mode, dots=None, out_base=0):
+        out_cap = 0 if dots is None else int(dots.shape[0])
+        check(load_library().kpm_cheb_steps2(
+            plan.handle, self.handle, ctypes.byref(mat.desc), ring.data_ptr(), slot_stride,
+            slots, None if dots is None else dots.data_ptr(), out_base, out_cap, n_pairs, mode,
+            stream_handle()), "kpm_cheb_steps2")
+
+
 def cheb_steps_static(mat, ab, ring, slots, first_step, n_steps, mode, dots=None,
                       out_base=0, w=None, grid_exp=None, variant=0):
     """kpm_cheb_steps_static: ``n_steps`` steps from step ``first_step`` of
@@ -500,15 +592,27 @@ class BlockRecursion:
     a_n when ``store_all`` (Correlator expansions)."""
 
     def __init__(self, mat, ab, vectors, width, mode, w=None, store_slots=None,
-                 variant=-1, plan=None, wrap=False, slot_pad=0):
+                 variant=-1, plan=None, wrap=False, slot_pad=0, tiles=None):
         """``wrap``: a ring of ``store_slots`` slots that the caller drains
         as it goes -- a_k lives in slot k mod slots, and the steps may run
         past the ring's end (the caller uses each a_k before slots steps
         overwrite it).  ``slot_pad``: elements of padding after each slot, so
         that slots read together as the rows of a GEMM are not a power of two
-        apart (the ring is then a strided (slots, N, W) view)."""
+        apart (the ring is then a strided (slots, N, W) view).  ``tiles``: a
+        TiledPlan of ``mat`` for this width; the steps then run two per launch
+        (step 0 and an odd last step one per launch) on a ring of four
+        slots."""
         import torch
         self.wrap = bool(wrap)
+        self.tiles = tiles
+        if tiles is not None:
+            if mode not in (DOTS_DOUBLING, DOTS_NONE):
+                raise ValueError("two-step launches take the doubling or no-dots mode")
+            if tiles.width != width or tiles.n_rows != mat.shape[0]:
+                raise ValueError("the tiling was built for another width or matrix")
+            if store_slots is None:  # a rolling ring: a_k in slot k mod 4
+                store_slots = 4
+                self.wrap = True
         dev = torch_device()
         n = mat.shape[0]
         self.mat, self.width, self.mode, self.n_rows = mat, width, mode, n
@@ -551,8 +655,21 @@ class BlockRecursion:
         if mode != DOTS_NONE:
             dots = torch.empty((n_steps, 1, self.width, 4), dtype=torch.float64,
                                device=self.ring.device)
-        self.plan.steps(self.mat, self.ring, self.slot_stride, self.slots, n_steps, mode,
-                        dots=dots, out_base=self.n, w=self.w)
+        if self.tiles is None:
+            self.plan.steps(self.mat, self.ring, self.slot_stride, self.slots, n_steps, mode,
+                            dots=dots, out_base=self.n, w=self.w)
+        else:
+            base, left = self.n, n_steps
+            if self.n == 0:  # a_1 has no a_{-1}: one step on its own
+                self.plan.steps(self.mat, self.ring, self.slot_stride, self.slots, 1, mode,
+                                dots=dots, out_base=base, w=self.w)
+                left -= 1
+            if left >= 2:
+                self.tiles.steps(self.plan, self.mat, self.ring, self.slot_stride, self.slots,
+                                 left // 2, mode, dots=dots, out_base=base)
+            if left % 2:
+                self.plan.steps(self.mat, self.ring, self.slot_stride, self.slots, 1, mode,
+                                dots=dots, out_base=base, w=self.w)
         self.n += n_steps
         if dots is None:
             return None

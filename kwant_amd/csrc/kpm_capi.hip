@@ -104,6 +104,30 @@ size_t elem_bytes(int value_type) { return value_type == KPM_C128 ? 16 : 8; }
 
 }  // namespace
 
+namespace kpm {
+
+int plan_state(const kpm_plan* plan, PlanState* out) {
+  if (!plan || !out) return fail(KPM_ERR_ARG, "plan is NULL");
+  out->n_rows = plan->n_rows;
+  out->width = plan->width;
+  out->value_type = plan->value_type;
+  out->n_segments = plan->nseg;
+  out->step = plan->step;
+  out->scale_set = plan->scale_set;
+  out->grid_set = plan->grid_set;
+  out->scale = plan->scale;
+  out->im_exp = plan->im_exp;
+  return KPM_OK;
+}
+
+int plan_advance(kpm_plan* plan, int by) {
+  if (!plan) return fail(KPM_ERR_ARG, "plan is NULL");
+  plan->step += by;
+  return KPM_OK;
+}
+
+}  // namespace kpm
+
 extern "C" {
 
 const char* kpm_last_error(void) { return g_last_error.c_str(); }

@@ -113,6 +113,18 @@ constexpr int kMaxVariants = 96;
 int variants_for(int width, int value_type, KernelShape* out, int max_out);
 int blocks_per_cu(int value_type, const KernelShape& ks, int* out);
 
+// The state of a plan that the tiled two-step launches (kpm_tiled.hip)
+// share with kpm_cheb_steps: shape, rescaling, dots grid and step counter.
+struct PlanState {
+  int32_t n_rows, width, value_type, n_segments, step;
+  bool scale_set, grid_set;
+  const double* scale;     // device (a, b) per segment
+  const int32_t* im_exp;   // device grid exponents per segment and column
+};
+int plan_state(const kpm_plan* plan, PlanState* out);
+// Advance the plan's step counter by `by` steps.
+int plan_advance(kpm_plan* plan, int by);
+
 int launch_corr_gemm(const void* A, int64_t lda, const void* B, int64_t ldb,
                      double* mu, int64_t ldc, int m_rows, int n_rows, int64_t k,
                      int precision, int k_chunk, double* work, int64_t work_elems,

@@ -24,7 +24,10 @@ launch shape, see kwant_amd.autotune), ``reorder`` (True | False | 'auto' |
 a permutation: the device row order, see kwant_amd.ordering; the moments do
 not depend on it), ``matrix_format`` ('csr' | 'bsr': the recursion reads the
 Hamiltonian as 4 x 4 site blocks -- for 4-orbital systems, operator=None; the
-moments are bitwise the CSR ones), ``gemm_precision`` for the Correlator
+moments are bitwise the CSR ones), ``tiles`` (a kwant_amd.ordering.Tiling, or
+True to tile a kwant system by its site positions: the device row order groups
+the rows into compact tiles and the plain DOS path runs two Chebyshev steps per
+launch; the moments are bitwise those without tiles), ``gemm_precision`` for the Correlator
 ('fp64' | 'fp32' | 'isliced4' | 'isliced5' | 'sliced3' | 'sliced4': the 'isliced'
 modes are the split-precision GEMM on the INT8 matrix cores, 'sliced' on BF16).
 """
@@ -87,6 +90,36 @@ def _as_csr(hamiltonian, params=None):
         return csr_matrix(hamiltonian)
     except Exception:
         raise ValueError("'hamiltonian' is neither a matrix nor a Kwant system.")
+
+
+def _tiling_of(hamiltonian, tiles, tile_rows=512):
+    """The Tiling for ``tiles``: None/False (no tiles), a Tiling, or True
+    (a kwant system tiled by its site positions, each site's orbitals kept
+    together)."""
+    if tiles is None or tiles is False:
+        return None
+    if isinstance(tiles, ordering.Tiling):
+        return tiles
+    if tiles is True:
+        syst = hamiltonian
+        if not _is_system(syst) or not (hasattr(syst, 'pos_array') or hasattr(syst, 'sites')
+                                        or hasattr(syst, 'pos')):
+            raise ValueError("tiles=True needs a Kwant system with site positions; pass "
+                             "a kwant_amd.ordering.Tiling for a matrix")
+        ranges = np.asarray(syst.site_ranges, dtype=np.int64)
+        if hasattr(syst, 'pos_array'):
+            pos = np.asarray(syst.pos_array, dtype=float)
+        elif hasattr(syst, 'sites'):
+            pos = np.array([np.asarray(site.pos, dtype=float) for site in syst.sites])
+        else:   # kwant's System.pos(i)
+            pos = np.array([np.asarray(syst.pos(i), dtype=float)
+                            for i in range(syst.graph.num_nodes)])
+        norbs = np.zeros(len(pos), dtype=np.int64)
+        for (first, n_orb, _), (nxt, _, _) in zip(ranges[:-1], ranges[1:]):
+            norbs[first:nxt] = n_orb
+        return ordering.tile_order(np.repeat(pos.reshape(len(pos), -1), norbs, axis=0),
+                                   tile_rows)
+    raise ValueError("tiles must be None, True or a kwant_amd.ordering.Tiling")
 
 
 def _orbitals_of(syst, where):
@@ -365,7 +398,7 @@ def _rescale(hamiltonian, eps, v0, bounds, precision='complex128', order=None):
 class _DosBlock:
     """W vectors expanded together: one BlockRecursion plus their moments."""
 
-    def __init__(self, dmat, ab, vectors, width, mode, w=None, variant=-1):
+    def __init__(self, dmat, ab, vectors, width, mode, w=None, variant=-1, tiles=None):
         import torch
         dev = eng.torch_device()
         cdt, _ = eng.complex_dtype(dmat.precision)
@@ -373,7 +406,8 @@ class _DosBlock:
         _check_finite_vectors(host)
         block = torch.from_numpy(np.ascontiguousarray(dmat.rows_in(host))).to(dev).to(cdt)
         self.count = len(vectors)
-        self.rec = eng.BlockRecursion(dmat, ab, block, width, mode, w=w, variant=variant)
+        self.rec = eng.BlockRecursion(dmat, ab, block, width, mode, w=w, variant=variant,
+                                      tiles=tiles)
         self.moments = None  # (count, M) complex128
 
 
@@ -427,11 +461,12 @@ class SpectralDensity:
                  vector_factory=None, bounds=None, eps=0.05, rng=None,
                  kernel=None, mean=True, accumulate_vectors=True, *,
                  precision='complex128', max_block_width=64, variant=-1, reorder='auto',
-                 matrix_format='csr'):
+                 matrix_format='csr', tiles=None):
         if num_moments and energy_resolution:
             raise TypeError("either 'num_moments' or 'energy_resolution' "
                             "must be provided.")
         self.eps = eps
+        tiling = _tiling_of(hamiltonian, tiles)
         hamiltonian = _as_csr(hamiltonian, params)
 
         self._set_operator(operator, params, hamiltonian)
@@ -451,7 +486,18 @@ class SpectralDensity:
             raise ValueError("matrix_format must be 'csr' or 'bsr'")
         if matrix_format == 'bsr' and self.operator is not None:
             raise ValueError("matrix_format='bsr' runs the doubling path (operator=None)")
-        order = None if matrix_format == 'bsr' else ordering.choose_order(hamiltonian, reorder)
+        if tiling is not None and matrix_format == 'bsr':
+            raise ValueError("tiles run the CSR recursion (matrix_format='csr')")
+        if tiling is not None and len(tiling.order) != hamiltonian.shape[0]:
+            raise ValueError("the tiling covers %d rows, the Hamiltonian has %d"
+                             % (len(tiling.order), hamiltonian.shape[0]))
+        self._tiling, self._tiled = tiling, {}
+        if tiling is not None:
+            order = tiling.order
+        elif matrix_format == 'bsr':
+            order = None
+        else:
+            order = ordering.choose_order(hamiltonian, reorder)
         self.hamiltonian, (self._a, self._b) = _rescale(
             hamiltonian, self.eps, self._v0, bounds, precision, order)
         self._dmat = self.hamiltonian._dmat
@@ -705,9 +751,28 @@ class SpectralDensity:
             mode = tune = eng.DOTS_NONE
         return mode, self._launch_shape(width, tune)
 
+    def _tiles_for(self, width, mode):
+        """The two-step tiling of the device matrix for blocks of ``width``
+        vectors (doubling mode only), or None: the first tiled launch shape
+        whose LDS holds a tile and its halo."""
+        if self._tiling is None or mode != eng.DOTS_DOUBLING or \
+                not isinstance(self._dmat, eng.DeviceCSR):
+            return None
+        if width not in self._tiled:
+            plan = None
+            for v in range(eng.tiled_variant_count(width, self._dmat.value_type)):
+                try:
+                    plan = eng.TiledPlan(self._dmat, width, self._tiling.tile_begin, variant=v)
+                    break
+                except RuntimeError:
+                    continue
+            self._tiled[width] = plan
+        return self._tiled[width]
+
     def _new_block(self, vectors, width):
         mode, variant = self._block_mode(width)
-        blk = _DosBlock(self._dmat, (self._a, self._b), vectors, width, mode, variant=variant)
+        blk = _DosBlock(self._dmat, (self._a, self._b), vectors, width, mode, variant=variant,
+                        tiles=self._tiles_for(width, mode))
         self._attach(blk, width)
         return blk
 

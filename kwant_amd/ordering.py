@@ -83,3 +83,72 @@ def choose_order(csr, reorder):
             return None
         return order
     return np.asarray(reorder, dtype=np.int64)
+
+
+class Tiling:
+    """A device row order that groups the rows into compact tiles, and the
+    tiles' row boundaries: device rows tile_begin[t] .. tile_begin[t+1]-1 form
+    tile t.  ``order`` follows the module's convention (device row i = caller
+    row order[i]).  With tiles, the recursion runs two Chebyshev steps per
+    launch (each tile's halo recomputed from LDS, see kpm_cheb_steps2)."""
+
+    def __init__(self, order, tile_begin):
+        self.order = np.asarray(order, dtype=np.int64)
+        self.tile_begin = np.asarray(tile_begin, dtype=np.int64)
+        n = len(self.order)
+        tb = self.tile_begin
+        if tb.ndim != 1 or len(tb) < 2 or tb[0] != 0 or tb[-1] != n or np.any(np.diff(tb) <= 0):
+            raise ValueError("tile_begin must rise strictly from 0 to the row count")
+
+    @property
+    def n_tiles(self):
+        return len(self.tile_begin) - 1
+
+
+def tile_order(coords, tile_rows=512, iterations=4):
+    """Tiling of rows with coordinates ``coords`` ((N,) or (N, d) array, one
+    point per row -- the site position, repeated for each orbital of a site)
+    into axis-aligned boxes of about ``tile_rows`` rows.
+
+    The box edge starts from the mean density of the bounding box and is then
+    scaled so that the 95th percentile of the occupied boxes' row counts meets
+    ``tile_rows``; boxes that still hold more rows are cut into consecutive
+    pieces.  Boxes are numbered with the first coordinate fastest, and within
+    a box the rows keep their relative order.  Returns a Tiling."""
+    x = np.asarray(coords, dtype=np.float64)
+    if x.ndim == 1:
+        x = x[:, None]
+    n, d = x.shape
+    if n == 0:
+        raise ValueError("no rows to tile")
+    tile_rows = int(tile_rows)
+    if tile_rows < 1:
+        raise ValueError("tile_rows must be positive")
+    lo = x.min(axis=0)
+    span = np.maximum(x.max(axis=0) - lo, 1e-12)
+    edge = (np.prod(span) * tile_rows / n) ** (1.0 / d)
+    for _ in range(max(1, int(iterations))):
+        box = np.floor((x - lo) / edge).astype(np.int64)
+        dims = box.max(axis=0) + 1
+        key = np.ravel_multi_index(tuple(box[:, k] for k in range(d - 1, -1, -1)),
+                                   tuple(dims[::-1]))
+        counts = np.bincount(key)
+        occ = counts[counts > 0]
+        p95 = float(np.percentile(occ, 95))
+        if abs(p95 - tile_rows) <= 0.02 * tile_rows:
+            break
+        edge *= (tile_rows / p95) ** (1.0 / d)
+    box = np.floor((x - lo) / edge).astype(np.int64)
+    dims = box.max(axis=0) + 1
+    key = np.ravel_multi_index(tuple(box[:, k] for k in range(d - 1, -1, -1)), tuple(dims[::-1]))
+    order = np.argsort(key, kind="stable")
+    sorted_key = key[order]
+    starts = np.flatnonzero(np.r_[True, sorted_key[1:] != sorted_key[:-1]])
+    ends = np.r_[starts[1:], n]
+    bounds = [0]
+    for s, e in zip(starts, ends):
+        pieces = -(-(e - s) // tile_rows)
+        step = -(-(e - s) // pieces)
+        bounds.extend(range(s + step, e, step))
+        bounds.append(e)
+    return Tiling(order, np.asarray(bounds, dtype=np.int64))

@@ -78,6 +78,9 @@ def save(sd, path):
         rng = None
     if sd._dmat.order is not None:
         arrays["order"] = np.asarray(sd._dmat.order, dtype=np.int64)
+    tiling = getattr(sd, "_tiling", None)
+    if tiling is not None:
+        arrays["tile_begin"] = np.asarray(tiling.tile_begin, dtype=np.int64)
     from . import kpm
     kernel = {kpm.jackson_kernel: "jackson", kpm.lorentz_kernel: "lorentz"}.get(sd.kernel,
                                                                               "custom")
@@ -145,6 +148,10 @@ def load(path, hamiltonian, params=None, operator=None, kernel=None, vector_fact
     sd._variant = meta["variant"]
     sd._a, sd._b = meta["a"], meta["b"]
     order = arrays.get("order")
+    sd._tiling, sd._tiled = None, {}
+    if "tile_begin" in arrays:
+        from .ordering import Tiling
+        sd._tiling = Tiling(order, arrays["tile_begin"])
     dmat = eng.DeviceCSR(h, sd._precision, order=order)
     sd.hamiltonian = kpm._RescaledOperator(dmat, sd._a, sd._b)
     sd._dmat = dmat
@@ -190,7 +197,8 @@ def load(path, hamiltonian, params=None, operator=None, kernel=None, vector_fact
                 raise ValueError("the saved vectors do not cover block %d" % i)
             width = entry["width"]
             blk = kpm._DosBlock(dmat, (sd._a, sd._b), saved[first:first + count], width,
-                                entry["mode"], variant=entry["variant"])
+                                entry["mode"], variant=entry["variant"],
+                                tiles=sd._tiles_for(width, entry["mode"]))
             sd._attach(blk, width)
             ring = torch.from_numpy(arrays["block%d_ring" % i]).to(dev)
             if ring.dtype != cdt or tuple(ring.shape) != tuple(blk.rec.ring.shape):
