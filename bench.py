@@ -40,7 +40,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cheb_step_c64_graphene2M_traffic.txt")
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
+                 ("cheb_step_c64_graphene2M_traffic.txt", "cheb2_tiled_c64_graphene2M_traffic.txt")]
 
 
 def log(msg):
@@ -85,12 +86,13 @@ def read_traffic(variant):
     calibration) for this launch shape ("variant V" or "tiled V"), if
     committed; else None."""
     key = "traffic_bytes_per_launch %s:" % variant
-    try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith(key):
-                return float(line.split(":", 1)[1])
-    except OSError:
-        pass
+    for path in TRAFFIC_FILES:
+        try:
+            for line in open(path):
+                if line.startswith(key):
+                    return float(line.split(":", 1)[1])
+        except OSError:
+            pass
     return None
 
 
@@ -120,7 +122,7 @@ def two_step_bench(h, pos, ab, block, width, warmup, steps, tile_rows, variant,
     for v in shapes:
         try:
             tp = eng.TiledPlan(mat, width, tiling.tile_begin, variant=v)
-        except RuntimeError as err:
+        except (RuntimeError, ValueError) as err:
             log_fn("two-step shape %d: %s" % (v, err))
             continue
         rec = eng.BlockRecursion(mat, ab, blk, width, eng.DOTS_DOUBLING, tiles=tp)
@@ -880,7 +882,7 @@ def main():
     ap.add_argument("--cells", type=int, default=1000, help="unit cells per side")
     ap.add_argument("--width", type=int, default=64, help="vectors per block")
     ap.add_argument("--variant", type=int, default=-1, help="kernel launch shape")
-    ap.add_argument("--tile-rows", type=int, default=512,
+    ap.add_argument("--tile-rows", type=int, default=256,
                     help="rows per tile of the two-step kernel (0: one-step kernel only)")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="two-step launch shape (default: the fastest that fits)")

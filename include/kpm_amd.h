@@ -209,23 +209,25 @@ int kpm_bsr_steps(kpm_plan* plan, const kpm_bsr* h, void* ring, int64_t slot_str
  * what two kpm_cheb_steps launches give.  Tiles pay off when the row order
  * makes them compact patches of the lattice (few halo rows per tile row).
  *
- * kpm_tiled_create builds the tiling of a CSR matrix (host arrays indptr and
- * indices, the same matrix later passed to kpm_cheb_steps2 in device memory):
- * tile t owns rows [tile_row_begin[t], tile_row_begin[t+1]) (host array of
- * n_tiles + 1), at most the variant's max_tile_rows each.  It computes each
- * tile's halo and the LDS slot of every entry's column, and fails with
- * KPM_ERR_UNSUPPORTED when a tile and its halo do not fit in LDS.  variant
- * selects the launch shape (kpm_tiled_variant_info: columns per workgroup,
- * waves per workgroup, the largest tile); -1 picks the default.  The tiling
- * holds its own partials workspace; use one tiling per plan at a time.
+ * kpm_tiled_create builds the tiling of the device CSR matrix h (the same
+ * matrix later passed to kpm_cheb_steps2): tile t owns rows
+ * [tile_row_begin[t], tile_row_begin[t+1]) (host array of n_tiles + 1), at
+ * most the variant's max_tile_rows each.  It reads the pattern back, computes
+ * each tile's halo and the LDS slot of every entry's column, and packs the
+ * first entries of every row (columns, LDS slots, values) so that a row's
+ * entries need no row-extent read first; it fails with KPM_ERR_UNSUPPORTED
+ * when a tile and its halo do not fit in LDS.  The packed values are a copy:
+ * build the tiling again after changing the matrix.  variant selects the
+ * launch shape (kpm_tiled_variant_info: columns per workgroup, waves per
+ * workgroup, the largest tile); -1 picks the default.  The tiling holds its
+ * own partials workspace; use one tiling per plan at a time.  Synchronous.
  * Replaces the per-vector loop of kwant/kpm.py:436-477 for the plain DOS
  * path, like kpm_cheb_steps. */
 typedef struct kpm_tiled kpm_tiled;
 int kpm_tiled_variant_count(int32_t width, int32_t value_type, int32_t* count);
 int kpm_tiled_variant_info(int32_t width, int32_t value_type, int32_t variant,
                            int32_t* group_width, int32_t* waves, int32_t* max_tile_rows);
-int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t nnz,
-                     const int32_t* h_indptr, const int32_t* h_indices, int32_t n_tiles,
+int kpm_tiled_create(const kpm_csr* h, int32_t width, int32_t n_tiles,
                      const int32_t* h_tile_row_begin, int32_t variant, kpm_tiled** out);
 int kpm_tiled_destroy(kpm_tiled* tiles);
 /* Tiles, the largest tile, the largest halo, all halo rows and their matrix

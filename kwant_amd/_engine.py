@@ -136,7 +136,7 @@ def load_library(path=LIB_PATH):
                                          i32, i32, i32, vp]),
         "kpm_tiled_variant_count": (ctypes.c_int, [i32, i32, pi32]),
         "kpm_tiled_variant_info": (ctypes.c_int, [i32, i32, i32, pi32, pi32, pi32]),
-        "kpm_tiled_create": (ctypes.c_int, [i32, i32, i32, i64, pi32, pi32, i32, pi32, i32,
+        "kpm_tiled_create": (ctypes.c_int, [ctypes.POINTER(KpmCsr), i32, i32, pi32, i32,
                                             ctypes.POINTER(vp)]),
         "kpm_tiled_destroy": (ctypes.c_int, [vp]),
         "kpm_tiled_info": (ctypes.c_int, [vp, pi32, pi32, pi32] +
@@ -496,21 +496,18 @@ class TiledPlan:
     """Tiles of consecutive device rows of ``mat`` (tile t owns rows
     tile_begin[t] .. tile_begin[t+1]-1) for the two-step launches
     (kpm_cheb_steps2): the library computes each tile's halo and the LDS slot
-    of every entry from the matrix pattern (read back from the device)."""
+    of every entry from the matrix pattern, and packs each row's first
+    entries."""
 
     def __init__(self, mat, width, tile_begin, variant=-1):
         lib = load_library()
         tb = np.ascontiguousarray(tile_begin, dtype=np.int32)
-        indptr = np.ascontiguousarray(mat.indptr.cpu().numpy(), dtype=np.int32)
-        indices = np.ascontiguousarray(mat.indices.cpu().numpy(), dtype=np.int32)
-        if indices.size == 0:
-            indices = np.zeros(1, dtype=np.int32)
         pi32 = ctypes.POINTER(ctypes.c_int32)
         handle = ctypes.c_void_p()
-        check(lib.kpm_tiled_create(int(mat.shape[0]), int(width), mat.value_type, mat.nnz,
-                                   indptr.ctypes.data_as(pi32), indices.ctypes.data_as(pi32),
-                                   len(tb) - 1, tb.ctypes.data_as(pi32), int(variant),
+        check(lib.kpm_tiled_create(ctypes.byref(mat.desc), int(width), len(tb) - 1,
+                                   tb.ctypes.data_as(pi32), int(variant),
                                    ctypes.byref(handle)), "kpm_tiled_create")
+        self.mat = mat   # the packed entries are a copy of this matrix's
         self.handle = handle
         self.width, self.n_rows, self.value_type = int(width), int(mat.shape[0]), mat.value_type
         self.tile_begin = tb

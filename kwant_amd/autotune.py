@@ -121,3 +121,71 @@ def pick_variant(dmat, ab, width, mode=eng.DOTS_DOUBLING, force=False):
                                                     range(max(cands) + 1)]}
     _store()
     return best
+
+
+def _tiled_key(dmat, width, tile_begin):
+    import zlib
+    import numpy as np
+    tb = np.ascontiguousarray(tile_begin, dtype=np.int64)
+    return "%s:tiled:%08x" % (key_of(dmat, width, eng.DOTS_DOUBLING), zlib.crc32(tb.tobytes()))
+
+
+def time_tiled(dmat, ab, width, tiles, block, steps=TIMED_STEPS):
+    """Milliseconds per Chebyshev step of two-step launches (``tiles``, a
+    TiledPlan) on ``block``."""
+    import torch
+    rec = eng.BlockRecursion(dmat, ab, block, width, eng.DOTS_DOUBLING, tiles=tiles)
+    pairs = max(1, steps // 2)
+    dots = torch.empty((3 + 2 * pairs, 1, width, 4), dtype=torch.float64, device=block.device)
+    rec.step(3)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    tiles.steps(rec.plan, dmat, rec.ring, rec.slot_stride, rec.slots, pairs, eng.DOTS_DOUBLING,
+                dots=dots, out_base=0)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / (2 * pairs)
+
+
+def pick_tiled(dmat, ab, width, tile_begin, force=False):
+    """The two-step launch (a TiledPlan of ``dmat`` for ``tile_begin``) to run
+    blocks of ``width`` vectors with, or None.  Shapes whose LDS cannot hold a
+    tile and its halo (or that take smaller tiles) are skipped.  For large
+    problems (or ``force``) the feasible shapes are timed, and None is returned
+    when the one-step kernel on the same rows is faster; small problems take
+    the first feasible shape.  Results are bitwise the same either way."""
+    import torch
+    plans = {}
+    for v in range(eng.tiled_variant_count(width, dmat.value_type)):
+        try:
+            plans[v] = eng.TiledPlan(dmat, width, tile_begin, variant=v)
+        except (RuntimeError, ValueError):
+            continue
+    if not plans:
+        return None
+    first = min(plans)
+    if not force and dmat.shape[0] * width < MIN_ELEMENTS:
+        return plans[first]
+    if not _CACHE:
+        _load()
+    key = _tiled_key(dmat, width, tile_begin)
+    if key in _CACHE:
+        v = int(_CACHE[key]["variant"])
+        return plans.get(v) if v >= 0 else None
+    cdt, _ = eng.complex_dtype(dmat.precision)
+    dev = eng.torch_device()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    phase = torch.rand((dmat.shape[0], width), generator=gen, device=dev, dtype=torch.float64)
+    block = torch.exp(2j * 3.141592653589793 * phase).to(cdt)
+    times = {v: time_tiled(dmat, ab, width, p, block, 3 * TIMED_STEPS) for v, p in plans.items()}
+    best = min(times, key=lambda v: times[v])
+    one = pick_variant(dmat, ab, width, eng.DOTS_DOUBLING, force=True)
+    one_ms = time_variant(dmat, ab, width, eng.DOTS_DOUBLING, one, block, 3 * TIMED_STEPS)
+    chosen = best if times[best] < one_ms else -1
+    _CACHE[key] = {"variant": chosen, "ms_per_step": {str(v): t for v, t in times.items()},
+                   "one_step_ms": one_ms}
+    _store()
+    return plans[best] if chosen >= 0 else None

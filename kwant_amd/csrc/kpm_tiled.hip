@@ -64,16 +64,30 @@ struct TiledParams {
   int32_t tile_cap;            // LDS rows of a_n (tile rows)
 };
 
+// Workgroup iterations over a tile's rows in phase B (a tile has at most
+// kTileIters x rows-per-iteration rows), unrolled so that every entry of the
+// phase is requested before the barrier that opens it.
+constexpr int kTileIters = 4;
+
+// The first PF entries of every row, packed so that a row's entries need no
+// row-extent load first: col_pf[r * PF + e] is the column of entry e (-1: the
+// row has fewer entries), and the last slot holds -2 - j when the row has
+// more than PF entries (the rest follow from the CSR); lidx_pf the same with
+// LDS slots (tile rows only); val_pf the values (0 where there is none).
 template <typename T, int VEC, int LPR, int WAVES, int MODE, int PF>
 __global__ __launch_bounds__(WAVES * 64) void tiled_kernel(TiledParams p,
                                                            const int32_t* __restrict__ indptr,
                                                            const int32_t* __restrict__ indices,
                                                            const void* __restrict__ values,
                                                            const int32_t* __restrict__ lidx,
+                                                           const int32_t* __restrict__ col_pf,
+                                                           const int32_t* __restrict__ lidx_pf,
+                                                           const void* __restrict__ val_pf,
                                                            int n) {
   constexpr int WC = LPR * VEC;     // columns per work item
   constexpr int RPW = 64 / LPR;     // rows per wavefront
   constexpr int G = WAVES * RPW;    // rows per workgroup iteration
+  constexpr int NB = kTileIters;
   constexpr int NC = NComp<MODE>::value;
   constexpr int NQ = NQuant<MODE>::value;
   constexpr int NQA = NQ > 0 ? NQ : 1;
@@ -136,47 +150,54 @@ __global__ __launch_bounds__(WAVES * 64) void tiled_kernel(TiledParams p,
   CL wz;
   zero(wz);
 
-  // The matrix entries of a row, cooperatively: lane `sub` of the row's
+  // The packed entries of a row, cooperatively: lane `sub` of the row's
   // group takes entries sub, sub + LPR, ... of the first PF, and the group
-  // shares them by shuffles.  ix: the column index array (global columns in
-  // phase A, LDS slots in phase B).
-  auto head = [&](const int32_t* __restrict__ ix, int k0, int k1, int (&j)[NCH], T (&hr)[NCH],
-                  T (&hi)[NCH]) {
+  // shares them by shuffles.  ix_pf: packed columns (phase A) or LDS slots
+  // (phase B).
+  auto head = [&](const int32_t* __restrict__ ix_pf, int row, bool live, int (&j)[NCH],
+                  T (&hr)[NCH], T (&hi)[NCH]) {
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
       const int e = c * LPR + sub;
-      j[c] = 0;
+      j[c] = -1;
       hr[c] = T(0);
       hi[c] = T(0);
-      if (e < PF && k0 + e < k1) {
-        j[c] = ix[k0 + e];
-        load_value<T>(values, k0 + e, hr[c], hi[c]);
+      if (live && e < PF) {
+        const int64_t k = int64_t(row) * PF + e;
+        j[c] = ix_pf[k];
+        load_value<T>(val_pf, k, hr[c], hi[c]);
       }
     }
   };
   // sum_k H[row, j_k] x[j_k, cols] with x rows at src + j * ld (complex
-  // elements): the first PF entries from the prefetched head, the rest in
-  // chunks of LPR, in order -- the FMA sequence of the one-step kernels.
-  auto row_sum = [&](const T* src, int ld, const int32_t* __restrict__ ix, int k0, int k1,
+  // elements): the first PF entries from the packed head, the rest (rows
+  // flagged longer) from the CSR in chunks of LPR, in order -- the FMA
+  // sequence of the one-step kernels.  ix: the CSR's column index array
+  // (global columns in phase A, LDS slots in phase B).
+  auto row_sum = [&](const T* src, int ld, const int32_t* __restrict__ ix, int row,
                      const int (&j)[NCH], const T (&hr)[NCH], const T (&hi)[NCH]) {
     CL acc;
     zero(acc);
-    const int deg = k1 - k0;
     CL x[PF];
     T er[PF], ei[PF];
+    bool has[PF];
+    int last = -1;
 #pragma unroll
     for (int e = 0; e < PF; ++e) {
       const int jj = __shfl(j[e / LPR], e % LPR, LPR);
       er[e] = __shfl(hr[e / LPR], e % LPR, LPR);
       ei[e] = __shfl(hi[e / LPR], e % LPR, LPR);
-      if (e < deg)
-        x[e] = IO::load(src + 2 * (int64_t(jj) * ld + lcol));
+      has[e] = jj != -1;
+      const int jr = jj >= 0 ? jj : -2 - jj;
+      if (has[e])
+        x[e] = IO::load(src + 2 * (int64_t(jr) * ld + lcol));
       else
         zero(x[e]);
+      if (e == PF - 1) last = jj;
     }
 #pragma unroll
     for (int e = 0; e < PF; ++e) {
-      if (e < deg) {
+      if (has[e]) {
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
           acc.re[v] = fma(er[e], x[e].re[v], acc.re[v]);
@@ -186,26 +207,30 @@ __global__ __launch_bounds__(WAVES * 64) void tiled_kernel(TiledParams p,
         }
       }
     }
-    for (int kt = k0 + PF; kt < k1; kt += LPR) {
-      int jt = 0;
-      T htr = T(0), hti = T(0);
-      if (kt + sub < k1) {
-        jt = ix[kt + sub];
-        load_value<T>(values, kt + sub, htr, hti);
-      }
+    if (last <= -2) {
+      // a row longer than PF: the whole group takes this loop together
+      const int k1 = indptr[row + 1];
+      for (int kt = indptr[row] + PF; kt < k1; kt += LPR) {
+        int jt = 0;
+        T htr = T(0), hti = T(0);
+        if (kt + sub < k1) {
+          jt = ix[kt + sub];
+          load_value<T>(values, kt + sub, htr, hti);
+        }
 #pragma unroll
-      for (int s = 0; s < LPR; ++s) {
-        const int jj = __shfl(jt, s, LPR);
-        const T tr = __shfl(htr, s, LPR);
-        const T ti = __shfl(hti, s, LPR);
-        if (kt + s < k1) {
-          const CL xe = IO::load(src + 2 * (int64_t(jj) * ld + lcol));
+        for (int s = 0; s < LPR; ++s) {
+          const int jj = __shfl(jt, s, LPR);
+          const T tr = __shfl(htr, s, LPR);
+          const T ti = __shfl(hti, s, LPR);
+          if (kt + s < k1) {
+            const CL xe = IO::load(src + 2 * (int64_t(jj) * ld + lcol));
 #pragma unroll
-          for (int v = 0; v < VEC; ++v) {
-            acc.re[v] = fma(tr, xe.re[v], acc.re[v]);
-            acc.re[v] = fma(-ti, xe.im[v], acc.re[v]);
-            acc.im[v] = fma(tr, xe.im[v], acc.im[v]);
-            acc.im[v] = fma(ti, xe.re[v], acc.im[v]);
+            for (int v = 0; v < VEC; ++v) {
+              acc.re[v] = fma(tr, xe.re[v], acc.re[v]);
+              acc.re[v] = fma(-ti, xe.im[v], acc.re[v]);
+              acc.im[v] = fma(tr, xe.im[v], acc.im[v]);
+              acc.im[v] = fma(ti, xe.re[v], acc.im[v]);
+            }
           }
         }
       }
@@ -215,63 +240,55 @@ __global__ __launch_bounds__(WAVES * 64) void tiled_kernel(TiledParams p,
 
   const int t_lo = int((int64_t(xcd) * p.n_tiles) / kNumXcd);
   const int t_hi = int((int64_t(xcd + 1) * p.n_tiles) / kNumXcd);
-  for (int t = t_lo + kk; t < t_hi; t += fronts) {
-    const int r0 = p.tile_begin[t];
-    const int R = p.tile_begin[t + 1] - r0;
-    const int h0 = p.halo_begin[t];
-    const int H = p.halo_begin[t + 1] - h0;
+  int t = t_lo + kk;
+  int r0 = 0, R = 0, h0 = 0, H = 0;
+  // phase A's pipeline registers: iteration it's row and packed entries
+  int rA = 0;
+  bool lA = false;
+  int jA[NCH];
+  T hrA[NCH], hiA[NCH];
+  auto open_tile = [&](int tt) {
+    // a tile's extent, and the entries of its first phase-A iteration
+    r0 = p.tile_begin[tt];
+    R = p.tile_begin[tt + 1] - r0;
+    h0 = p.halo_begin[tt];
+    H = p.halo_begin[tt + 1] - h0;
+    lA = gid < R;
+    rA = r0 + (lA ? gid : 0);
+    head(col_pf, rA, lA, jA, hrA, hiA);
+  };
+  if (t < t_hi) open_tile(t);
+  while (t < t_hi) {
     for (int s = tid; s < H; s += WAVES * 64) lds_h[s] = p.halo_rows[h0 + s];
     __syncthreads();  // halo rows visible; the previous tile's phase B is done
 
     // ---- phase A: a_{n+1} of the tile's rows (iterations 0 .. NI-1) and of
-    // its halo (iterations NI ..), software-pipelined: while iteration it
-    // gathers, the entries of it + 1 and the extent of it + 2 are in flight.
+    // its halo (iterations NI ..): while iteration it gathers, the entries of
+    // it + 1 are in flight.
     const int NI = (R + G - 1) / G;
     const int NA = NI + (H + G - 1) / G;
-    auto slot_row = [&](int it, int& row, bool& live) {
-      if (it < NI) {
-        const int s = gid + G * it;
-        live = s < R;
-        row = r0 + (live ? s : 0);
-      } else {
-        const int s = gid + G * (it - NI);
-        live = s < H;
-        row = live ? lds_h[s] : r0;
-      }
-    };
-    auto extent = [&](int it, int lim, int& row, bool& live, int& k0, int& k1) {
-      k0 = 0;
-      k1 = 0;
-      live = false;
-      row = r0;
-      if (it < lim) {
-        slot_row(it, row, live);
-        if (live) {
-          k0 = indptr[row];
-          k1 = indptr[row + 1];
-        }
-      }
-    };
-    int rA, kA0, kA1, rB, kB0, kB1;
-    bool lA, lB;
-    int jA[NCH];
-    T hrA[NCH], hiA[NCH];
-    extent(0, NA, rA, lA, kA0, kA1);
-    head(indices, kA0, kA1, jA, hrA, hiA);
-    extent(1, NA, rB, lB, kB0, kB1);
 #pragma unroll 1
     for (int it = 0; it < NA; ++it) {
+      const int nx = it + 1;
+      int rB = r0;
+      bool lB = false;
+      if (nx < NI) {
+        const int s = gid + G * nx;
+        lB = s < R;
+        rB = r0 + (lB ? s : 0);
+      } else if (nx < NA) {
+        const int s = gid + G * (nx - NI);
+        lB = s < H;
+        rB = lB ? lds_h[s] : r0;
+      }
       int jB[NCH];
       T hrB[NCH], hiB[NCH];
-      head(indices, kB0, kB1, jB, hrB, hiB);
-      int rC, kC0, kC1;
-      bool lC;
-      extent(it + 2, NA, rC, lC, kC0, kC1);
+      head(col_pf, rB, lB, jB, hrB, hiB);
       const bool inner = it < NI;
       const int64_t off = 2 * (int64_t(rA) * W + col);
       CL xs = IO::load(a0 + off);
       CL xp = inner ? IO::load_nt(am + off) : IO::load(am + off);
-      const CL acc = row_sum(a0 + 2 * cg * WC, W, indices, kA0, kA1, jA, hrA, hiA);
+      const CL acc = row_sum(a0 + 2 * cg * WC, W, indices, rA, jA, hrA, hiA);
       CL y;
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {
@@ -298,29 +315,29 @@ __global__ __launch_bounds__(WAVES * 64) void tiled_kernel(TiledParams p,
       }
       rA = rB;
       lA = lB;
-      kA0 = kB0;
-      kA1 = kB1;
-      rB = rC;
-      lB = lC;
-      kB0 = kC0;
-      kB1 = kC1;
+    }
+    // every entry of phase B, requested before the barrier
+    int jb[NB][NCH];
+    T hrb[NB][NCH], hib[NB][NCH];
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+      const int s = gid + G * it;
+      head(lidx_pf, r0 + (s < R ? s : 0), s < R, jb[it], hrb[it], hib[it]);
     }
     __syncthreads();  // a_{n+1} of the tile and its halo, a_n of the tile in LDS
 
-    // ---- phase B: a_{n+2} of the tile's rows from LDS (local slots)
-    extent(0, NI, rA, lA, kA0, kA1);
-    head(lidx, kA0, kA1, jA, hrA, hiA);
-    extent(1, NI, rB, lB, kB0, kB1);
-#pragma unroll 1
-    for (int it = 0; it < NI; ++it) {
-      int jB[NCH];
-      T hrB[NCH], hiB[NCH];
-      head(lidx, kB0, kB1, jB, hrB, hiB);
-      int rC, kC0, kC1;
-      bool lC;
-      extent(it + 2, NI, rC, lC, kC0, kC1);
-      const int s = lA ? gid + G * it : 0;
-      const CL acc = row_sum(lds, WC, lidx, kA0, kA1, jA, hrA, hiA);
+    // ---- phase B: a_{n+2} of the tile's rows from LDS (local slots); the
+    // next tile's first entries are requested meanwhile
+    const int rb0 = r0, RB = R;
+    t += fronts;
+    if (t < t_hi) open_tile(t);
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+      if (G * it >= RB) break;
+      const int s0 = gid + G * it;
+      const bool live = s0 < RB;
+      const int s = live ? s0 : 0;
+      const CL acc = row_sum(lds, WC, lidx, rb0 + s, jb[it], hrb[it], hib[it]);
       CL y1 = IO::load(lds + 2 * (int64_t(s) * WC + lcol));
       const CL xn = IO::load(lds_x + 2 * (int64_t(s) * WC + lcol));
       CL y2;
@@ -329,27 +346,13 @@ __global__ __launch_bounds__(WAVES * 64) void tiled_kernel(TiledParams p,
         y2.re[v] = fma(acc.re[v] - bsh * y1.re[v], mul, -xn.re[v]);
         y2.im[v] = fma(acc.im[v] - bsh * y1.im[v], mul, -xn.im[v]);
       }
-      if (lA) {
-        IO::store_nt(a2 + 2 * (int64_t(rA) * W + col), y2);
+      if (live) {
+        IO::store_nt(a2 + 2 * (int64_t(rb0 + s) * W + col), y2);
       } else {
         zero(y1);
         zero(y2);
       }
       if constexpr (dots) add_dots<T, VEC, MODE, NC, NQA>(asB, acB, aqB, qk, fg, y1, y2, wz);
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        jA[c] = jB[c];
-        hrA[c] = hrB[c];
-        hiA[c] = hiB[c];
-      }
-      rA = rB;
-      lA = lB;
-      kA0 = kB0;
-      kA1 = kB1;
-      rB = rC;
-      lB = lC;
-      kB0 = kC0;
-      kB1 = kC1;
     }
   }
   if constexpr (!dots) return;
@@ -369,6 +372,24 @@ __global__ __launch_bounds__(WAVES * 64) void tiled_kernel(TiledParams p,
                                                   tid, asB, acB, aqB, red_s, red_c, red_q);
 }
 
+// The packed values: val_pf[r * PF + e] = values[indptr[r] + e] for e below
+// the row's length, 0 past it.
+template <typename T>
+__global__ __launch_bounds__(256) void pack_values_kernel(const int32_t* __restrict__ indptr,
+                                                          const void* __restrict__ values,
+                                                          void* __restrict__ val_pf, int n_rows,
+                                                          int pf) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= int64_t(n_rows) * pf) return;
+  const int r = int(i / pf), e = int(i - int64_t(r) * pf);
+  const int k0 = indptr[r], k1 = indptr[r + 1];
+  T re = T(0), im = T(0);
+  if (k0 + e < k1) load_value<T>(values, k0 + e, re, im);
+  T* out = reinterpret_cast<T*>(val_pf);
+  out[2 * i] = re;
+  out[2 * i + 1] = im;
+}
+
 // ----------------------------------------------------------------------------
 // Launch shapes and dispatch.
 
@@ -378,16 +399,17 @@ struct TiledShape {
   int lpr;        // lanes per row
   int vec;        // complex values per lane (group width = lpr * vec)
   int waves;      // wavefronts per workgroup
-  int tile_rows;  // suggested tile size: a tile of this many rows and a
-                  // quarter as many halo rows fits the LDS share of a
-                  // workgroup at full occupancy (one 16-wave or two 8-wave
-                  // workgroups per CU)
+  int tile_rows;  // the largest tile: kTileIters x rows per workgroup
+                  // iteration; with a quarter as many halo rows it fits the
+                  // LDS share of a workgroup at full occupancy (one 16-wave
+                  // or two 8-wave workgroups per CU)
 };
 
-// complex64: 8 lanes x 2 columns (16-column groups, one 128-byte line per
-// row) or 16 lanes x 2 (32-column groups); complex128: 16 lanes x 1 or 8 x 1.
-constexpr TiledShape kC64Shapes[] = {{8, 2, 16, 512}, {8, 2, 8, 256}, {16, 2, 16, 256}};
-constexpr TiledShape kC128Shapes[] = {{8, 1, 16, 512}, {16, 1, 16, 256}};
+// complex64: 8, 16 or 32 lanes x 2 columns (groups of 16, 32 or 64 columns:
+// 128 to 512 bytes per row); complex128: 8, 16 or 32 lanes x 1.
+constexpr TiledShape kC64Shapes[] = {{8, 2, 16, 512},  {8, 2, 8, 256},  {16, 2, 16, 256},
+                                     {16, 2, 8, 128},  {32, 2, 16, 128}, {32, 2, 8, 64}};
+constexpr TiledShape kC128Shapes[] = {{8, 1, 16, 512}, {16, 1, 16, 256}, {32, 1, 16, 128}};
 constexpr int kTiledPrefetch[] = {4, 8};
 
 int tiled_shapes(int width, int value_type, TiledShape* out, int max_out) {
@@ -427,9 +449,13 @@ const void* tiled_kernel_for(int value_type, const TiledShape& s, int mode, int 
     if (s.lpr == 8 && s.waves == 16) return pick_mode<float, 2, 8, 16>(mode, pf);
     if (s.lpr == 8 && s.waves == 8) return pick_mode<float, 2, 8, 8>(mode, pf);
     if (s.lpr == 16 && s.waves == 16) return pick_mode<float, 2, 16, 16>(mode, pf);
+    if (s.lpr == 16 && s.waves == 8) return pick_mode<float, 2, 16, 8>(mode, pf);
+    if (s.lpr == 32 && s.waves == 16) return pick_mode<float, 2, 32, 16>(mode, pf);
+    if (s.lpr == 32 && s.waves == 8) return pick_mode<float, 2, 32, 8>(mode, pf);
   } else if (value_type == KPM_C128) {
     if (s.lpr == 8 && s.waves == 16) return pick_mode<double, 1, 8, 16>(mode, pf);
     if (s.lpr == 16 && s.waves == 16) return pick_mode<double, 1, 16, 16>(mode, pf);
+    if (s.lpr == 32 && s.waves == 16) return pick_mode<double, 1, 32, 16>(mode, pf);
   }
   return nullptr;
 }
@@ -451,7 +477,9 @@ using namespace kpm;
 struct kpm_tiled {
   int32_t n_rows = 0, width = 0, value_type = 0, n_tiles = 0;
   int64_t nnz = 0;
+  const void* values = nullptr;  // the matrix whose entries were packed
   TiledShape shape{};
+  int32_t pf = 0;          // packed entries per row
   int32_t groups = 0;      // column groups
   int32_t slot_cap = 0;    // max tile rows + halo rows
   int32_t halo_cap = 0;    // max halo rows
@@ -464,7 +492,10 @@ struct kpm_tiled {
   int32_t* tile_begin = nullptr;
   int32_t* halo_begin = nullptr;
   int32_t* halo_rows = nullptr;
-  int32_t* lidx = nullptr;
+  int32_t* lidx = nullptr;     // nnz: LDS slot of every entry of its row's tile
+  int32_t* col_pf = nullptr;   // n_rows x pf packed columns
+  int32_t* lidx_pf = nullptr;  // n_rows x pf packed LDS slots
+  void* val_pf = nullptr;      // n_rows x pf packed values
   int32_t* seg_blk = nullptr;
   double* part = nullptr;  // four buffers of part_q doubles: (step parity in the launch) x (launch parity)
   int64_t part_q = 0;
@@ -495,9 +526,25 @@ void free_tiled(kpm_tiled* t) {
   (void)hipFree(t->halo_begin);
   (void)hipFree(t->halo_rows);
   (void)hipFree(t->lidx);
+  (void)hipFree(t->col_pf);
+  (void)hipFree(t->lidx_pf);
+  (void)hipFree(t->val_pf);
   (void)hipFree(t->seg_blk);
   (void)hipFree(t->part);
   delete t;
+}
+
+// The first pf entries of each row of (indptr, ix) with the continuation
+// flag: -1 past the row's end, -2 - j in the last slot when more follow.
+std::vector<int32_t> pack_entries(const std::vector<int32_t>& indptr,
+                                  const std::vector<int32_t>& ix, int32_t n_rows, int pf) {
+  std::vector<int32_t> out(size_t(n_rows) * pf, -1);
+  for (int32_t r = 0; r < n_rows; ++r) {
+    const int64_t k0 = indptr[r], len = int64_t(indptr[r + 1]) - k0;
+    for (int e = 0; e < pf && e < len; ++e) out[size_t(r) * pf + e] = ix[size_t(k0 + e)];
+    if (len > pf) out[size_t(r) * pf + pf - 1] = -2 - out[size_t(r) * pf + pf - 1];
+  }
+  return out;
 }
 
 }  // namespace
@@ -522,15 +569,18 @@ int kpm_tiled_variant_info(int32_t width, int32_t value_type, int32_t variant,
   return KPM_OK;
 }
 
-int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t nnz,
-                     const int32_t* h_indptr, const int32_t* h_indices, int32_t n_tiles,
+int kpm_tiled_create(const kpm_csr* h, int32_t width, int32_t n_tiles,
                      const int32_t* h_tile_row_begin, int32_t variant, kpm_tiled** out) {
   if (!out) return fail(KPM_ERR_ARG, "out is NULL");
   *out = nullptr;
-  if (!h_indptr || (nnz > 0 && !h_indices) || !h_tile_row_begin)
-    return fail(KPM_ERR_ARG, "NULL argument");
-  if (n_rows <= 0 || n_tiles <= 0) return fail(KPM_ERR_ARG, "need rows and tiles");
+  if (!h || !h_tile_row_begin) return fail(KPM_ERR_ARG, "NULL argument");
+  const int32_t n_rows = h->n_rows, value_type = h->value_type;
+  const int64_t nnz = h->nnz;
+  if (n_rows <= 0 || h->n_cols != n_rows) return fail(KPM_ERR_ARG, "need a square matrix");
+  if (n_tiles <= 0) return fail(KPM_ERR_ARG, "need at least one tile");
   if (nnz < 0 || nnz > INT32_MAX) return fail(KPM_ERR_ARG, "nnz out of int32 range");
+  if (!h->indptr || (nnz > 0 && (!h->indices || !h->values)))
+    return fail(KPM_ERR_ARG, "matrix arrays are NULL");
   TiledShape shapes[kTiledMaxShapes];
   const int nv = tiled_shapes(width, value_type, shapes, kTiledMaxShapes);
   if (nv == 0)
@@ -539,18 +589,33 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
   if (variant >= nv) return fail(KPM_ERR_ARG, "tiled variant out of range");
   const TiledShape sh = shapes[variant];
   const int wc = sh.lpr * sh.vec;
-  if (h_indptr[0] != 0 || int64_t(h_indptr[n_rows]) != nnz)
-    return fail(KPM_ERR_ARG, "indptr must run from 0 to nnz");
-  for (int32_t r = 0; r < n_rows; ++r)
-    if (h_indptr[r + 1] < h_indptr[r]) return fail(KPM_ERR_ARG, "indptr must not decrease");
+  if (sh.tile_rows != kTileIters * sh.waves * 64 / sh.lpr)
+    return fail(KPM_ERR_UNSUPPORTED, "tiled shape table inconsistent with the kernel");
   if (h_tile_row_begin[0] != 0 || h_tile_row_begin[n_tiles] != n_rows)
     return fail(KPM_ERR_ARG, "tiles must cover rows [0, n_rows)");
   int32_t max_tile = 0;
   for (int32_t t = 0; t < n_tiles; ++t) {
     const int32_t r = h_tile_row_begin[t + 1] - h_tile_row_begin[t];
     if (r <= 0) return fail(KPM_ERR_ARG, "every tile needs at least one row");
+    if (r > sh.tile_rows)
+      return fail(KPM_ERR_ARG, "a tile has " + std::to_string(r) + " rows; this variant takes " +
+                                   std::to_string(sh.tile_rows));
     if (r > max_tile) max_tile = r;
   }
+  // the pattern, read back from the device
+  std::vector<int32_t> indptr(static_cast<size_t>(n_rows) + 1);
+  std::vector<int32_t> indices(static_cast<size_t>(nnz));
+  int rc = check_hip(hipMemcpy(indptr.data(), h->indptr, indptr.size() * 4, hipMemcpyDeviceToHost),
+                     "copy indptr");
+  if (!rc && nnz > 0)
+    rc = check_hip(hipMemcpy(indices.data(), h->indices, indices.size() * 4,
+                             hipMemcpyDeviceToHost),
+                   "copy indices");
+  if (rc) return rc;
+  if (indptr[0] != 0 || int64_t(indptr[n_rows]) != nnz)
+    return fail(KPM_ERR_ARG, "indptr must run from 0 to nnz");
+  for (int32_t r = 0; r < n_rows; ++r)
+    if (indptr[r + 1] < indptr[r]) return fail(KPM_ERR_ARG, "indptr must not decrease");
   // Halo of each tile (the columns its entries reach outside it, ascending)
   // and the local index of every entry: its column's LDS slot.
   std::vector<int32_t> halo_begin(size_t(n_tiles) + 1, 0), halo_rows;
@@ -562,8 +627,8 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
   for (int32_t t = 0; t < n_tiles; ++t) {
     const int32_t r0 = tile_begin[t], r1 = tile_begin[t + 1];
     outside.clear();
-    for (int64_t k = h_indptr[r0]; k < h_indptr[r1]; ++k) {
-      const int32_t j = h_indices[k];
+    for (int64_t k = indptr[r0]; k < indptr[r1]; ++k) {
+      const int32_t j = indices[k];
       if (j < 0 || j >= n_rows) return fail(KPM_ERR_ARG, "column index out of range");
       if (j < r0 || j >= r1) outside.push_back(j);
     }
@@ -572,8 +637,8 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
     const int32_t hcount = int32_t(outside.size());
     halo_rows.insert(halo_rows.end(), outside.begin(), outside.end());
     halo_begin[size_t(t) + 1] = int32_t(halo_rows.size());
-    for (int64_t k = h_indptr[r0]; k < h_indptr[r1]; ++k) {
-      const int32_t j = h_indices[k];
+    for (int64_t k = indptr[r0]; k < indptr[r1]; ++k) {
+      const int32_t j = indices[k];
       if (j >= r0 && j < r1) {
         lidx[size_t(k)] = j - r0;
       } else {
@@ -586,9 +651,9 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
     // what phase A gathers beyond the tile and its halo: the halo rows' reach
     reach.clear();
     for (const int32_t hr : outside) {
-      halo_entries += h_indptr[hr + 1] - h_indptr[hr];
-      for (int64_t k = h_indptr[hr]; k < h_indptr[hr + 1]; ++k) {
-        const int32_t j = h_indices[k];
+      halo_entries += indptr[hr + 1] - indptr[hr];
+      for (int64_t k = indptr[hr]; k < indptr[hr + 1]; ++k) {
+        const int32_t j = indices[k];
         if (j < 0 || j >= n_rows) return fail(KPM_ERR_ARG, "column index out of range");
         if ((j < r0 || j >= r1) && !std::binary_search(outside.begin(), outside.end(), j))
           reach.push_back(j);
@@ -598,10 +663,13 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
     reach.erase(std::unique(reach.begin(), reach.end()), reach.end());
     gather_rows += int64_t(r1 - r0) + hcount + int64_t(reach.size());
   }
+  const int pf = tiled_prefetch(h->row_nnz_hint);
+  const std::vector<int32_t> col_pf = pack_entries(indptr, indices, n_rows, pf);
+  const std::vector<int32_t> lidx_pf = pack_entries(indptr, lidx, n_rows, pf);
   const size_t elem = value_type == KPM_C128 ? 16 : 8;
   // a_{n+1} slots, a_n of the tile rows and halo row ids; at the end the
-  // workgroup reduction of the
-  // dots (2 x 2 TwoSum and 2 grid doubles per wave and column) reuses it
+  // workgroup reduction of the dots (2 x 2 TwoSum and 2 grid doubles per
+  // wave and column) reuses it
   size_t lds_bytes = size_t(slot_cap + max_tile) * wc * elem + size_t(halo_cap) * 4;
   const size_t red_bytes = size_t(sh.waves) * 6 * wc * sizeof(double);
   if (lds_bytes < red_bytes) lds_bytes = red_bytes;
@@ -612,7 +680,9 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
   p->value_type = value_type;
   p->n_tiles = n_tiles;
   p->nnz = nnz;
+  p->values = h->values;
   p->shape = sh;
+  p->pf = pf;
   p->groups = width / wc;
   p->slot_cap = slot_cap;
   p->halo_cap = halo_cap;
@@ -623,7 +693,7 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
   p->lds_bytes = lds_bytes;
 
   int dev = 0, cus = 0, max_lds = 0;
-  int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
   if (!rc)
     rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
                    "hipDeviceGetAttribute");
@@ -633,26 +703,25 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
   if (!rc && lds_bytes > size_t(max_lds))
     rc = fail(KPM_ERR_UNSUPPORTED, "tiles need " + std::to_string(lds_bytes) +
                                        " bytes of LDS (tile rows + halo too large)");
-  // occupancy with the largest LDS request of the modes and depths
+  // occupancy with the LDS request, for the modes of this prefetch depth
   int occ = 1 << 20;
-  for (int mode : {KPM_DOTS_DOUBLING, KPM_DOTS_NONE})
-    for (int pf : kTiledPrefetch) {
-      if (rc) break;
-      const void* k = tiled_kernel_for(value_type, sh, mode, pf);
-      if (!k) {
-        rc = fail(KPM_ERR_UNSUPPORTED, "no tiled kernel for this shape");
-        break;
-      }
-      rc = check_hip(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         int(lds_bytes)),
-                     "hipFuncSetAttribute");
-      int o = 0;
-      if (!rc)
-        rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, sh.waves * 64,
-                                                                    lds_bytes),
-                       "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-      if (!rc && o < occ) occ = o;
+  for (int mode : {KPM_DOTS_DOUBLING, KPM_DOTS_NONE}) {
+    if (rc) break;
+    const void* k = tiled_kernel_for(value_type, sh, mode, pf);
+    if (!k) {
+      rc = fail(KPM_ERR_UNSUPPORTED, "no tiled kernel for this shape");
+      break;
     }
+    rc = check_hip(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(lds_bytes)),
+                   "hipFuncSetAttribute");
+    int o = 0;
+    if (!rc)
+      rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, sh.waves * 64,
+                                                                  lds_bytes),
+                     "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (!rc && o < occ) occ = o;
+  }
   if (!rc && occ < 1) rc = fail(KPM_ERR_UNSUPPORTED, "tiled kernel does not fit on a CU");
   if (rc) {
     delete p;
@@ -667,17 +736,34 @@ int kpm_tiled_create(int32_t n_rows, int32_t width, int32_t value_type, int64_t 
   const int maxnc = 4;  // doubling: 2 TwoSum accumulators + 2 grid sums
   p->part_q = int64_t(p->nb) * maxnc * width * 2;
   const std::vector<int32_t> seg_blk = {0, p->nb};
+  const size_t packed = size_t(n_rows) * pf;
   if (!rc) rc = tdev_alloc(&p->tile_begin, tile_begin.size(), "tile_begin");
   if (!rc) rc = tdev_alloc(&p->halo_begin, halo_begin.size(), "halo_begin");
   if (!rc) rc = tdev_alloc(&p->halo_rows, halo_rows.size(), "halo_rows");
   if (!rc) rc = tdev_alloc(&p->lidx, lidx.size(), "local indices");
+  if (!rc) rc = tdev_alloc(&p->col_pf, packed, "packed columns");
+  if (!rc) rc = tdev_alloc(&p->lidx_pf, packed, "packed local indices");
+  if (!rc) rc = tdev_alloc(reinterpret_cast<char**>(&p->val_pf), packed * elem, "packed values");
   if (!rc) rc = tdev_alloc(&p->seg_blk, seg_blk.size(), "seg_blk");
   if (!rc) rc = tdev_alloc(&p->part, size_t(4 * p->part_q), "tiled partials");
   if (!rc) rc = tupload(p->tile_begin, tile_begin, "upload tile_begin");
   if (!rc) rc = tupload(p->halo_begin, halo_begin, "upload halo_begin");
   if (!rc) rc = tupload(p->halo_rows, halo_rows, "upload halo_rows");
   if (!rc) rc = tupload(p->lidx, lidx, "upload local indices");
+  if (!rc) rc = tupload(p->col_pf, col_pf, "upload packed columns");
+  if (!rc) rc = tupload(p->lidx_pf, lidx_pf, "upload packed local indices");
   if (!rc) rc = tupload(p->seg_blk, seg_blk, "upload seg_blk");
+  if (!rc) {
+    const int64_t total = int64_t(packed);
+    const dim3 grid(unsigned((total + 255) / 256));
+    if (value_type == KPM_C64)
+      hipLaunchKernelGGL(pack_values_kernel<float>, grid, dim3(256), 0, 0, h->indptr, h->values,
+                         p->val_pf, n_rows, pf);
+    else
+      hipLaunchKernelGGL(pack_values_kernel<double>, grid, dim3(256), 0, 0, h->indptr,
+                         h->values, p->val_pf, n_rows, pf);
+    rc = check_hip(hipGetLastError(), "pack_values_kernel launch");
+  }
   if (!rc) rc = check_hip(hipDeviceSynchronize(), "tiled setup");
   if (rc) {
     free_tiled(p);
@@ -723,7 +809,8 @@ int kpm_cheb_steps2(kpm_plan* plan, kpm_tiled* tiles, const kpm_csr* h, void* ri
   if (ps.width != tiles->width) return fail(KPM_ERR_ARG, "plan and tiling widths differ");
   if (ps.n_rows != tiles->n_rows || h->n_rows != tiles->n_rows || h->n_cols != tiles->n_rows)
     return fail(KPM_ERR_ARG, "matrix, plan and tiling row counts differ");
-  if (h->nnz != tiles->nnz) return fail(KPM_ERR_ARG, "matrix nnz differs from the tiling's");
+  if (h->nnz != tiles->nnz || h->values != tiles->values)
+    return fail(KPM_ERR_ARG, "the matrix is not the one the tiling was built from");
   if (!h->indptr || (h->nnz > 0 && (!h->indices || !h->values)))
     return fail(KPM_ERR_ARG, "matrix arrays are NULL");
   if (!ps.scale_set) return fail(KPM_ERR_ARG, "scale not set (kpm_plan_set_scale)");
@@ -743,8 +830,7 @@ int kpm_cheb_steps2(kpm_plan* plan, kpm_tiled* tiles, const kpm_csr* h, void* ri
   if (dots_mode == KPM_DOTS_DOUBLING &&
       (ps.step - out_base < 0 || ps.step + 2 * n_pairs - out_base > out_cap))
     return fail(KPM_ERR_ARG, "dots slots [step - out_base, + 2 n_pairs) outside [0, out_cap)");
-  const int pf = tiled_prefetch(h->row_nnz_hint);
-  const void* kern = tiled_kernel_for(tiles->value_type, tiles->shape, dots_mode, pf);
+  const void* kern = tiled_kernel_for(tiles->value_type, tiles->shape, dots_mode, tiles->pf);
   if (!kern) return fail(KPM_ERR_UNSUPPORTED, "no tiled kernel for this shape");
   // the kernel's dynamic LDS limit follows this tiling (another tiling of the
   // same shape may have set a smaller one)
@@ -789,6 +875,9 @@ int kpm_cheb_steps2(kpm_plan* plan, kpm_tiled* tiles, const kpm_csr* h, void* ri
                     const_cast<int32_t**>(&h->indices),
                     const_cast<void**>(&h->values),
                     &tiles->lidx,
+                    &tiles->col_pf,
+                    &tiles->lidx_pf,
+                    &tiles->val_pf,
                     &n};
     rc = check_hip(hipLaunchKernel(kern, dim3(tiles->nwg), dim3(tiles->shape.waves * 64), args,
                                    tiles->lds_bytes, s),

@@ -92,7 +92,7 @@ def _as_csr(hamiltonian, params=None):
         raise ValueError("'hamiltonian' is neither a matrix nor a Kwant system.")
 
 
-def _tiling_of(hamiltonian, tiles, tile_rows=512):
+def _tiling_of(hamiltonian, tiles, tile_rows=256):
     """The Tiling for ``tiles``: None/False (no tiles), a Tiling, or True
     (a kwant system tiled by its site positions, each site's orbitals kept
     together)."""
@@ -753,20 +753,13 @@ class SpectralDensity:
 
     def _tiles_for(self, width, mode):
         """The two-step tiling of the device matrix for blocks of ``width``
-        vectors (doubling mode only), or None: the first tiled launch shape
-        whose LDS holds a tile and its halo."""
+        vectors (doubling mode only), or None (see autotune.pick_tiled)."""
         if self._tiling is None or mode != eng.DOTS_DOUBLING or \
                 not isinstance(self._dmat, eng.DeviceCSR):
             return None
         if width not in self._tiled:
-            plan = None
-            for v in range(eng.tiled_variant_count(width, self._dmat.value_type)):
-                try:
-                    plan = eng.TiledPlan(self._dmat, width, self._tiling.tile_begin, variant=v)
-                    break
-                except RuntimeError:
-                    continue
-            self._tiled[width] = plan
+            self._tiled[width] = autotune.pick_tiled(self._dmat, (self._a, self._b), width,
+                                                     self._tiling.tile_begin)
         return self._tiled[width]
 
     def _new_block(self, vectors, width):

@@ -30,20 +30,26 @@ def run(mat, ab, block, width, mode, chunks, tiles=None):
 
 
 def check_same(mat, ab, width, prec, tile_begin, chunks=(5, 8, 1, 6), seed=0):
+    """tile_begin: one array for every shape, or a function of the shape's
+    largest tile giving (device matrix, tile boundaries)."""
     e = eng()
     cdt, vt = e.complex_dtype(prec)
-    n = mat.shape[0]
-    block = phase_block(n, width, cdt, seed)
     ran = 0
     for mode in (e.DOTS_DOUBLING, e.DOTS_NONE):
-        ref_dots, ref_last = run(mat, ab, block, width, mode, chunks)
         for v in range(e.tiled_variant_count(width, vt)):
+            m, tb = (mat, tile_begin) if not callable(tile_begin) else \
+                tile_begin(e.tiled_variant_info(width, vt, v)[2])
             try:
-                tiles = e.TiledPlan(mat, width, tile_begin, variant=v)
+                tiles = e.TiledPlan(m, width, tb, variant=v)
             except RuntimeError as err:   # tile + halo larger than this shape's LDS
                 assert "LDS" in str(err), err
                 continue
-            dots, last = run(mat, ab, block, width, mode, chunks, tiles=tiles)
+            except ValueError as err:     # a tile larger than this shape takes
+                assert "this variant takes" in str(err), err
+                continue
+            block = phase_block(m.shape[0], width, cdt, seed)
+            ref_dots, ref_last = run(m, ab, block, width, mode, chunks)
+            dots, last = run(m, ab, block, width, mode, chunks, tiles=tiles)
             for a, b in zip(dots, ref_dots):
                 if mode == e.DOTS_DOUBLING:
                     assert np.array_equal(a, b), (prec, width, v)
@@ -51,6 +57,7 @@ def check_same(mat, ab, width, prec, tile_begin, chunks=(5, 8, 1, 6), seed=0):
             assert np.array_equal(last[1], ref_last[1]), (prec, width, v, mode)
             ran += 1
     assert ran > 0
+    return ran
 
 
 @pytest.mark.parametrize("prec", ["complex64", "complex128"])
@@ -60,10 +67,15 @@ def test_tiled_graphene_bitwise(prec, width):
     from kwant_amd.ordering import tile_order
     e = eng()
     h, pos = honeycomb(40, disorder=0.5, seed=3)
-    tiling = tile_order(pos, 256)
-    mat = e.DeviceCSR(h, prec, order=tiling.order)
-    print("graphene 40x40, %s, width %d, %d tiles" % (prec, width, tiling.n_tiles), flush=True)
-    check_same(mat, (3.5, 0.1), width, prec, tiling.tile_begin)
+
+    def tiled(rows):
+        tiling = tile_order(pos, rows)
+        print("graphene 40x40, %s, width %d, %d tiles of <= %d rows"
+              % (prec, width, tiling.n_tiles, rows), flush=True)
+        return e.DeviceCSR(h, prec, order=tiling.order), tiling.tile_begin
+    ran = check_same(None, (3.5, 0.1), width, prec, tiled)
+    _, vt = e.complex_dtype(prec)
+    assert ran == 2 * e.tiled_variant_count(width, vt)   # every shape, both modes
 
 
 @pytest.mark.parametrize("prec", ["complex64", "complex128"])
