@@ -145,10 +145,26 @@ def tile_order(coords, tile_rows=512, iterations=4):
     sorted_key = key[order]
     starts = np.flatnonzero(np.r_[True, sorted_key[1:] != sorted_key[:-1]])
     ends = np.r_[starts[1:], n]
+    # rows at the same point (the orbitals of one site) stay in one tile
+    xs = x[order]
+    site_start = np.flatnonzero(np.r_[True, np.any(xs[1:] != xs[:-1], axis=1)])
     bounds = [0]
     for s, e in zip(starts, ends):
-        pieces = -(-(e - s) // tile_rows)
-        step = -(-(e - s) // pieces)
-        bounds.extend(range(s + step, e, step))
+        if e - s > tile_rows:
+            # a box with more rows than a tile: pieces of about equal size, cut
+            # between sites (a site larger than a tile is cut inside)
+            target = -(-(e - s) // -(-(e - s) // tile_rows))
+            inner = site_start[(site_start > s) & (site_start < e)]
+            begin = prev = s
+            for b in list(inner) + [e]:
+                b = int(b)
+                while b - begin > tile_rows and prev == begin:
+                    begin += tile_rows
+                    bounds.append(begin)
+                    prev = begin
+                if b - begin > target and prev > begin:
+                    bounds.append(prev)
+                    begin = prev
+                prev = b
         bounds.append(e)
     return Tiling(order, np.asarray(bounds, dtype=np.int64))

@@ -78,3 +78,49 @@ def test_no_gpu_no_fallback():
     h = sp.diags([np.ones(9), np.ones(9)], [1, -1], format='csr', dtype=complex)
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         kpm.SpectralDensity(h, num_moments=10, num_vectors=1, bounds=(-2.1, 2.1))
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_tiled_argument_errors_without_gpu():
+    """The two-step entry points validate their arguments before any device
+    work: shapes, NULL arrays, tile boundaries."""
+    import torch  # noqa: F401
+    from kwant_amd._engine import KpmCsr
+    lib = ctypes.CDLL(LIB)
+    lib.kpm_last_error.restype = ctypes.c_char_p
+    count = ctypes.c_int32()
+    assert lib.kpm_tiled_variant_count(64, 1, ctypes.byref(count)) == 0 and count.value >= 3
+    assert lib.kpm_tiled_variant_count(3, 1, ctypes.byref(count)) == 0 and count.value == 0
+    gw, waves, rows = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    for v in range(lib_count(lib, 64, 1)):
+        assert lib.kpm_tiled_variant_info(64, 1, v, ctypes.byref(gw), ctypes.byref(waves),
+                                          ctypes.byref(rows)) == 0
+        # the largest tile is four workgroup iterations of rows
+        lpr = gw.value // 2
+        assert rows.value == 4 * waves.value * 64 // lpr
+    assert lib.kpm_tiled_variant_info(64, 1, 99, ctypes.byref(gw), ctypes.byref(waves),
+                                      ctypes.byref(rows)) == 1
+    out = ctypes.c_void_p()
+    tb = (ctypes.c_int32 * 3)(0, 5, 10)
+    csr = KpmCsr(10, 10, 30, None, None, None, 1, 4)
+    rc = lib.kpm_tiled_create(ctypes.byref(csr), 64, 2, tb, -1, ctypes.byref(out))
+    assert rc == 1 and b"NULL" in lib.kpm_last_error()
+    dummy = (ctypes.c_int32 * 1)(0)
+    csr = KpmCsr(10, 10, 30, ctypes.addressof(dummy), ctypes.addressof(dummy),
+                 ctypes.addressof(dummy), 1, 4)
+    bad = (ctypes.c_int32 * 3)(0, 5, 9)
+    rc = lib.kpm_tiled_create(ctypes.byref(csr), 64, 2, bad, -1, ctypes.byref(out))
+    assert rc == 1 and b"cover" in lib.kpm_last_error()
+    empty = (ctypes.c_int32 * 3)(0, 10, 10)
+    rc = lib.kpm_tiled_create(ctypes.byref(csr), 64, 2, empty, -1, ctypes.byref(out))
+    assert rc == 1 and b"at least one row" in lib.kpm_last_error()
+    rc = lib.kpm_tiled_create(ctypes.byref(csr), 3, 2, tb, -1, ctypes.byref(out))
+    assert rc == 3  # no tiled kernel for width 3
+    rc = lib.kpm_cheb_steps2(None, None, None, None, 0, 4, None, 0, 0, 1, 1, None)
+    assert rc == 1
+
+
+def lib_count(lib, width, value_type):
+    count = ctypes.c_int32()
+    lib.kpm_tiled_variant_count(width, value_type, ctypes.byref(count))
+    return count.value

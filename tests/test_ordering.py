@@ -52,3 +52,77 @@ def test_rcm_restores_locality_and_auto_decides():
     assert np.array_equal(explicit, np.arange(n)[::-1])
     with pytest.raises(ValueError):
         ordering.choose_order(h, 'sometimes')
+
+
+def _halo_rows(h, tile_begin):
+    """Rows outside each tile that its entries reach, summed over tiles
+    (the halo the two-step kernel recomputes)."""
+    sizes = np.diff(tile_begin)
+    tile_of = np.repeat(np.arange(len(sizes)), sizes)
+    coo = sp.csr_matrix(h).tocoo()
+    out = tile_of[coo.row] != tile_of[coo.col]
+    pairs = np.unique(tile_of[coo.row[out]].astype(np.int64) * h.shape[0] + coo.col[out])
+    return len(pairs)
+
+
+def test_tile_order_groups_rows_into_compact_tiles():
+    h, pos = square(64, disorder=1.0, seed=1)
+    n = h.shape[0]
+    tiling = ordering.tile_order(pos, 256)
+    assert sorted(tiling.order) == list(range(n))
+    sizes = np.diff(tiling.tile_begin)
+    assert sizes.min() > 0 and sizes.max() <= 256
+    # 16 x 16 boxes of a square lattice: 4 x 16 boundary neighbours per box,
+    # against about 2 x 256 for tiles of 256 consecutive rows of the row-major
+    # order (the rows above and below)
+    hp = ordering.permute_csr(h, tiling.order)
+    tiled = _halo_rows(hp, tiling.tile_begin)
+    strips = _halo_rows(h, np.arange(0, n + 1, 256))
+    assert tiled <= 0.3 * n
+    assert tiled < 0.6 * strips
+    # the rows of a tile keep their relative order
+    for t in range(tiling.n_tiles):
+        rows = tiling.order[tiling.tile_begin[t]:tiling.tile_begin[t + 1]]
+        assert np.all(np.diff(rows) > 0)
+
+
+def test_tile_order_graphene_and_three_dimensions():
+    _, pos = honeycomb(50, disorder=0.5, seed=2)
+    tiling = ordering.tile_order(pos, 128)
+    assert np.diff(tiling.tile_begin).max() <= 128
+    rng = np.random.RandomState(0)
+    cloud = rng.uniform(0, 10, (5000, 3))
+    tiling3 = ordering.tile_order(cloud, 100)
+    assert sorted(tiling3.order) == list(range(5000))
+    assert np.diff(tiling3.tile_begin).max() <= 100
+    # orbitals of one site (equal positions) stay together when they fit a tile
+    four = np.repeat(cloud[:500], 4, axis=0)
+    t4 = ordering.tile_order(four, 64)
+    site_of = t4.order // 4
+    starts = t4.tile_begin[:-1]
+    for s, e in zip(starts, t4.tile_begin[1:]):
+        counts = np.bincount(site_of[s:e])
+        assert np.all(counts[counts > 0] == 4)
+
+
+def test_tiling_validation():
+    with pytest.raises(ValueError):
+        ordering.Tiling([0, 1, 2], [0, 2])               # does not reach the row count
+    with pytest.raises(ValueError):
+        ordering.Tiling([0, 1, 2], [0, 2, 2, 3])         # an empty tile
+    with pytest.raises(ValueError):
+        ordering.tile_order(np.zeros((0, 2)), 16)
+    with pytest.raises(ValueError):
+        ordering.tile_order(np.zeros((4, 2)), 0)
+    t = ordering.Tiling([2, 0, 1], [0, 1, 3])
+    assert t.n_tiles == 2
+
+
+def test_tiles_argument_needs_positions():
+    from kwant_amd import kpm
+    h, _ = square(8, disorder=1.0, seed=1)
+    with pytest.raises(ValueError, match="site positions"):
+        kpm._tiling_of(h, True)
+    with pytest.raises(ValueError):
+        kpm._tiling_of(h, 'auto')
+    assert kpm._tiling_of(h, None) is None and kpm._tiling_of(h, False) is None
