@@ -97,7 +97,7 @@ def read_traffic(variant):
 
 
 def two_step_bench(h, pos, ab, block, width, warmup, steps, tile_rows, variant,
-                   one_step_variant, log_fn):
+                   one_step_variant, log_fn, precision="complex64"):
     """The same expansion with two Chebyshev steps per launch: the flake's
     rows in a tiled order (kwant_amd.ordering.tile_order, tiles of about
     ``tile_rows`` rows), the vectors permuted alike.  Every launch shape whose
@@ -111,12 +111,14 @@ def two_step_bench(h, pos, ab, block, width, warmup, steps, tile_rows, variant,
     n, nnz = h.shape[0], h.nnz
     t0 = time.time()
     tiling = tile_order(pos, tile_rows)
-    mat = eng.DeviceCSR(h, "complex64", order=tiling.order)
-    blk = block[torch.from_numpy(tiling.order).to(block.device)]
-    log_fn("two-step: %d tiles of <= %d rows (%.1fs)"
-           % (tiling.n_tiles, int(np.max(np.diff(tiling.tile_begin))), time.time() - t0))
-    shapes = [variant] if variant >= 0 else list(range(eng.tiled_variant_count(width,
-                                                                                eng.KPM_C64)))
+    cdt, vt = eng.complex_dtype(precision)
+    s_v = 16 if vt == eng.KPM_C128 else 8
+    mat = eng.DeviceCSR(h, precision, order=tiling.order)
+    blk = block[torch.from_numpy(tiling.order).to(block.device)].to(cdt)
+    log_fn("two-step, %s: %d tiles of <= %d rows (%.1fs)"
+           % (precision, tiling.n_tiles, int(np.max(np.diff(tiling.tile_begin))),
+              time.time() - t0))
+    shapes = [variant] if variant >= 0 else list(range(eng.tiled_variant_count(width, vt)))
     tried = {}
     best = None
     for v in shapes:
@@ -165,9 +167,10 @@ def two_step_bench(h, pos, ab, block, width, warmup, steps, tile_rows, variant,
     torch.cuda.synchronize()
     ms_launch = start.elapsed_time(end) / pairs
     assert rec.plan.get_step() == rec.n + 2 * pairs
-    kb = tp.kernel_bytes(nnz, 8, 8)
-    gw, waves, _ = eng.tiled_variant_info(width, eng.KPM_C64, v)
-    out = {"ms_per_step": ms_launch / 2, "ms_per_launch": ms_launch, "steps": 2 * pairs,
+    kb = tp.kernel_bytes(nnz, s_v, s_v)
+    gw, waves, _ = eng.tiled_variant_info(width, vt, v)
+    out = {"precision": precision, "ms_per_step": ms_launch / 2, "ms_per_launch": ms_launch,
+           "steps": 2 * pairs,
            "variant": v, "group_width": gw, "waves": waves, "tile_rows": tile_rows,
            "tiles": info["tiles"], "max_tile_rows": info["max_tile_rows"],
            "halo_rows": info["halo_rows"], "gather_rows": info["gather_rows"],
@@ -177,9 +180,9 @@ def two_step_bench(h, pos, ab, block, width, warmup, steps, tile_rows, variant,
            "kernel_frac_of_hbm_peak": kb / (ms_launch * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "shapes_ms_per_step": {str(k): t for k, t in tried.items()},
            "dots_bitwise_vs_one_step": bitwise, "checked_steps": check}
-    log_fn("two-step shape %d: %.4f ms/step (%.4f ms per launch), kernel bytes %.3f GB per "
-           "launch = %.0f GB/s; dots bitwise vs one-step: %s"
-           % (v, ms_launch / 2, ms_launch, kb / 1e9, out["kernel_GBs"], bitwise))
+    log_fn("two-step shape %d, %s: %.4f ms/step (%.4f ms per launch), kernel bytes %.3f GB "
+           "per launch = %.0f GB/s; dots bitwise vs one-step: %s"
+           % (v, precision, ms_launch / 2, ms_launch, kb / 1e9, out["kernel_GBs"], bitwise))
     del rec, dots, mat, blk
     return out
 
@@ -1019,6 +1022,18 @@ def main():
                  "c128_steps_per_s": full / t128}
         log("c64 vs c128 moments: max |d mu| / mu0 = %.3e, DOS L1 %.1e; c128 %.0f steps/s"
             % (err, l1, full / t128))
+        if args.tile_rows > 0:
+            c128_two = two_step_bench(h, pos, (a, b), block, width, args.warmup, 256,
+                                      args.tile_rows, -1, -1, log, precision="complex128")
+            torch.cuda.empty_cache()
+            if c128_two is not None:
+                check["c128_two_step"] = {k: c128_two[k] for k in (
+                    "ms_per_step", "variant", "group_width", "waves", "kernel_frac_of_hbm_peak",
+                    "dots_bitwise_vs_one_step")}
+                check["c128_two_step"]["steps_per_s"] = 1000.0 / c128_two["ms_per_step"]
+                check["c128_two_step"]["frac_of_hbm_peak"] = (
+                    bytes_per_step(n, nnz, width, 16, 16) / (c128_two["ms_per_step"] * 1e-3)
+                    / 1e9 / HBM_PEAK_GBS)
 
     block64 = block.to(torch.complex64)
     del block

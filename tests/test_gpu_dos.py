@@ -1,6 +1,6 @@
 """GPU parity of the SpectralDensity path against the CPU oracle and the dense
 brute force.  Tolerances: complex128 moments within 1e-11 of the oracle
-relative to mu_0 (the BASELINE target), complex64 within 1e-5; the
+relative to mu_0 (the FP64 parity target), complex64 within 1e-5; the
 reference's own TOL=1e-12 / TOL_SP=1e-6 for the mirrored API tests
 (kwant/tests/test_kpm.py:38-40)."""
 import numpy as np
