@@ -669,42 +669,68 @@ def assembly_bench(cells, sample_cells, log_fn):
             "loop_sample_cells": sample_cells, "max_abs_diff_sample": err}
 
 
-def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
-    """The batched disorder ensemble: ``count`` cubic 4-orbital systems with
-    L ~ U{20..60}, generated on the GPU, ``moments`` moments with ``width``
-    vectors each, every step one launch over all systems, the step sequence
-    replayed from a captured HIP graph."""
-    import torch
-    from kwant_amd.ensemble import CubicEnsemble, ensemble_sizes
-    sizes = ensemble_sizes(count, 20, 60, seed=0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ens = CubicEnsemble(sizes, seed=0, num_moments=moments, num_vectors=width, variant=variant)
-    torch.cuda.synchronize()
-    t_setup = time.perf_counter() - t0
+def _time_ensemble(ens, graph):
+    """(eager seconds, graph seconds) of one expansion of ``ens``."""
     t_eager = ens.run(graph=False)
     if graph:
         ens.run(graph=True)                  # capture + first replay
         t_graph = min(ens.run(graph=True) for _ in range(2))
     else:
         t_graph = min(t_eager, ens.run(graph=False))
-    bstep = ens.bytes_per_step()
-    ms_step = 1e3 * t_graph / ens.steps
-    achieved = bstep / (ms_step * 1e-3) / 1e9
-    log_fn("ensemble: %d systems, N=%d, nnz=%d, W=%d: %.3f ms/step (graph), eager %.3f "
-           "ms/step, %.0f GB/s (%.1f%% of 8 TB/s); setup %.1f s"
-           % (count, ens.n, ens.nnz, width, ms_step, 1e3 * t_eager / ens.steps, achieved,
-              100 * achieved / HBM_PEAK_GBS, t_setup))
+    return t_eager, t_graph
+
+
+def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
+    """The batched disorder ensemble: ``count`` cubic 4-orbital systems with
+    L ~ U{20..60}, generated on the GPU, ``moments`` moments with ``width``
+    vectors each, every step one launch over all systems, the step sequence
+    replayed from a captured HIP graph.  Both matrix formats run: the CSR
+    (the general recursion kernel) and the lattice stencil (on-site blocks
+    only, each system marched along x); their dots are compared bitwise, and
+    the faster is the configuration's figure."""
+    import numpy as np
+    import torch
+    from kwant_amd.ensemble import CubicEnsemble, ensemble_sizes
+    sizes = ensemble_sizes(count, 20, 60, seed=0)
+    res, dots = {}, {}
+    for fmt in ("stencil", "csr"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens = CubicEnsemble(sizes, seed=0, num_moments=moments, num_vectors=width,
+                            variant=variant if fmt == "csr" else -1, matrix_format=fmt)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        t_eager, t_graph = _time_ensemble(ens, graph)
+        dots[fmt] = ens.dots.cpu().numpy()
+        bstep = ens.bytes_per_step()
+        ms_step = 1e3 * t_graph / ens.steps
+        achieved = bstep / (ms_step * 1e-3) / 1e9
+        res[fmt] = {"ms_per_step": ms_step, "ms_per_step_eager": 1e3 * t_eager / ens.steps,
+                    "expansion_s": t_graph, "setup_s": t_setup, "achieved_GBs": achieved,
+                    "frac_of_hbm_peak": achieved / HBM_PEAK_GBS, "variant": ens.variant}
+        if fmt == "stencil":
+            own = ens.stencil_bytes_per_step()
+            res[fmt]["kernel_bytes_per_step"] = own
+            res[fmt]["kernel_frac_of_hbm_peak"] = own / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+        log_fn("ensemble (%s): %d systems, N=%d, W=%d: %.3f ms/step (graph), eager %.3f "
+               "ms/step, %.0f GB/s of the path's bytes (%.1f%% of 8 TB/s); setup %.1f s"
+               % (fmt, count, ens.n, width, ms_step, 1e3 * t_eager / ens.steps, achieved,
+                  100 * achieved / HBM_PEAK_GBS, t_setup))
+        n_rows, nnz, steps = ens.n, ens.nnz, ens.steps
+        del ens
+        torch.cuda.empty_cache()
+    bitwise = bool(np.array_equal(dots["stencil"], dots["csr"]))
+    log_fn("ensemble: stencil dots bitwise those of the CSR path: %s" % bitwise)
+    best = min(res, key=lambda f: res[f]["ms_per_step"])
     out = {"workload": "%d cubic 4-orbital systems, L ~ U{20..60}, sigma 0.3, t 1, %d moments, "
                        "%d vectors each, complex64, one launch per step for all systems"
                        % (count, moments, width),
-           "rows": ens.n, "nnz": ens.nnz, "steps": ens.steps, "ms_per_step": ms_step,
-           "ms_per_step_eager": 1e3 * t_eager / ens.steps,
-           "expansion_s": t_graph, "setup_s": t_setup,
-           "algorithmic_bytes_per_step": bstep, "achieved_GBs": achieved,
-           "frac_of_hbm_peak": achieved / HBM_PEAK_GBS, "variant": ens.variant}
-    del ens
-    torch.cuda.empty_cache()
+           "rows": n_rows, "nnz": nnz, "steps": steps, "matrix_format": best,
+           "algorithmic_bytes_per_step": bytes_per_step(n_rows, nnz, width, 8, 8),
+           "dots_bitwise_stencil_vs_csr": bitwise}
+    out.update(res[best])
+    out["csr"] = res["csr"]
+    out["stencil"] = res["stencil"]
     return out
 
 

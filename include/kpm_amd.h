@@ -429,6 +429,39 @@ int kpm_gen_cubic4(int32_t L, uint64_t seed, double sigma, double t, int64_t row
                    int64_t nz0, int32_t* indptr, int32_t* indices, void* values,
                    int32_t value_type, unsigned int* radius_bits, kpm_stream_t stream);
 
+/* The same system's on-site blocks alone, for the lattice-stencil recursion:
+ * onsite[(row0 + r) * 4 + j] = A_s[o, j] (complex of value_type) for the
+ * system's rows r = 4 s + o -- bitwise the values kpm_gen_cubic4 writes --
+ * and the same Gershgorin radius.  Asynchronous on stream. */
+int kpm_gen_cubic4_onsite(int32_t L, uint64_t seed, double sigma, double t, int64_t row0,
+                          void* onsite, int32_t value_type, unsigned int* radius_bits,
+                          kpm_stream_t stream);
+
+/* The Chebyshev steps of an ensemble of such systems without a stored matrix
+ * (the lattice stencil): kpm_cheb_steps (kwant/kpm.py:464-473 with the
+ * rescaled matvec of kwant/kpm.py:1182-1183) for the block-diagonal matrix
+ * of the systems of sides h_sides[0 .. n_systems), stacked in that order,
+ * with on-site blocks `onsite` (n_rows x 4 complex, kpm_gen_cubic4_onsite)
+ * and hopping t I_4.  A workgroup marches columns (y, z, orbital) of one
+ * system along x, keeping a_n of the x neighbours in registers; the row sums
+ * take the CSR's entries in its column order, so vectors and dots are bitwise
+ * those of kpm_cheb_steps on kpm_gen_cubic4's CSR.  kpm_march_create
+ * builds the work list (waves: 4 or 8 per workgroup; depth: planes, 1 or 2,
+ * that the streams are requested ahead, 3 or 4 the same with each plane's
+ * a_n exchanged through LDS inside the workgroup); kpm_cheb_steps_march
+ * runs n_steps steps from the plan's counter (the plan: one segment per
+ * system, this row count, width and value type; scale, and for
+ * KPM_DOTS_DOUBLING the dots grid, set).  Dots as kpm_cheb_steps; modes
+ * KPM_DOTS_DOUBLING and KPM_DOTS_NONE. */
+typedef struct kpm_march kpm_march;
+int kpm_march_create(int32_t n_systems, const int32_t* h_sides, int32_t width,
+                     int32_t value_type, int32_t waves, int32_t depth, kpm_march** out);
+int kpm_march_destroy(kpm_march* m);
+int kpm_cheb_steps_march(kpm_plan* plan, kpm_march* m, const void* onsite, double t,
+                         void* ring, int64_t slot_stride, int32_t slots, double* d_dots,
+                         int32_t out_base, int32_t out_cap, int32_t n_steps, int32_t dots_mode,
+                         kpm_stream_t stream);
+
 /* ---- moment surrogate (no counterpart in kwant/kpm.py) ------------------
  * A small MLP that maps Hamiltonian parameters to Jackson-damped moments,
  * trained on moments from kpm_cheb_steps and evaluated in place of the

@@ -44,7 +44,8 @@ EXPORTS = (
     "kpm_bond_records", "kpm_spmm", "kpm_corr_gemm", "kpm_corr_gemm_sliced_work", "kpm_corr_gemm_sliced",
     "kpm_corr_gemm_islice_work", "kpm_corr_gemm_islice", "kpm_islice_prep_bytes",
     "kpm_islice_prep_rows", "kpm_corr_gemm_islice_prep_work", "kpm_corr_gemm_islice_prep",
-    "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4",
+    "kpm_moment_shift", "kpm_cubic4_nnz", "kpm_gen_cubic4", "kpm_gen_cubic4_onsite",
+    "kpm_march_create", "kpm_march_destroy", "kpm_cheb_steps_march",
     "kpm_mlp_gemm", "kpm_mlp_colsum", "kpm_mlp_mse",
     "kpm_mlp_ensemble", "kpm_mlp_adam",
 )
@@ -168,6 +169,12 @@ def load_library(path=LIB_PATH):
         "kpm_cubic4_nnz": (ctypes.c_int, [i32, ctypes.POINTER(ctypes.c_int64)]),
         "kpm_gen_cubic4": (ctypes.c_int, [i32, ctypes.c_uint64, ctypes.c_double,
                                           ctypes.c_double, i64, i64, vp, vp, vp, i32, vp, vp]),
+        "kpm_gen_cubic4_onsite": (ctypes.c_int, [i32, ctypes.c_uint64, ctypes.c_double,
+                                                 ctypes.c_double, i64, vp, i32, vp, vp]),
+        "kpm_march_create": (ctypes.c_int, [i32, pi32, i32, i32, i32, i32, ctypes.POINTER(vp)]),
+        "kpm_march_destroy": (ctypes.c_int, [vp]),
+        "kpm_cheb_steps_march": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, vp, i64, i32, vp,
+                                                i32, i32, i32, i32, vp]),
         "kpm_mlp_gemm": (ctypes.c_int, [ctypes.POINTER(MlpGemmDesc), vp]),
         "kpm_mlp_colsum": (ctypes.c_int, [vp, i64, i64, i32, i32, i32, vp, i64, vp]),
         "kpm_mlp_mse": (ctypes.c_int, [vp, vp, vp, i64, f32, vp, i32, vp]),
@@ -870,6 +877,62 @@ def gen_cubic4(L, seed, sigma, t, row0, nz0, indptr, indices, values, radius_bit
                                         indices.data_ptr(), values.data_ptr(), vt,
                                         radius_bits.data_ptr(), stream_handle()),
           "kpm_gen_cubic4")
+
+
+def gen_cubic4_onsite(L, seed, sigma, t, row0, onsite, radius_bits):
+    """The on-site blocks of one cubic 4-orbital system, rows row0 ..
+    row0 + 4 L^3 of the (N, 4) complex device array ``onsite`` (see
+    kpm_gen_cubic4_onsite)."""
+    import torch
+    if radius_bits.dtype != torch.int32 or radius_bits.numel() != 1:
+        raise ValueError("radius_bits must be one int32 element")
+    if onsite.dim() != 2 or onsite.shape[1] != 4 or not onsite.is_contiguous():
+        raise ValueError("onsite must be a contiguous (N, 4) complex tensor")
+    _, vt = complex_dtype(onsite.dtype)
+    if row0 + 4 * L ** 3 > onsite.shape[0]:
+        raise ValueError("cubic4 block outside the on-site buffer")
+    check(load_library().kpm_gen_cubic4_onsite(int(L), int(seed) & (2 ** 64 - 1), float(sigma),
+                                               float(t), int(row0), onsite.data_ptr(), vt,
+                                               radius_bits.data_ptr(), stream_handle()),
+          "kpm_gen_cubic4_onsite")
+
+
+class MarchPlan:
+    """Work list of the lattice-stencil steps for cubic 4-orbital systems of
+    the given sides (kpm_march_create)."""
+
+    # (waves per workgroup, depth) of each shape: depth 1 or 2 planes of
+    # prefetch, 3 or 4 the same with the plane exchanged through LDS
+    SHAPES = ((4, 1), (8, 1), (4, 2), (8, 2), (4, 3), (8, 3), (4, 4), (8, 4))
+
+    def __init__(self, sides, width, value_type, shape=0):
+        lib = load_library()
+        sd = np.ascontiguousarray(sides, dtype=np.int32)
+        waves, depth = self.SHAPES[int(shape)]
+        handle = ctypes.c_void_p()
+        check(lib.kpm_march_create(len(sd), sd.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                   int(width), int(value_type), waves, depth,
+                                   ctypes.byref(handle)), "kpm_march_create")
+        self.handle = handle
+        self.width, self.value_type = int(width), int(value_type)
+        self.shape, self.waves, self.depth = int(shape), waves, depth
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            try:
+                _lib.kpm_march_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def steps(self, plan, onsite, t, ring, slot_stride, slots, n_steps, mode, dots=None,
+              out_base=0):
+        out_cap = 0 if dots is None else int(dots.shape[0])
+        check(load_library().kpm_cheb_steps_march(
+            plan.handle, self.handle, onsite.data_ptr(), float(t), ring.data_ptr(), slot_stride,
+            slots, None if dots is None else dots.data_ptr(), out_base, out_cap, n_steps, mode,
+            stream_handle()), "kpm_cheb_steps_march")
 
 
 def _f32_view(t, name, dims=3):

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkpm_amd.so")
 SOURCES = ["kpm_capi.hip", "kpm_cheb.hip", "kpm_gemm.hip", "kpm_mlp.hip", "kpm_shift.hip",
-           "kpm_lattice.hip", "kpm_sliced.hip", "kpm_static.hip", "kpm_tiled.hip"]
+           "kpm_lattice.hip", "kpm_sliced.hip", "kpm_static.hip", "kpm_tiled.hip",
+           "kpm_march.hip"]
 ARCH = os.environ.get("KPM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -132,7 +132,7 @@ extern "C" {
 
 const char* kpm_last_error(void) { return g_last_error.c_str(); }
 
-int kpm_abi_version(void) { return 106; }
+int kpm_abi_version(void) { return 107; }
 
 int kpm_device_info(int32_t device, int32_t* num_cu, int32_t* gfx_arch) {
   hipDeviceProp_t prop;

@@ -55,6 +55,27 @@ __host__ __device__ inline int64_t cubic4_entries_before(int64_t L, int64_t x, i
   return 4 * (4 * s + nb);
 }
 
+// On-site entry (o, j) of site s: (X + X^H)_oj / 2 with X_ij = sigma (g_re + i g_im).
+__device__ inline void cubic4_onsite(uint64_t seed, double sigma, int64_t s, int o, int j,
+                                     double& ar, double& ai) {
+  const uint64_t kij = uint64_t(16 * s + 4 * o + j) * 2;
+  const uint64_t kji = uint64_t(16 * s + 4 * j + o) * 2;
+  const double xr = sigma * std_normal(seed, kij), xi = sigma * std_normal(seed, kij + 1);
+  const double yr = sigma * std_normal(seed, kji), yi = sigma * std_normal(seed, kji + 1);
+  ar = 0.5 * (xr + yr);
+  ai = 0.5 * (xi - yi);
+}
+
+// wavefront maximum of a non-negative row sum, then one atomic per wavefront;
+// non-negative floats order like their bit patterns, rounded up so the
+// radius is an upper bound
+__device__ inline void radius_max(double rowsum, unsigned int* radius_bits) {
+  float v = float(rowsum) * (1.0f + 1e-6f);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(radius_bits, __float_as_uint(v));
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void cubic4_kernel(int L, uint64_t seed, double sigma,
                                                      double t, int64_t row0, int64_t nz0,
@@ -84,11 +105,8 @@ __global__ __launch_bounds__(256) void cubic4_kernel(int L, uint64_t seed, doubl
     if (y > 0) emit(4 * (s - L) + o, t, 0.0);
     if (z > 0) emit(4 * (s - 1) + o, t, 0.0);
     for (int j = 0; j < 4; ++j) {
-      const uint64_t kij = uint64_t(16 * s + 4 * o + j) * 2;
-      const uint64_t kji = uint64_t(16 * s + 4 * j + o) * 2;
-      const double xr = sigma * std_normal(seed, kij), xi = sigma * std_normal(seed, kij + 1);
-      const double yr = sigma * std_normal(seed, kji), yi = sigma * std_normal(seed, kji + 1);
-      const double ar = 0.5 * (xr + yr), ai = 0.5 * (xi - yi);  // (X + X^H)_oj / 2
+      double ar, ai;
+      cubic4_onsite(seed, sigma, s, o, j, ar, ai);
       emit(4 * s + j, ar, ai);
       rowsum += sqrt(ar * ar + ai * ai);
     }
@@ -97,12 +115,35 @@ __global__ __launch_bounds__(256) void cubic4_kernel(int L, uint64_t seed, doubl
     if (x < L - 1) emit(4 * (s + LL) + o, t, 0.0);
     rowsum += deg * fabs(t);
   }
-  // wavefront maximum, then one atomic per wavefront; non-negative floats
-  // order like their bit patterns, rounded up so the radius is an upper bound
-  float v = float(rowsum) * (1.0f + 1e-6f);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(radius_bits, __float_as_uint(v));
+  radius_max(rowsum, radius_bits);
+}
+
+// The same system as cubic4_kernel for the lattice-stencil recursion: only
+// the on-site blocks, onsite[4 (row0 + r) + j] = A_s[o, j] for row r = 4 s + o
+// (the hoppings t I_4 are implicit), and the same Gershgorin radius.
+template <typename T>
+__global__ __launch_bounds__(256) void cubic4_onsite_kernel(int L, uint64_t seed, double sigma,
+                                                            double t, int64_t row0,
+                                                            T* __restrict__ onsite,
+                                                            unsigned int* __restrict__ radius_bits) {
+  const int64_t n = 4 * int64_t(L) * L * L;
+  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double rowsum = 0.0;
+  if (r < n) {
+    const int64_t s = r >> 2;
+    const int o = int(r & 3);
+    const int64_t z = s % L, y = (s / L) % L, x = s / (int64_t(L) * L);
+    const int deg = (x > 0) + (x < L - 1) + (y > 0) + (y < L - 1) + (z > 0) + (z < L - 1);
+    for (int j = 0; j < 4; ++j) {
+      double ar, ai;
+      cubic4_onsite(seed, sigma, s, o, j, ar, ai);
+      onsite[2 * ((row0 + r) * 4 + j)] = T(ar);
+      onsite[2 * ((row0 + r) * 4 + j) + 1] = T(ai);
+      rowsum += sqrt(ar * ar + ai * ai);
+    }
+    rowsum += deg * fabs(t);
+  }
+  radius_max(rowsum, radius_bits);
 }
 
 }  // namespace
@@ -140,6 +181,26 @@ int kpm_gen_cubic4(int32_t L, uint64_t seed, double sigma, double t, int64_t row
   else
     return fail(KPM_ERR_ARG, "unknown value type");
   return check_hip(hipGetLastError(), "cubic4 generator launch");
+}
+
+int kpm_gen_cubic4_onsite(int32_t L, uint64_t seed, double sigma, double t, int64_t row0,
+                          void* onsite, int32_t value_type, unsigned int* radius_bits,
+                          kpm_stream_t stream) {
+  if (!onsite || !radius_bits) return fail(KPM_ERR_ARG, "NULL buffer");
+  if (L < 1 || row0 < 0) return fail(KPM_ERR_ARG, "bad cubic4 block position");
+  const int64_t n = 4 * int64_t(L) * L * L;
+  if (row0 + n >= (int64_t(1) << 31)) return fail(KPM_ERR_ARG, "ensemble too large");
+  const dim3 grid(unsigned((n + 255) / 256)), block(256);
+  hipStream_t s = (hipStream_t)stream;
+  if (value_type == KPM_C64)
+    hipLaunchKernelGGL(cubic4_onsite_kernel<float>, grid, block, 0, s, L, seed, sigma, t, row0,
+                       static_cast<float*>(onsite), radius_bits);
+  else if (value_type == KPM_C128)
+    hipLaunchKernelGGL(cubic4_onsite_kernel<double>, grid, block, 0, s, L, seed, sigma, t, row0,
+                       static_cast<double*>(onsite), radius_bits);
+  else
+    return fail(KPM_ERR_ARG, "unknown value type");
+  return check_hip(hipGetLastError(), "cubic4 on-site generator launch");
 }
 
 }  // extern "C"

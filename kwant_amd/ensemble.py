@@ -30,15 +30,26 @@ class CubicEnsemble:
     length in ``sizes``, ``num_vectors`` random-phase vectors each."""
 
     def __init__(self, sizes, seed=0, sigma=0.3, t=1.0, num_moments=512, num_vectors=16,
-                 precision="complex64", eps=0.05, variant=-1, vector_seed=1):
+                 precision="complex64", eps=0.05, variant=-1, vector_seed=1,
+                 matrix_format="csr"):
+        """``matrix_format``: 'csr' (the systems' CSR, the general recursion
+        kernel) or 'stencil' (only the on-site blocks; the lattice-stencil
+        kernel marches each system along x, bitwise the same moments).
+        ``variant``: the CSR kernel's launch shape, or for 'stencil' the shape
+        of the stencil kernel (``_engine.MarchPlan.SHAPES``: waves per
+        workgroup and prefetch depth); -1 picks by timing."""
         import torch
+        if matrix_format not in ("csr", "stencil"):
+            raise ValueError("matrix_format must be 'csr' or 'stencil'")
         if num_moments < 2 or num_moments % 2:
             raise ValueError("num_moments must be even and >= 2")
         dev = eng.torch_device()
         self._config = dict(sizes=[int(x) for x in np.asarray(sizes).ravel()], seed=int(seed),
                             sigma=float(sigma), t=float(t), num_moments=int(num_moments),
                             num_vectors=int(num_vectors), precision=precision, eps=float(eps),
-                            vector_seed=int(vector_seed))
+                            vector_seed=int(vector_seed), matrix_format=matrix_format)
+        self.matrix_format = matrix_format
+        self.t = float(t)
         self._ran = False
         self.sizes = np.asarray(sizes, dtype=np.int64)
         if self.sizes.ndim != 1 or len(self.sizes) == 0 or self.sizes.min() < 1:
@@ -53,26 +64,33 @@ class CubicEnsemble:
             raise ValueError("ensemble too large for int32 CSR indices")
         self.n, self.nnz = n, nnz
         self.cdt, vt = eng.complex_dtype(precision)
-        self.indptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
-        self.indices = torch.empty(nnz, dtype=torch.int32, device=dev)
-        self.values = torch.empty(nnz, dtype=self.cdt, device=dev)
         radius = torch.zeros(len(self.sizes), dtype=torch.int32, device=dev)
         self.seeds = [(int(seed) << 20) + s for s in range(len(self.sizes))]
-        for s, L in enumerate(self.sizes):
-            eng.gen_cubic4(int(L), self.seeds[s], sigma, t, int(self.row_off[s]),
-                           int(self.nz_off[s]), self.indptr, self.indices, self.values,
-                           radius[s:s + 1])
+        self.mat = self.onsite = self.march = None
+        if matrix_format == "csr":
+            self.indptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+            self.indices = torch.empty(nnz, dtype=torch.int32, device=dev)
+            self.values = torch.empty(nnz, dtype=self.cdt, device=dev)
+            for s, L in enumerate(self.sizes):
+                eng.gen_cubic4(int(L), self.seeds[s], sigma, t, int(self.row_off[s]),
+                               int(self.nz_off[s]), self.indptr, self.indices, self.values,
+                               radius[s:s + 1])
+            self.mat = eng.DeviceCSR.from_device_arrays((n, n), self.indptr, self.indices,
+                                                        self.values, precision)
+        else:
+            self.onsite = torch.empty((n, 4), dtype=self.cdt, device=dev)
+            for s, L in enumerate(self.sizes):
+                eng.gen_cubic4_onsite(int(L), self.seeds[s], sigma, t, int(self.row_off[s]),
+                                      self.onsite, radius[s:s + 1])
         self.radius = radius.cpu().numpy().view(np.float32).astype(np.float64)
         self.scales = np.array([_scale_from_bounds(-r, r, eps) for r in self.radius])
-        self.mat = eng.DeviceCSR.from_device_arrays((n, n), self.indptr, self.indices,
-                                                    self.values, precision)
-        if variant < 0:
+        if matrix_format == "csr" and variant < 0:
             from . import autotune
             variant = autotune.pick_variant(self.mat, tuple(self.scales[0]), num_vectors,
                                             eng.DOTS_DOUBLING)
-        self.variant = variant
+        plan_variant = variant if matrix_format == "csr" else 0
         self.plan = eng.Plan(n, num_vectors, vt, seg_rows=self.row_off.astype(np.int32),
-                             variant=variant)
+                             variant=plan_variant)
         self.plan.set_scale(self.scales.ravel())
         self.ring = torch.empty((2, n, num_vectors), dtype=self.cdt, device=dev)
         self.vectors = torch.empty((n, num_vectors), dtype=self.cdt, device=dev)
@@ -89,17 +107,59 @@ class CubicEnsemble:
         self.dots = torch.empty((self.steps, len(self.sizes), num_vectors, 4),
                                 dtype=torch.float64, device=dev)
         self._graph = None
+        if matrix_format == "stencil":
+            if variant < 0:
+                variant = self._pick_shape(vt)
+            self.march = eng.MarchPlan(self.sizes, num_vectors, vt, shape=variant)
+        self.variant = variant
+
+    def _pick_shape(self, vt):
+        """The stencil kernel's shape (``MarchPlan.SHAPES``): the fastest over
+        a few steps of this ensemble (below 2^20 vector elements, shape 0)."""
+        import torch
+        if self.n * self.num_vectors < (1 << 20):
+            return 0
+        times = {}
+        for w in range(len(eng.MarchPlan.SHAPES)):
+            self.march = eng.MarchPlan(self.sizes, self.num_vectors, vt, shape=w)
+            self.ring[0].copy_(self.vectors)
+            self.plan.set_step(0)
+            self._steps_on(2)
+            torch.cuda.synchronize()
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            self._steps_on(6)
+            end.record()
+            torch.cuda.synchronize()
+            times[w] = start.elapsed_time(end)
+        return min(times, key=lambda w: times[w])
+
+    def _steps_on(self, count):
+        """``count`` steps from the plan's counter (dots into self.dots)."""
+        if self.march is not None:
+            self.march.steps(self.plan, self.onsite, self.t, self.ring,
+                             self.n * self.num_vectors, 2, count, eng.DOTS_DOUBLING,
+                             dots=self.dots)
+        else:
+            self.plan.steps(self.mat, self.ring, self.n * self.num_vectors, 2, count,
+                            eng.DOTS_DOUBLING, dots=self.dots)
 
     def bytes_per_step(self):
-        """Algorithmic HBM bytes of one step: a_n, a_{n-1} read, a_{n+1}
-        written, the CSR read once."""
+        """Algorithmic HBM bytes of one step with the CSR: a_n, a_{n-1} read,
+        a_{n+1} written, the CSR read once (the same for both formats: the
+        path's bytes per step)."""
         s_v = 8 if self.cdt.itemsize == 8 else 16
         return 3 * self.num_vectors * self.n * s_v + self.nnz * (s_v + 4) + 4 * (self.n + 1)
 
+    def stencil_bytes_per_step(self):
+        """The stencil kernel's own bytes per step: the three vector passes
+        and the on-site blocks (4 complex per row)."""
+        s_v = 8 if self.cdt.itemsize == 8 else 16
+        return 3 * self.num_vectors * self.n * s_v + 4 * self.n * s_v
+
     def _launch(self):
         self.plan.set_step(0)
-        self.plan.steps(self.mat, self.ring, self.n * self.num_vectors, 2, self.steps,
-                        eng.DOTS_DOUBLING, dots=self.dots)
+        self._steps_on(self.steps)
 
     def run(self, graph=True):
         """One expansion of every system; returns the seconds from the first
@@ -146,6 +206,7 @@ class CubicEnsemble:
         cfg = {k: meta[k] for k in ("sizes", "seed", "sigma", "t", "num_moments",
                                      "num_vectors", "precision", "eps", "vector_seed",
                                      "variant")}
+        cfg["matrix_format"] = meta.get("matrix_format", "csr")
         ens = cls(**cfg)
         dots = torch.from_numpy(arrays["dots"])
         if tuple(dots.shape) != tuple(ens.dots.shape):

@@ -78,3 +78,83 @@ def test_ensemble_graph_replay_equals_eager():
         assert np.array_equal(ens.dots.cpu().numpy(), eager)
         ens.run(graph=False)
         assert np.array_equal(ens.dots.cpu().numpy(), eager)
+
+
+def test_gen_cubic4_onsite_matches_csr_generator():
+    """The stencil's on-site blocks are the CSR generator's on-site entries,
+    bitwise, with the same radius."""
+    import torch
+    from kwant_amd import _engine as eng
+    sizes, seeds = [3, 1, 4], [21, 22, 23]
+    mats = [lo.cubic4(L, s, sigma=0.3, t=1.0) for L, s in zip(sizes, seeds)]
+    n = sum(4 * L ** 3 for L in sizes)
+    dev = eng.torch_device()
+    for cdt in (torch.complex64, torch.complex128):
+        onsite = torch.zeros((n, 4), dtype=cdt, device=dev)
+        radius = torch.zeros(len(sizes), dtype=torch.int32, device=dev)
+        csr_radius = torch.zeros(len(sizes), dtype=torch.int32, device=dev)
+        r0 = 0
+        for i, (L, s) in enumerate(zip(sizes, seeds)):
+            eng.gen_cubic4_onsite(L, s, 0.3, 1.0, r0, onsite, radius[i:i + 1])
+            nnz = eng.cubic4_nnz(L)
+            indptr = torch.empty(4 * L ** 3 + 1, dtype=torch.int32, device=dev)
+            indices = torch.empty(nnz, dtype=torch.int32, device=dev)
+            values = torch.empty(nnz, dtype=cdt, device=dev)
+            eng.gen_cubic4(L, s, 0.3, 1.0, 0, 0, indptr, indices, values,
+                           csr_radius[i:i + 1])
+            csr = sp.csr_matrix((values.cpu().numpy(), indices.cpu().numpy(),
+                                 indptr.cpu().numpy()), shape=(4 * L ** 3,) * 2)
+            got = onsite[r0:r0 + 4 * L ** 3].cpu().numpy()
+            for site in range(L ** 3):
+                block = csr[4 * site:4 * site + 4, 4 * site:4 * site + 4].toarray()
+                assert np.array_equal(got[4 * site:4 * site + 4], block), (L, site)
+            r0 += 4 * L ** 3
+        assert np.array_equal(radius.cpu().numpy(), csr_radius.cpu().numpy())
+
+
+@pytest.mark.parametrize("precision,width", [("complex64", 8), ("complex64", 16),
+                                             ("complex128", 4), ("complex128", 16)])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6, 7])
+def test_stencil_ensemble_bitwise_csr(precision, width, shape):
+    """The lattice-stencil steps give the CSR kernel's dots and vectors
+    bitwise, for sides 1 to 8 (every boundary case, workgroups that end inside
+    a system), eager and replayed from a graph."""
+    from kwant_amd.ensemble import CubicEnsemble
+    sizes = [1, 2, 3, 5, 8, 4, 7]
+    kw = dict(seed=5, num_moments=40, num_vectors=width, precision=precision)
+    csr = CubicEnsemble(sizes, variant=0, **kw)
+    st = CubicEnsemble(sizes, variant=shape, matrix_format="stencil", **kw)
+    assert np.array_equal(st.radius, csr.radius)
+    assert np.array_equal(st.scales, csr.scales)
+    csr.run(graph=False)
+    st.run(graph=False)
+    assert np.array_equal(st.dots.cpu().numpy(), csr.dots.cpu().numpy())
+    assert np.array_equal(st.ring.cpu().numpy(), csr.ring.cpu().numpy())
+    eager = st.dots.cpu().numpy().copy()
+    for _ in range(2):
+        st.dots.zero_()
+        st.run(graph=True)
+        assert np.array_equal(st.dots.cpu().numpy(), eager)
+
+
+def test_stencil_ensemble_moments_vs_oracle():
+    """Per-system moments of the stencil path against the oracle's
+    SpectralDensity of each system (complex128, 1e-11)."""
+    from oracle import kpm_oracle as ko
+    from kwant_amd.ensemble import CubicEnsemble
+    sizes = [3, 2, 4]
+    ens = CubicEnsemble(sizes, seed=9, num_moments=40, num_vectors=4,
+                        precision="complex128", matrix_format="stencil")
+    ens.run(graph=False)
+    got = ens.moments(damped=False)
+    vecs = ens.vectors.cpu().numpy().astype(np.complex128)
+    for s, L in enumerate(sizes):
+        m = lo.cubic4(L, ens.seeds[s])
+        r0, r1 = ens.row_off[s], ens.row_off[s + 1]
+        rad = ens.radius[s]
+        sd = ko.SpectralDensity(m, num_moments=40, bounds=(-rad, rad),
+                                vector_factory=[vecs[r0:r1, c] for c in range(4)],
+                                num_vectors=4, kernel=lambda x: x, eps=0.05)
+        ref = np.real(sd._moments())
+        ref = ref / ref[0]
+        assert np.max(np.abs(got[s] - ref)) < 1e-11, s
