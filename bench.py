@@ -680,24 +680,27 @@ def _time_ensemble(ens, graph):
     return t_eager, t_graph
 
 
-def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
+def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True,
+                   formats=("stencil", "csr"), shape=-1):
     """The batched disorder ensemble: ``count`` cubic 4-orbital systems with
     L ~ U{20..60}, generated on the GPU, ``moments`` moments with ``width``
     vectors each, every step one launch over all systems, the step sequence
     replayed from a captured HIP graph.  Both matrix formats run: the CSR
     (the general recursion kernel) and the lattice stencil (on-site blocks
     only, each system marched along x); their dots are compared bitwise, and
-    the faster is the configuration's figure."""
+    the faster is the configuration's figure.  ``formats`` narrows the run
+    to one format (for profiling); ``variant`` and ``shape`` fix the CSR and
+    stencil launch shapes (-1: timed)."""
     import numpy as np
     import torch
     from kwant_amd.ensemble import CubicEnsemble, ensemble_sizes
     sizes = ensemble_sizes(count, 20, 60, seed=0)
     res, dots = {}, {}
-    for fmt in ("stencil", "csr"):
+    for fmt in formats:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ens = CubicEnsemble(sizes, seed=0, num_moments=moments, num_vectors=width,
-                            variant=variant if fmt == "csr" else -1, matrix_format=fmt)
+                            variant=variant if fmt == "csr" else shape, matrix_format=fmt)
         torch.cuda.synchronize()
         t_setup = time.perf_counter() - t0
         t_eager, t_graph = _time_ensemble(ens, graph)
@@ -712,6 +715,9 @@ def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
             own = ens.stencil_bytes_per_step()
             res[fmt]["kernel_bytes_per_step"] = own
             res[fmt]["kernel_frac_of_hbm_peak"] = own / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+            res[fmt]["shape_waves_depth_tile_y"] = list(ens.march.SHAPES[ens.variant])
+            res[fmt]["tuning_ms_per_step"] = {str(k): round(v, 4) for k, v in
+                                              getattr(ens, "shape_ms", {}).items()}
         log_fn("ensemble (%s): %d systems, N=%d, W=%d: %.3f ms/step (graph), eager %.3f "
                "ms/step, %.0f GB/s of the path's bytes (%.1f%% of 8 TB/s); setup %.1f s"
                % (fmt, count, ens.n, width, ms_step, 1e3 * t_eager / ens.steps, achieved,
@@ -719,8 +725,10 @@ def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
         n_rows, nnz, steps = ens.n, ens.nnz, ens.steps
         del ens
         torch.cuda.empty_cache()
-    bitwise = bool(np.array_equal(dots["stencil"], dots["csr"]))
-    log_fn("ensemble: stencil dots bitwise those of the CSR path: %s" % bitwise)
+    bitwise = None
+    if len(dots) == 2:
+        bitwise = bool(np.array_equal(dots["stencil"], dots["csr"]))
+        log_fn("ensemble: stencil dots bitwise those of the CSR path: %s" % bitwise)
     best = min(res, key=lambda f: res[f]["ms_per_step"])
     out = {"workload": "%d cubic 4-orbital systems, L ~ U{20..60}, sigma 0.3, t 1, %d moments, "
                        "%d vectors each, complex64, one launch per step for all systems"
@@ -729,8 +737,8 @@ def ensemble_bench(count, moments, width, log_fn, variant=-1, graph=True):
            "algorithmic_bytes_per_step": bytes_per_step(n_rows, nnz, width, 8, 8),
            "dots_bitwise_stencil_vs_csr": bitwise}
     out.update(res[best])
-    out["csr"] = res["csr"]
-    out["stencil"] = res["stencil"]
+    for fmt in res:
+        out[fmt] = res[fmt]
     return out
 
 
@@ -917,6 +925,10 @@ def main():
                     help="two-step launch shape (default: the fastest that fits)")
     ap.add_argument("--ens-variant", type=int, default=-1,
                     help="kernel launch shape of the ensemble (default: autotuned)")
+    ap.add_argument("--ens-shape", type=int, default=-1,
+                    help="lattice-stencil launch shape of the ensemble (default: autotuned)")
+    ap.add_argument("--ens-format", choices=["both", "stencil", "csr"], default="both",
+                    help="matrix formats of the ensemble measurement")
     ap.add_argument("--ens-eager", action="store_true",
                     help="time the ensemble with eager launches only (no HIP graph)")
     ap.add_argument("--cpu-steps", type=int, default=48)
@@ -943,11 +955,13 @@ def main():
     if args.calibrate:
         calibrate()
         return
+    ens_formats = ("stencil", "csr") if args.ens_format == "both" else (args.ens_format,)
     if args.only:
         fn = {"correlator": lambda: correlator_bench(512, 1024, 32, 2, log, full=False),
               "correlator-full": lambda: correlator_bench(512, 1024, 32, 2, log),
               "ensemble": lambda: ensemble_bench(256, 512, 16, log, args.ens_variant,
-                                                 not args.ens_eager),
+                                                 not args.ens_eager, ens_formats,
+                                                 args.ens_shape),
               "surrogate": lambda: surrogate_bench(args.sur_epochs, args.sur_tol,
                                                    args.sur_points, 256, 16, log),
               "surrogate-literal": lambda: surrogate_bench(args.sur_epochs, args.sur_tol,
@@ -1113,7 +1127,8 @@ def main():
         correlator = correlator_bench(512, 1024, 32, 2, log)
     ensemble = None
     if not args.no_ensemble:
-        ensemble = ensemble_bench(256, 512, 16, log, args.ens_variant, not args.ens_eager)
+        ensemble = ensemble_bench(256, 512, 16, log, args.ens_variant, not args.ens_eager,
+                                  ens_formats, args.ens_shape)
     transform = assembly = None
     if not args.no_density:
         transform = transform_bench(1024, 65536, log)

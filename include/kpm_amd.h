@@ -448,14 +448,19 @@ int kpm_gen_cubic4_onsite(int32_t L, uint64_t seed, double sigma, double t, int6
  * those of kpm_cheb_steps on kpm_gen_cubic4's CSR.  kpm_march_create
  * builds the work list (waves: 4 or 8 per workgroup; depth: planes, 1 or 2,
  * that the streams are requested ahead, 3 or 4 the same with each plane's
- * a_n exchanged through LDS inside the workgroup); kpm_cheb_steps_march
+ * a_n exchanged through LDS inside the workgroup; tile_y, 1 to 1024: each
+ * workgroup takes waves * 16 / (lanes per row) consecutive sites of a plane
+ * in band order -- bands of tile_y rows along y, z-major inside a band --
+ * with lanes per row width / 2 for KPM_C64, width for KPM_C128);
+ * kpm_cheb_steps_march
  * runs n_steps steps from the plan's counter (the plan: one segment per
  * system, this row count, width and value type; scale, and for
  * KPM_DOTS_DOUBLING the dots grid, set).  Dots as kpm_cheb_steps; modes
  * KPM_DOTS_DOUBLING and KPM_DOTS_NONE. */
 typedef struct kpm_march kpm_march;
 int kpm_march_create(int32_t n_systems, const int32_t* h_sides, int32_t width,
-                     int32_t value_type, int32_t waves, int32_t depth, kpm_march** out);
+                     int32_t value_type, int32_t waves, int32_t depth, int32_t tile_y,
+                     kpm_march** out);
 int kpm_march_destroy(kpm_march* m);
 int kpm_cheb_steps_march(kpm_plan* plan, kpm_march* m, const void* onsite, double t,
                          void* ring, int64_t slot_stride, int32_t slots, double* d_dots,

@@ -171,7 +171,8 @@ def load_library(path=LIB_PATH):
                                           ctypes.c_double, i64, i64, vp, vp, vp, i32, vp, vp]),
         "kpm_gen_cubic4_onsite": (ctypes.c_int, [i32, ctypes.c_uint64, ctypes.c_double,
                                                  ctypes.c_double, i64, vp, i32, vp, vp]),
-        "kpm_march_create": (ctypes.c_int, [i32, pi32, i32, i32, i32, i32, ctypes.POINTER(vp)]),
+        "kpm_march_create": (ctypes.c_int, [i32, pi32, i32, i32, i32, i32, i32,
+                                            ctypes.POINTER(vp)]),
         "kpm_march_destroy": (ctypes.c_int, [vp]),
         "kpm_cheb_steps_march": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, vp, i64, i32, vp,
                                                 i32, i32, i32, i32, vp]),
@@ -901,21 +902,31 @@ class MarchPlan:
     """Work list of the lattice-stencil steps for cubic 4-orbital systems of
     the given sides (kpm_march_create)."""
 
-    # (waves per workgroup, depth) of each shape: depth 1 or 2 planes of
-    # prefetch, 3 or 4 the same with the plane exchanged through LDS
-    SHAPES = ((4, 1), (8, 1), (4, 2), (8, 2), (4, 3), (8, 3), (4, 4), (8, 4))
+    # (waves per workgroup, depth, tile_y) of each shape: depth 1 or 2 planes
+    # of prefetch, 3 or 4 the same with the plane exchanged through LDS;
+    # tile_y rows along y of the bands whose sites (z-major) workgroups take
+    SHAPES = ((4, 1, 1), (8, 1, 1), (4, 2, 1), (8, 2, 1), (4, 3, 1), (8, 3, 1), (4, 4, 1),
+              (8, 4, 1), (4, 3, 2), (8, 3, 2), (4, 3, 4), (8, 3, 4), (4, 1, 2), (4, 1, 4),
+              (8, 3, 8), (4, 4, 2))
+
+    @staticmethod
+    def shape_ok(width, value_type, shape):
+        """Whether the library has a stencil kernel for this width (at least
+        4 lanes per row; every shape takes every such width)."""
+        lpr = width // 2 if value_type == KPM_C64 else width
+        return 0 <= int(shape) < len(MarchPlan.SHAPES) and 4 <= lpr <= 64
 
     def __init__(self, sides, width, value_type, shape=0):
         lib = load_library()
         sd = np.ascontiguousarray(sides, dtype=np.int32)
-        waves, depth = self.SHAPES[int(shape)]
+        waves, depth, tile_y = self.SHAPES[int(shape)]
         handle = ctypes.c_void_p()
         check(lib.kpm_march_create(len(sd), sd.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                   int(width), int(value_type), waves, depth,
+                                   int(width), int(value_type), waves, depth, tile_y,
                                    ctypes.byref(handle)), "kpm_march_create")
         self.handle = handle
         self.width, self.value_type = int(width), int(value_type)
-        self.shape, self.waves, self.depth = int(shape), waves, depth
+        self.shape, self.waves, self.depth, self.tile_y = int(shape), waves, depth, tile_y
 
     def __del__(self):
         h = getattr(self, "handle", None)

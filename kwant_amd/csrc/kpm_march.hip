@@ -31,7 +31,7 @@ struct MarchParams {
   char* ring;
   int64_t slot_bytes;
   int32_t slots;
-  const int4* jobs;          // logical block -> (row0, L, first column, columns)
+  const int4* jobs;          // logical block -> (row0, L, first site, sites) in band order
   const int32_t* blk_seg;    // logical block -> system (segment)
   const int32_t* seg_blk;    // system -> first logical block (nseg + 1)
   const double* scale;       // system -> (a, b)
@@ -44,13 +44,17 @@ struct MarchParams {
   int32_t nblocks;
   int32_t nblocks_launch;
   int32_t nseg;
+  int32_t tile_y;            // rows along y of a band
 };
 
 // PD: planes the streams of a_n (beyond the x + 1 plane), a_{n-1} and the
 // on-site values are requested ahead (1 or 2).  LX: the workgroup's columns
 // put this plane's a_n in LDS (double-buffered, one barrier per plane), and
-// the site's other orbitals and the z neighbours inside the workgroup are
-// read from there instead of through the vector memory path.
+// the site's other orbitals and the y and z neighbours inside the workgroup's
+// tile of sites are read from there instead of through the vector memory path.
+// A workgroup's columns are the 4 orbitals of consecutive sites of one
+// system's plane in band order (bands of tile_y rows, z-major inside a band),
+// so a tile holds up to tile_y rows of y neighbours.
 template <typename T, int VEC, int LPR, int WAVES, int MODE, int PD, bool LX>
 __global__ __launch_bounds__(WAVES * 64) void march_kernel(MarchParams p, int n) {
   constexpr int W = LPR * VEC;
@@ -80,16 +84,28 @@ __global__ __launch_bounds__(WAVES * 64) void march_kernel(MarchParams p, int n)
       reduce_sums<MODE>(p.prev_part, p.nblocks, p.seg_blk, p.nseg, W, p.im_exp, p.prev_dots,
                         lb * WAVES + wave, p.nblocks * WAVES);
   }
+  // the workgroup's tile: `cnt` consecutive sites from site K0 of a plane in
+  // band order -- bands of tile_y rows along y (the last one may have fewer),
+  // each band's sites z-major: K = band tile_y L + z rows + ty; slot = 4 (K -
+  // K0) + o.  Lanes past the last site of a system take site K0 (every address
+  // they form is valid) and store nothing.
   const int4 job = p.jobs[lb];
   const int seg = p.blk_seg[lb];
   const int64_t row0 = job.x;
   const int L = job.y;
-  const int c_raw = job.z + wave * RPW + grp;
-  const bool live = c_raw < job.z + job.w;
-  const int c = live ? c_raw : job.z;
+  const int K0 = job.z, cnt = job.w;
+  const int slot = wave * RPW + grp;
+  const int o = slot & 3, kk = slot >> 2;
+  const bool live = kk < cnt;
+  const int TY = p.tile_y;
+  const int K = K0 + (live ? kk : 0);
+  const int band = K / (TY * L);
+  const int kb = K - band * TY * L;
+  const int rb = min(TY, L - band * TY);  // rows of this band
+  const int z = kb / rb, ty = kb - z * rb;
+  const int y = band * TY + ty;
+  const int c = 4 * (y * L + z) + o;
   const int64_t P = 4 * int64_t(L) * L;  // rows from one x plane to the next
-  const int q = c >> 2, o = c & 3;
-  const int y = q / L, z = q - y * L;
   const double sa = p.scale[2 * seg], sb = p.scale[2 * seg + 1];
   const T mul = T((n == 0 ? 1.0 : 2.0) / sa);
   const T bsh = T(sb);
@@ -131,6 +147,11 @@ __global__ __launch_bounds__(WAVES * 64) void march_kernel(MarchParams p, int n)
 
   __shared__ __attribute__((aligned(16))) T lds_x[LX ? 2 * WAVES * RPW * W * 2 : 2];
   const int64_t r0 = row0 + c;
+  // with the LDS exchange, which in-plane neighbours are in the tile (the
+  // same at every x): d sites away in band order
+  auto in_tile = [&](int d) { return LX && unsigned(kk + d) < unsigned(cnt); };
+  const bool zm_lds = z > 0 && in_tile(-rb), zp_lds = z < L - 1 && in_tile(rb);
+  const bool ym_lds = ty > 0 && in_tile(-1), yp_lds = ty + 1 < rb && in_tile(1);
   // registers of the march: a_n of planes x-1 .. x+1+PD, a_{n-1} and the
   // on-site values of planes x .. x+PD-1 (slot k holds plane x + k)
   CL am, an[PD + 2], xq[PD];
@@ -162,40 +183,33 @@ __global__ __launch_bounds__(WAVES * 64) void march_kernel(MarchParams p, int n)
     }
     const CL& acur = an[0];
     const CL& ap = an[1];
-    // this plane's neighbours
+    // this plane's neighbours: those read from memory are requested first,
+    // so that their latency overlaps the LDS exchange
     CL gym, gzm, gzp, gyp, site[4];
-    if (y > 0) gym = IO::load(cur + off(r - 4 * L)); else zero(gym);
-    if (y < L - 1) gyp = IO::load(cur + off(r + 4 * L)); else zero(gyp);
+    if (y > 0 && !ym_lds) gym = IO::load(cur + off(r - 4 * L)); else zero(gym);
+    if (y < L - 1 && !yp_lds) gyp = IO::load(cur + off(r + 4 * L)); else zero(gyp);
+    if (z > 0 && !zm_lds) gzm = IO::load(cur + off(r - 4)); else zero(gzm);
+    if (z < L - 1 && !zp_lds) gzp = IO::load(cur + off(r + 4)); else zero(gzp);
     if constexpr (LX) {
-      // slot = the column's place in the workgroup; a site's 4 columns share
-      // a workgroup (its first column is a multiple of 4 x rows per wave)
+      // every slot puts its a_n in LDS; the site's other orbitals and the
+      // neighbours inside the tile come from there
       T* buf = lds_x + (x & 1) * (WAVES * RPW * W * 2);
-      const int slot = wave * RPW + grp;
       IO::store(buf + 2 * (slot * W + col), acur);
       __syncthreads();
+      auto from = [&](int s) { return IO::load(buf + 2 * (s * W + col)); };
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        if (j != o) site[j] = IO::load(buf + 2 * ((slot - o + j) * W + col)); else site[j] = acur;
+        if (j != o) site[j] = from(slot - o + j); else site[j] = acur;
       }
-      if (z > 0) {
-        if (slot >= 4) gzm = IO::load(buf + 2 * ((slot - 4) * W + col));
-        else gzm = IO::load(cur + off(r - 4));
-      } else {
-        zero(gzm);
-      }
-      if (z < L - 1) {
-        if (slot + 4 < job.w) gzp = IO::load(buf + 2 * ((slot + 4) * W + col));
-        else gzp = IO::load(cur + off(r + 4));
-      } else {
-        zero(gzp);
-      }
+      if (zm_lds) gzm = from(slot - 4 * rb);
+      if (zp_lds) gzp = from(slot + 4 * rb);
+      if (ym_lds) gym = from(slot - 4);
+      if (yp_lds) gyp = from(slot + 4);
     } else {
-      if (z > 0) gzm = IO::load(cur + off(r - 4)); else zero(gzm);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         if (j != o) site[j] = IO::load(cur + off(r - o + j)); else site[j] = acur;
       }
-      if (z < L - 1) gzp = IO::load(cur + off(r + 4)); else zero(gzp);
     }
 
     T er[4], ei[4];
@@ -316,7 +330,7 @@ int lanes_per_row(int value_type, int width) {
 using namespace kpm;
 
 struct kpm_march {
-  int32_t n_rows = 0, width = 0, value_type = 0, nseg = 0, waves = 0, depth = 1;
+  int32_t n_rows = 0, width = 0, value_type = 0, nseg = 0, waves = 0, depth = 1, tile_y = 1;
   int32_t nblocks = 0, nblocks_launch = 0;
   int4* jobs = nullptr;
   int32_t* blk_seg = nullptr;
@@ -357,7 +371,8 @@ int march_upload(T* d, const std::vector<T>& h, const char* what) {
 extern "C" {
 
 int kpm_march_create(int32_t n_systems, const int32_t* h_sides, int32_t width,
-                     int32_t value_type, int32_t waves, int32_t depth, kpm_march** out) {
+                     int32_t value_type, int32_t waves, int32_t depth, int32_t tile_y,
+                     kpm_march** out) {
   if (!out) return fail(KPM_ERR_ARG, "out is NULL");
   *out = nullptr;
   if (n_systems <= 0 || !h_sides) return fail(KPM_ERR_ARG, "need at least one system");
@@ -369,20 +384,21 @@ int kpm_march_create(int32_t n_systems, const int32_t* h_sides, int32_t width,
   if (lpr < 4 || lpr > 64 || (width & (width - 1)) != 0 ||
       (value_type == KPM_C64 && width > 64) || (value_type == KPM_C128 && width > 64))
     return fail(KPM_ERR_UNSUPPORTED, "no lattice-stencil kernel for this width and value type");
-  const int G = waves * 64 / lpr;  // columns per workgroup
+  const int S = waves * 64 / lpr / 4;  // sites per workgroup
+  if (tile_y < 1 || tile_y > 1024) return fail(KPM_ERR_ARG, "tile_y must be 1 to 1024");
   std::vector<int4> jobs;
   std::vector<int32_t> blk_seg, seg_blk;
   int64_t row0 = 0;
   for (int s = 0; s < n_systems; ++s) {
     const int64_t L = h_sides[s];
     if (L < 1 || L > 1290) return fail(KPM_ERR_ARG, "system side out of range");
-    const int64_t cols = 4 * L * L;
     if (row0 + 4 * L * L * L >= (int64_t(1) << 31))
       return fail(KPM_ERR_ARG, "ensemble too large for int32 rows");
     seg_blk.push_back(int32_t(jobs.size()));
-    for (int64_t c0 = 0; c0 < cols; c0 += G) {
-      const int64_t cnt = cols - c0 < G ? cols - c0 : G;
-      jobs.push_back(make_int4(int(row0), int(L), int(c0), int(cnt)));
+    // S consecutive sites of the plane in band order per workgroup
+    for (int64_t k0 = 0; k0 < L * L; k0 += S) {
+      const int64_t cnt = L * L - k0 < S ? L * L - k0 : S;
+      jobs.push_back(make_int4(int(row0), int(L), int(k0), int(cnt)));
       blk_seg.push_back(s);
     }
     row0 += 4 * L * L * L;
@@ -395,6 +411,7 @@ int kpm_march_create(int32_t n_systems, const int32_t* h_sides, int32_t width,
   m->nseg = n_systems;
   m->waves = waves;
   m->depth = depth;
+  m->tile_y = tile_y;
   m->nblocks = int32_t(jobs.size());
   m->nblocks_launch = (m->nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;
   const int maxnc = 4;  // doubling: 2 TwoSum accumulators + 2 grid sums
@@ -460,6 +477,7 @@ int kpm_cheb_steps_march(kpm_plan* plan, kpm_march* m, const void* onsite, doubl
   p.nblocks = m->nblocks;
   p.nblocks_launch = m->nblocks_launch;
   p.nseg = m->nseg;
+  p.tile_y = m->tile_y;
   const int64_t rec = int64_t(m->nseg) * m->width * 4;
   hipStream_t s = (hipStream_t)stream;
   for (int i = 0; i < n_steps; ++i) {

@@ -37,7 +37,7 @@ class CubicEnsemble:
         kernel marches each system along x, bitwise the same moments).
         ``variant``: the CSR kernel's launch shape, or for 'stencil' the shape
         of the stencil kernel (``_engine.MarchPlan.SHAPES``: waves per
-        workgroup and prefetch depth); -1 picks by timing."""
+        workgroup, prefetch depth, tile of sites); -1 picks by timing."""
         import torch
         if matrix_format not in ("csr", "stencil"):
             raise ValueError("matrix_format must be 'csr' or 'stencil'")
@@ -115,24 +115,35 @@ class CubicEnsemble:
 
     def _pick_shape(self, vt):
         """The stencil kernel's shape (``MarchPlan.SHAPES``): the fastest over
-        a few steps of this ensemble (below 2^20 vector elements, shape 0)."""
-        import torch
+        a few steps of this ensemble, the three fastest timed again three
+        times longer (below 2^20 vector elements, shape 0)."""
         if self.n * self.num_vectors < (1 << 20):
             return 0
-        times = {}
-        for w in range(len(eng.MarchPlan.SHAPES)):
-            self.march = eng.MarchPlan(self.sizes, self.num_vectors, vt, shape=w)
+        shapes = [w for w in range(len(eng.MarchPlan.SHAPES))
+                  if eng.MarchPlan.shape_ok(self.num_vectors, vt, w)]
+        times = {w: self._time_shape(vt, w, 6) for w in shapes}
+        final = {w: self._time_shape(vt, w, 18)
+                 for w in sorted(times, key=lambda w: times[w])[:3]}
+        times.update(final)
+        self.shape_ms = times
+        return min(final, key=lambda w: final[w])
+
+    def _time_shape(self, vt, shape, count):
+        """Milliseconds per step of ``count`` steps (at most the expansion's)
+        with the stencil kernel's ``shape``, after two untimed steps."""
+        import torch
+        self.march = eng.MarchPlan(self.sizes, self.num_vectors, vt, shape=shape)
+        count = min(count, self.steps)
+        for run in range(2):
             self.ring[0].copy_(self.vectors)
             self.plan.set_step(0)
-            self._steps_on(2)
             torch.cuda.synchronize()
             start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             start.record()
-            self._steps_on(6)
+            self._steps_on(min(2, count) if run == 0 else count)
             end.record()
             torch.cuda.synchronize()
-            times[w] = start.elapsed_time(end)
-        return min(times, key=lambda w: times[w])
+        return start.elapsed_time(end) / count
 
     def _steps_on(self, count):
         """``count`` steps from the plan's counter (dots into self.dots)."""

@@ -124,3 +124,32 @@ def lib_count(lib, width, value_type):
     count = ctypes.c_int32()
     lib.kpm_tiled_variant_count(width, value_type, ctypes.byref(count))
     return count.value
+
+
+def test_march_argument_errors_without_gpu():
+    """The lattice-stencil work list validates its arguments before any device
+    work; MarchPlan.shape_ok accepts the widths the library has kernels for."""
+    import torch  # noqa: F401
+    from kwant_amd._engine import MarchPlan
+    lib = ctypes.CDLL(LIB)
+    lib.kpm_last_error.restype = ctypes.c_char_p
+    out = ctypes.c_void_p()
+    sides = (ctypes.c_int32 * 2)(3, 5)
+    create = lib.kpm_march_create
+    assert create(2, sides, 16, 1, 5, 1, 1, ctypes.byref(out)) == 1          # waves
+    assert create(2, sides, 16, 1, 4, 0, 1, ctypes.byref(out)) == 1          # depth
+    assert create(0, sides, 16, 1, 4, 1, 1, ctypes.byref(out)) == 1          # no systems
+    assert create(2, sides, 4, 1, 4, 1, 1, ctypes.byref(out)) == 3           # 2 lanes per row
+    bad = (ctypes.c_int32 * 1)(0)
+    assert create(1, bad, 16, 1, 4, 1, 1, ctypes.byref(out)) == 1            # side 0
+    for width, vt in ((8, 1), (16, 1), (64, 1), (4, 2), (16, 2), (64, 2)):
+        for shape in range(len(MarchPlan.SHAPES)):
+            assert MarchPlan.shape_ok(width, vt, shape)
+        assert not MarchPlan.shape_ok(width, vt, len(MarchPlan.SHAPES))
+        for tile_y in (0, 1025):
+            rc = create(2, sides, width, vt, 8, 1, tile_y, ctypes.byref(out))
+            assert rc == 1 and b"tile_y" in lib.kpm_last_error(), (width, vt, tile_y)
+    assert not MarchPlan.shape_ok(4, 1, 0) and not MarchPlan.shape_ok(256, 2, 0)
+    rc = lib.kpm_cheb_steps_march(None, None, None, ctypes.c_double(1.0), None, 0, 4, None, 0,
+                                  0, 1, 1, None)
+    assert rc == 1

@@ -114,13 +114,18 @@ def test_gen_cubic4_onsite_matches_csr_generator():
 
 @pytest.mark.parametrize("precision,width", [("complex64", 8), ("complex64", 16),
                                              ("complex128", 4), ("complex128", 16)])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("shape", range(16))
 def test_stencil_ensemble_bitwise_csr(precision, width, shape):
     """The lattice-stencil steps give the CSR kernel's dots and vectors
-    bitwise, for sides 1 to 8 (every boundary case, workgroups that end inside
-    a system), eager and replayed from a graph."""
+    bitwise, for sides 1 to 11 (every boundary case; tiles that span bands of
+    y rows and end inside a system), for every launch shape, eager and
+    replayed from a graph."""
+    from kwant_amd import _engine as eng
     from kwant_amd.ensemble import CubicEnsemble
-    sizes = [1, 2, 3, 5, 8, 4, 7]
+    assert len(eng.MarchPlan.SHAPES) == 16
+    _, vt = eng.complex_dtype(precision)
+    assert eng.MarchPlan.shape_ok(width, vt, shape)
+    sizes = [1, 2, 3, 5, 8, 4, 11, 7]
     kw = dict(seed=5, num_moments=40, num_vectors=width, precision=precision)
     csr = CubicEnsemble(sizes, variant=0, **kw)
     st = CubicEnsemble(sizes, variant=shape, matrix_format="stencil", **kw)
