@@ -537,12 +537,19 @@ def _velocity(hamiltonian, params, op_type, positions):
             op, norbs, _ = hamiltonian.hamiltonian_submatrix(
                 params=params, sparse=True, return_norb=True)
             positions = np.vstack([[hamiltonian.pos(i)] * nb for i, nb in enumerate(norbs)])
-            op = coo_matrix(op)
         else:
-            op = coo_matrix(hamiltonian, copy=True)
-        disp = positions[op.col] - positions[op.row]
-        if direction > disp.shape[1]:
+            op = hamiltonian
+        positions = np.asarray(positions)
+        if direction > positions.shape[1]:
             raise ValueError("{} is not an allowed direction.".format(op_type))
+        if isinstance(op, csr_matrix) and op.has_canonical_format:
+            # the same entries as the COO route below, without its sort
+            x = positions[:, direction]
+            rows = np.repeat(np.arange(op.shape[0]), np.diff(op.indptr))
+            data = op.data * (1j * (x[op.indices] - x[rows]))
+            return csr_matrix((data, op.indices.copy(), op.indptr.copy()), shape=op.shape)
+        op = coo_matrix(op, copy=True)
+        disp = positions[op.col] - positions[op.row]
         op.data = op.data * (1j * disp[:, direction])
         return op.tocsr()
     try:

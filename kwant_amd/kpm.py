@@ -146,13 +146,13 @@ def _system_orbitals(syst, where):
             ids = getattr(syst, 'id_by_site', None)
             sites = [ids[s] if ids is not None and not isinstance(s, numbers.Integral)
                      else int(s) for s in where]
-    orbs = []
-    for site in sites:
-        k = np.searchsorted(ranges[:, 0], site, side='right') - 1
-        first, norbs, offset = ranges[k]
-        start = int(offset + (site - first) * norbs)
-        orbs.extend(range(start, start + int(norbs)))
-    return tot, orbs
+    # each site's orbitals in site order, the sites in the given order
+    sites = np.asarray(sites, dtype=np.int64).reshape(-1)
+    k = np.searchsorted(ranges[:, 0], sites, side='right') - 1
+    first, norbs, offset = ranges[k, 0], ranges[k, 1], ranges[k, 2]
+    start = offset + (sites - first) * norbs
+    within = np.arange(int(norbs.sum()), dtype=np.int64) - np.repeat(np.cumsum(norbs) - norbs, norbs)
+    return tot, np.repeat(start, norbs) + within
 
 
 def RandomVectors(syst, where=None, rng=None):
@@ -161,9 +161,16 @@ def RandomVectors(syst, where=None, rng=None):
     per vector (kwant/kpm.py:932-951)."""
     rng = ensure_rng(rng)
     tot, orbs = _orbitals_of(syst, where)
+    # every orbital in order: one slice assignment instead of a gather
+    if len(orbs) == tot and np.array_equal(orbs, np.arange(tot)):
+        orbs = slice(None)
+        count = tot
+    else:
+        orbs = np.asarray(orbs, dtype=np.intp)
+        count = len(orbs)
     while True:
         vec = np.zeros(tot, dtype=complex)
-        vec[orbs] = np.exp(2j * np.pi * rng.random_sample(len(orbs)))
+        vec[orbs] = np.exp(2j * np.pi * rng.random_sample(count))
         yield vec
 
 
