@@ -291,6 +291,27 @@ class DeviceCSR:
                            self.values.data_ptr(), self.value_type,
                            row_nnz_hint(np.diff(csr.indptr)))
 
+    def row_block(self, r0, r1):
+        """Rows r0 .. r1-1 of this device matrix as an (r1 - r0) x n_cols
+        DeviceCSR that shares its column indices and values (an SpMM with it
+        writes only those rows, wherever its output points)."""
+        if not 0 <= r0 < r1 <= self.shape[0]:
+            raise ValueError("row block outside the matrix")
+        k0, k1 = int(self.indptr[r0].item()), int(self.indptr[r1].item())
+        sub = DeviceCSR.__new__(DeviceCSR)
+        sub.order = sub.inverse = None
+        sub.shape = (r1 - r0, self.shape[1])
+        sub.nnz = k1 - k0
+        sub.precision, sub.value_type = self.precision, self.value_type
+        sub.indptr = (self.indptr[r0:r1 + 1] - k0).contiguous()
+        sub.indices = self.indices[k0:max(k1, k0 + 1)] if self.indices.numel() else self.indices
+        sub.values = self.values[k0:max(k1, k0 + 1)]
+        sub._parent = self  # the shared arrays stay alive
+        sub.desc = KpmCsr(r1 - r0, int(self.shape[1]), sub.nnz, sub.indptr.data_ptr(),
+                          sub.indices.data_ptr(), sub.values.data_ptr(), self.value_type,
+                          self.desc.row_nnz_hint)
+        return sub
+
     def rows_in(self, host):
         """Caller-order rows (axis 0) of a host array, in device order."""
         return host if self.order is None else np.asarray(host)[self.order]

@@ -386,9 +386,18 @@ class Correlator:
         # each run of Psi rows passes through a buffer of OMEGA_CHUNK rows.
         pad = self.SLOT_PAD
         prepared = isinstance(gp, tuple) and gp[0] == 'int8'
+        # the FP64 / FP32 GEMMs read Psi and Omega in row panels (see _panels)
+        panelled = not isinstance(gp, tuple)
         psi_slots = min(m_count, self.OMEGA_CHUNK) if prepared else m_count
-        psi_store = torch.empty(psi_slots * (n * max(widths, default=1) + pad), dtype=cdt,
-                                device=dev)
+        psi_len = max([n * w + pad for w in widths] + [1])
+        if panelled:
+            psi_len = max([len(self._panels(n, w, cdt)) * (self._panels(n, w, cdt)[0][1] * w + pad)
+                           for w in widths] + [psi_len])
+        psi_store = torch.empty(psi_slots * psi_len, dtype=cdt, device=dev)
+        a_pan = None
+        if panelled and widths:
+            a_pan = torch.empty(min(m_count, self.OMEGA_CHUNK) * max(
+                self._panels(n, w, cdt)[0][1] * w + pad for w in widths), dtype=cdt, device=dev)
         start = r0
         host_next = draw(start, widths[0]) if widths else None
         for bi, width in enumerate(widths):
@@ -397,8 +406,14 @@ class Correlator:
             block = torch.from_numpy(host).to(dev).to(cdt)
             # Psi_n = op2 T_n(H~)|r>, resident for the whole block
             k = n * width
-            psi_rows = psi_store[:psi_slots * (k + pad)].view(psi_slots, k + pad)[:, :k]
-            psi = psi_rows.view(psi_slots, n, width)
+            if panelled:
+                ranges = self._panels(n, width, cdt)
+                kp = ranges[0][1] * width + pad
+                psi_pan = psi_store[:len(ranges) * psi_slots * kp].view(len(ranges), psi_slots, kp)
+                psi = psi_rows = None
+            else:
+                psi_rows = psi_store[:psi_slots * (k + pad)].view(psi_slots, k + pad)[:, :k]
+                psi = psi_rows.view(psi_slots, n, width)
             if prepared:
                 prep = getattr(self, '_psi_prep', None)
                 if prep is None or prep.k != k:
@@ -406,7 +421,10 @@ class Correlator:
                     prep = self._psi_prep = eng.IsliceOperand(m_count, k, gp[1])
 
             def to_psi(m0, m1, slots):
-                if prepared:
+                if panelled:
+                    with self._timed('operator_s'):
+                        self._apply_panels('op2', self._op2, slots, psi_pan, m0, ranges, width)
+                elif prepared:
                     with self._timed('operator_s'):
                         self._apply('op2', self._op2, slots, psi[:m1 - m0], width, m1 - m0)
                     with self._timed('gemm_s'):
@@ -428,8 +446,19 @@ class Correlator:
             psi_k = psi_rows
 
             def gemm(m0, m1, slots):
+                cnt = m1 - m0
+                if panelled:
+                    # each row panel of the ring slots is copied into a
+                    # buffer of short rows and multiplied by Psi's same panel
+                    with self._timed('gemm_s'):
+                        for p, (r0, r1) in enumerate(ranges):
+                            kk = (r1 - r0) * width
+                            a = a_pan[:cnt * kp].view(cnt, kp)[:, :kk]
+                            a.copy_(slots[:, r0:r1, :].reshape(cnt, kk))
+                            eng.corr_gemm(a, psi_pan[p, :m_count, :kk], mu[m0:m1], gp)
+                    return
                 # the ring slots as GEMM rows: k elements each, padded pitch
-                a = slots.as_strided((m1 - m0, k), (slots.stride(0), 1))
+                a = slots.as_strided((cnt, k), (slots.stride(0), 1))
                 with self._timed('gemm_s'):
                     if prepared:
                         prep.gemm(a, mu[m0:m1])
@@ -453,11 +482,54 @@ class Correlator:
             self.moments_matrix = np.array(self._mu_list)
         self._collect_timings()
 
+    # bytes of a GEMM row panel: the FP64 / FP32 GEMMs slow down ~2.3x when
+    # their operand rows are 128 MB apart (8 complex128 vectors of 2^20
+    # sites) rather than 32 MB, so Psi and Omega reach them in row panels
+    PANEL_BYTES = 1 << 25
+
+    def _panels(self, n, width, cdt):
+        """Row ranges [(r0, r1), ...] that cut an (n, width) block into panels
+        of at most about PANEL_BYTES, all but the last of equal length."""
+        import torch
+        size = n * width * torch.empty((), dtype=cdt).element_size()
+        count = max(1, -(-size // self.PANEL_BYTES))
+        rows = -(-n // count)
+        return [(r0, min(n, r0 + rows)) for r0 in range(0, n, rows)]
+
+    def _apply_panels(self, key, op, src, dst, m0, ranges, width):
+        """dst[p, m0 + slot, :(r1 - r0) width] = rows r0 .. r1-1 of op src[slot]
+        for each panel p = (r0, r1): the operator's row blocks on the GPU
+        (matrix), a copy (identity), or the host path a slot at a time."""
+        import torch
+        kind, payload = op
+        cnt = src.shape[0]
+        if kind == 'matrix':
+            full = self._device_op(key, kind, payload)
+            blocks = self.__dict__.setdefault('_row_blocks', {})
+            for p, (r0, r1) in enumerate(ranges):
+                sub = blocks.get((key, r0, r1))
+                if sub is None:
+                    sub = blocks[(key, r0, r1)] = full.row_block(r0, r1)
+                sub.spmm(src, dst[p, m0:m0 + cnt], width, n_slots=cnt, x_stride=src.stride(0),
+                         y_stride=dst.stride(1))
+            return
+        tmp = src if kind == 'id' else torch.empty_like(src[:1])
+        for s in range(cnt if kind != 'id' else 1):
+            if kind != 'id':
+                self._apply(key, op, src[s:s + 1], tmp, width, 1)
+            for p, (r0, r1) in enumerate(ranges):
+                kk = (r1 - r0) * width
+                rows = tmp[:, r0:r1, :].reshape(tmp.shape[0], kk)
+                if kind == 'id':
+                    dst[p, m0:m0 + cnt, :kk].copy_(rows)
+                else:
+                    dst[p, m0 + s, :kk].copy_(rows[0])
+
     def _width_cap(self, n, m_count, cdt):
         """Largest power-of-two block width (<= max_block_width) whose live
         vectors -- Psi (m_count slots) and one streaming ring of OMEGA_CHUNK
         slots, plus two single blocks -- take at most 3/4 of the free device
-        memory."""
+        memory less the GEMMs' panel buffer (OMEGA_CHUNK row panels)."""
         import torch
         free, _ = torch.cuda.mem_get_info()
         # memory that torch's allocator holds but does not use is free for us
@@ -465,7 +537,7 @@ class Correlator:
         slots = m_count + min(m_count, self.OMEGA_CHUNK) + 2
         per_width = slots * n * torch.empty((), dtype=cdt).element_size()
         width = self._max_width
-        budget = free * 3 // 4
+        budget = free * 3 // 4 - min(m_count, self.OMEGA_CHUNK) * (self.PANEL_BYTES + 65536)
         while width > 1 and width * per_width > budget:
             width //= 2
         if per_width > budget:

@@ -77,6 +77,31 @@ def test_correlator_streamed_rings_match_oracle(monkeypatch, gemm):
     assert set(gpu.timings) == {"expansion_s", "operator_s", "gemm_s"}
 
 
+@pytest.mark.parametrize("ops", ["matrices", "psi_identity", "psi_callable"])
+def test_correlator_row_panels_match_oracle(monkeypatch, ops):
+    """The FP64 GEMM reads Psi and Omega in row panels: with panels of a few
+    rows (ragged last panel, a different cut per block width) and short rings,
+    the moment matrix is the oracle's, for operators on the GPU (matrices), a
+    copy (identity) and the host path (callable)."""
+    from kwant_amd import correlator
+    monkeypatch.setattr(correlator.Correlator, "OMEGA_CHUNK", 7)
+    monkeypatch.setattr(correlator.Correlator, "PANEL_BYTES", 300)
+    h = brute.gaussian_unitary(18, 13)
+    m1, m2 = brute.gaussian_unitary(18, 14), brute.gaussian_unitary(18, 15)
+    op1, op2 = {"matrices": (m1, m2), "psi_identity": (lambda v: m1 @ v, None),
+                "psi_callable": (None, lambda v: m2 @ v)}[ops]
+    r = float(np.max(np.abs(np.linalg.eigvalsh(h)))) * 1.05
+    kw = dict(num_moments=40, num_vectors=6, bounds=(-r, r), rng=5)
+    gpu = kpm().Correlator(h, operator1=op1, operator2=op2, max_block_width=4, **kw)
+    # 18 rows: 4 panels of <= 5 rows at width 4, 2 of 9 at width 2
+    assert gpu.block_widths == [4, 2]
+    cdt = __import__("torch").complex128
+    assert len(gpu._panels(18, 4, cdt)) == 4 and len(gpu._panels(18, 2, cdt)) == 2
+    ora = ko.Correlator(h, operator1=op1, operator2=op2, **kw)
+    scale = np.abs(ora.moments_matrix).max()
+    assert rel(gpu.moments_matrix, ora.moments_matrix, scale) < 1e-10
+
+
 def test_correlator_mean_false_matches_oracle():
     h = brute.gaussian_unitary(12, 6)
     r = float(np.max(np.abs(np.linalg.eigvalsh(h)))) * 1.05
