@@ -27,7 +27,10 @@ Hamiltonian as 4 x 4 site blocks -- for 4-orbital systems, operator=None; the
 moments are bitwise the CSR ones), ``tiles`` (a kwant_amd.ordering.Tiling, or
 True to tile a kwant system by its site positions: the device row order groups
 the rows into compact tiles and the plain DOS path runs two Chebyshev steps per
-launch; the moments are bitwise those without tiles), ``gemm_precision`` for the Correlator
+launch; the moments are bitwise those without tiles.  The default 'auto' tiles
+kwant systems with positions and at least 2^18 orbitals on the plain DOS path
+with the default row order, and each block keeps whichever of the two-step and
+one-step launches its timing picks), ``gemm_precision`` for the Correlator
 ('fp64' | 'fp32' | 'isliced4' | 'isliced5' | 'sliced3' | 'sliced4': the 'isliced'
 modes are the split-precision GEMM on the INT8 matrix cores, 'sliced' on BF16).
 """
@@ -92,18 +95,34 @@ def _as_csr(hamiltonian, params=None):
         raise ValueError("'hamiltonian' is neither a matrix nor a Kwant system.")
 
 
-def _tiling_of(hamiltonian, tiles, tile_rows=256):
-    """The Tiling for ``tiles``: None/False (no tiles), a Tiling, or True
-    (a kwant system tiled by its site positions, each site's orbitals kept
-    together)."""
+# 'auto' tiles kwant systems with site positions from this many orbitals on
+TILES_AUTO_ROWS = 1 << 18
+
+
+def _has_positions(syst):
+    return _is_system(syst) and (hasattr(syst, 'pos_array') or hasattr(syst, 'sites')
+                                 or hasattr(syst, 'pos'))
+
+
+def _tiling_of(hamiltonian, tiles, tile_rows=256, auto_ok=False):
+    """The Tiling for ``tiles``: None/False (no tiles), a Tiling, True (a
+    kwant system tiled by its site positions, each site's orbitals kept
+    together), or 'auto' (True for a kwant system with positions and at least
+    TILES_AUTO_ROWS orbitals when ``auto_ok``, else no tiles)."""
+    if isinstance(tiles, str):
+        if tiles != 'auto':
+            raise ValueError("tiles must be None, False, True, 'auto' or a Tiling")
+        if not (auto_ok and _has_positions(hamiltonian)
+                and int(np.asarray(hamiltonian.site_ranges)[-1][2]) >= TILES_AUTO_ROWS):
+            return None
+        tiles = True
     if tiles is None or tiles is False:
         return None
     if isinstance(tiles, ordering.Tiling):
         return tiles
     if tiles is True:
         syst = hamiltonian
-        if not _is_system(syst) or not (hasattr(syst, 'pos_array') or hasattr(syst, 'sites')
-                                        or hasattr(syst, 'pos')):
+        if not _has_positions(syst):
             raise ValueError("tiles=True needs a Kwant system with site positions; pass "
                              "a kwant_amd.ordering.Tiling for a matrix")
         ranges = np.asarray(syst.site_ranges, dtype=np.int64)
@@ -119,7 +138,7 @@ def _tiling_of(hamiltonian, tiles, tile_rows=256):
             norbs[first:nxt] = n_orb
         return ordering.tile_order(np.repeat(pos.reshape(len(pos), -1), norbs, axis=0),
                                    tile_rows)
-    raise ValueError("tiles must be None, True or a kwant_amd.ordering.Tiling")
+    raise ValueError("tiles must be None, False, True, 'auto' or a Tiling")
 
 
 def _orbitals_of(syst, where):
@@ -468,12 +487,15 @@ class SpectralDensity:
                  vector_factory=None, bounds=None, eps=0.05, rng=None,
                  kernel=None, mean=True, accumulate_vectors=True, *,
                  precision='complex128', max_block_width=64, variant=-1, reorder='auto',
-                 matrix_format='csr', tiles=None):
+                 matrix_format='csr', tiles='auto'):
         if num_moments and energy_resolution:
             raise TypeError("either 'num_moments' or 'energy_resolution' "
                             "must be provided.")
         self.eps = eps
-        tiling = _tiling_of(hamiltonian, tiles)
+        # 'auto' tiles only the plain DOS path with the default row order
+        tiling = _tiling_of(hamiltonian, tiles, auto_ok=(
+            operator is None and matrix_format == 'csr'
+            and isinstance(reorder, str) and reorder == 'auto'))
         hamiltonian = _as_csr(hamiltonian, params)
 
         self._set_operator(operator, params, hamiltonian)

@@ -173,3 +173,32 @@ def test_tiles_true_from_site_positions():
     assert tiled._tiling is not None and tiled._blocks[0].rec.tiles is not None
     plain = kpm.SpectralDensity(syst, reorder=tiled._tiling.order, **kw)
     assert np.array_equal(np.array(tiled._moments_list), np.array(plain._moments_list))
+
+
+def test_tiles_auto_on_large_systems():
+    """The default tiles='auto' tiles a kwant system with site positions from
+    2^18 orbitals on (plain DOS path, default row order) and nothing else: not
+    a smaller system, not an operator run, not an explicit row order.  The
+    moments equal those of the same row order without tiles, bitwise."""
+    from kwant_amd import kpm
+    from kwant_amd.assembly import LatticeSystem
+
+    def square(side):
+        syst = LatticeSystem([(1, 0), (0, 1)], [(0, 0)], [(0, side), (0, side)], norbs=1)
+        syst.set_onsite(lambda pos: 0.3 * np.cos(1.7 * pos[:, 0] + 0.9 * pos[:, 1]))
+        syst.add_hoppings((1, 0), 0, 0, 1.0)
+        syst.add_hoppings((0, 1), 0, 0, 1.0)
+        return syst
+
+    kw = dict(num_moments=24, num_vectors=16, bounds=(-4.5, 4.5), rng=3)
+    big = square(512)
+    assert big.site_ranges[-1][2] == kpm.TILES_AUTO_ROWS
+    auto = kpm.SpectralDensity(big, **kw)
+    assert auto._tiling is not None
+    plain = kpm.SpectralDensity(big, tiles=None, reorder=auto._tiling.order, **kw)
+    assert np.array_equal(np.array(auto._moments_list), np.array(plain._moments_list))
+    assert kpm.SpectralDensity(square(40), **kw)._tiling is None
+    assert kpm.SpectralDensity(big, reorder=False, **kw)._tiling is None
+    dens = np.ones(big.site_ranges[-1][2])
+    assert kpm.SpectralDensity(big, operator=__import__("scipy").sparse.diags(dens),
+                               **kw)._tiling is None

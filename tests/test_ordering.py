@@ -118,11 +118,24 @@ def test_tiling_validation():
     assert t.n_tiles == 2
 
 
-def test_tiles_argument_needs_positions():
+def test_tiles_argument_needs_positions(monkeypatch):
+    """True needs site positions; 'auto' tiles only a kwant system with
+    positions, at least TILES_AUTO_ROWS orbitals and the caller's consent
+    (the plain DOS path with the default row order)."""
     from kwant_amd import kpm
+    from kwant_amd.assembly import LatticeSystem
     h, _ = square(8, disorder=1.0, seed=1)
     with pytest.raises(ValueError, match="site positions"):
         kpm._tiling_of(h, True)
     with pytest.raises(ValueError):
-        kpm._tiling_of(h, 'auto')
+        kpm._tiling_of(h, 'yes')
+    assert kpm._tiling_of(h, 'auto', auto_ok=True) is None      # a matrix
     assert kpm._tiling_of(h, None) is None and kpm._tiling_of(h, False) is None
+    syst = LatticeSystem([(1, 0), (0, 1)], [(0, 0)], [(0, 20), (0, 20)], norbs=2)
+    assert kpm._tiling_of(syst, 'auto', auto_ok=True) is None   # 800 orbitals
+    monkeypatch.setattr(kpm, "TILES_AUTO_ROWS", 800)
+    assert kpm._tiling_of(syst, 'auto', auto_ok=False) is None
+    tiling = kpm._tiling_of(syst, 'auto', auto_ok=True, tile_rows=64)
+    assert tiling is not None and sorted(tiling.order) == list(range(800))
+    # a site's two orbitals stay together and in order
+    assert np.all(tiling.order[1::2] == tiling.order[0::2] + 1)
