@@ -30,7 +30,8 @@ the rows into compact tiles and the plain DOS path runs two Chebyshev steps per
 launch; the moments are bitwise those without tiles.  The default 'auto' tiles
 kwant systems with positions and at least 2^18 orbitals on the plain DOS path
 with the default row order, and each block keeps whichever of the two-step and
-one-step launches its timing picks), ``gemm_precision`` for the Correlator
+one-step launches its timing picks; 'graph' tiles any matrix by hop distances
+in its graph, about 1 s per 10^6 rows of host set-up), ``gemm_precision`` for the Correlator
 ('fp64' | 'fp32' | 'isliced4' | 'isliced5' | 'sliced3' | 'sliced4': the 'isliced'
 modes are the split-precision GEMM on the INT8 matrix cores, 'sliced' on BF16).
 """
@@ -104,14 +105,17 @@ def _has_positions(syst):
                                  or hasattr(syst, 'pos'))
 
 
-def _tiling_of(hamiltonian, tiles, tile_rows=256, auto_ok=False):
+def _tiling_of(hamiltonian, tiles, tile_rows=256, auto_ok=False, csr=None):
     """The Tiling for ``tiles``: None/False (no tiles), a Tiling, True (a
     kwant system tiled by its site positions, each site's orbitals kept
-    together), or 'auto' (True for a kwant system with positions and at least
-    TILES_AUTO_ROWS orbitals when ``auto_ok``, else no tiles)."""
+    together), 'graph' (any matrix, tiled by hop distances in the graph of
+    ``csr``, its CSR form) or 'auto' (True for a kwant system with positions
+    and at least TILES_AUTO_ROWS orbitals when ``auto_ok``, else no tiles)."""
+    if isinstance(tiles, str) and tiles == 'graph':
+        return ordering.tile_order(ordering.graph_coordinates(csr), tile_rows)
     if isinstance(tiles, str):
         if tiles != 'auto':
-            raise ValueError("tiles must be None, False, True, 'auto' or a Tiling")
+            raise ValueError("tiles must be None, False, True, 'auto', 'graph' or a Tiling")
         if not (auto_ok and _has_positions(hamiltonian)
                 and int(np.asarray(hamiltonian.site_ranges)[-1][2]) >= TILES_AUTO_ROWS):
             return None
@@ -138,7 +142,7 @@ def _tiling_of(hamiltonian, tiles, tile_rows=256, auto_ok=False):
             norbs[first:nxt] = n_orb
         return ordering.tile_order(np.repeat(pos.reshape(len(pos), -1), norbs, axis=0),
                                    tile_rows)
-    raise ValueError("tiles must be None, False, True, 'auto' or a Tiling")
+    raise ValueError("tiles must be None, False, True, 'auto', 'graph' or a Tiling")
 
 
 def _orbitals_of(syst, where):
@@ -492,11 +496,12 @@ class SpectralDensity:
             raise TypeError("either 'num_moments' or 'energy_resolution' "
                             "must be provided.")
         self.eps = eps
+        csr = _as_csr(hamiltonian, params)
         # 'auto' tiles only the plain DOS path with the default row order
-        tiling = _tiling_of(hamiltonian, tiles, auto_ok=(
+        tiling = _tiling_of(hamiltonian, tiles, csr=csr, auto_ok=(
             operator is None and matrix_format == 'csr'
             and isinstance(reorder, str) and reorder == 'auto'))
-        hamiltonian = _as_csr(hamiltonian, params)
+        hamiltonian = csr
 
         self._set_operator(operator, params, hamiltonian)
 

@@ -105,6 +105,35 @@ class Tiling:
         return len(self.tile_begin) - 1
 
 
+def graph_coordinates(csr):
+    """Pseudo-coordinates of the rows from the matrix's graph alone: hop
+    distances to two far-apart rows.  Row a is the row farthest from row 0, b
+    the row farthest from a, and c the row whose distance to the nearer of a
+    and b is largest; the coordinates are (d_a, d_c).  On a lattice, hop
+    distances to two such corners grow along two independent directions (the
+    corners (0, 0) and (W, 0) of a rectangle see x + y and W - x + y), so boxes
+    of these coordinates are compact patches of the lattice.  Rows that a
+    cannot reach (other components) get one coordinate past the largest."""
+    from scipy.sparse.csgraph import dijkstra
+    csr = scipy.sparse.csr_matrix(csr)
+    n = csr.shape[0]
+    pattern = scipy.sparse.csr_matrix(
+        (np.ones(len(csr.indices), dtype=np.int8), csr.indices, csr.indptr), shape=csr.shape)
+
+    def dist(row):
+        d = dijkstra(pattern, directed=False, indices=int(row), unweighted=True)
+        finite = np.isfinite(d)
+        d[~finite] = d[finite].max() + 1
+        return d
+
+    if n == 0:
+        return np.zeros((0, 2))
+    da = dist(np.argmax(dist(0)))
+    db = dist(np.argmax(da))
+    dc = dist(np.argmax(np.minimum(da, db)))
+    return np.stack([da, dc], axis=1)
+
+
 def tile_order(coords, tile_rows=512, iterations=4):
     """Tiling of rows with coordinates ``coords`` ((N,) or (N, d) array, one
     point per row -- the site position, repeated for each orbital of a site)

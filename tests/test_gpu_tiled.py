@@ -202,3 +202,17 @@ def test_tiles_auto_on_large_systems():
     dens = np.ones(big.site_ranges[-1][2])
     assert kpm.SpectralDensity(big, operator=__import__("scipy").sparse.diags(dens),
                                **kw)._tiling is None
+
+
+def test_tiles_graph_for_a_matrix():
+    """tiles='graph' runs a plain matrix (no positions) over tiles of its
+    graph's hop distances: two steps per launch, bitwise the moments of the
+    same row order without tiles."""
+    from kwant_amd import kpm
+    from kwant_amd.lattices import honeycomb, honeycomb_bounds
+    h, _ = honeycomb(60, disorder=0.5, seed=4)
+    kw = dict(num_moments=31, num_vectors=16, bounds=honeycomb_bounds(1.0, 0.5), rng=5)
+    tiled = kpm.SpectralDensity(h, tiles='graph', **kw)
+    assert tiled._tiling is not None and tiled._blocks[0].rec.tiles is not None
+    plain = kpm.SpectralDensity(h, reorder=tiled._tiling.order, **kw)
+    assert np.array_equal(np.array(tiled._moments_list), np.array(plain._moments_list))

@@ -139,3 +139,35 @@ def test_tiles_argument_needs_positions(monkeypatch):
     assert tiling is not None and sorted(tiling.order) == list(range(800))
     # a site's two orbitals stay together and in order
     assert np.all(tiling.order[1::2] == tiling.order[0::2] + 1)
+
+
+def test_graph_coordinates_tile_like_positions():
+    """Tiles from hop distances alone (no positions) are about as compact as
+    tiles from the site positions, on the square and honeycomb lattices; a
+    graph of two components still tiles every row."""
+    for h, pos in (square(48, disorder=1.0, seed=1), honeycomb(40, disorder=0.5, seed=2)):
+        n = h.shape[0]
+        coords = ordering.graph_coordinates(h)
+        assert coords.shape == (n, 2)
+        by_pos = ordering.tile_order(pos, 256)
+        by_graph = ordering.tile_order(coords, 256)
+        assert sorted(by_graph.order) == list(range(n))
+        halo_pos = _halo_rows(ordering.permute_csr(h, by_pos.order), by_pos.tile_begin)
+        halo_graph = _halo_rows(ordering.permute_csr(h, by_graph.order), by_graph.tile_begin)
+        assert halo_graph < 1.3 * halo_pos, (halo_graph, halo_pos)
+    a, _ = square(10, disorder=1.0, seed=3)
+    b, _ = square(7, disorder=1.0, seed=4)
+    two = sp.block_diag([a, b]).tocsr()
+    coords = ordering.graph_coordinates(two)
+    assert np.all(np.isfinite(coords))
+    tiling = ordering.tile_order(coords, 32)
+    assert sorted(tiling.order) == list(range(two.shape[0]))
+
+
+def test_tiles_graph_for_a_matrix():
+    """tiles='graph' tiles a plain matrix by its graph."""
+    from kwant_amd import kpm
+    h, _ = square(30, disorder=1.0, seed=1)
+    tiling = kpm._tiling_of(h, 'graph', csr=sp.csr_matrix(h), tile_rows=128)
+    assert sorted(tiling.order) == list(range(h.shape[0]))
+    assert np.diff(tiling.tile_begin).max() <= 128
