@@ -561,6 +561,73 @@ __global__ __launch_bounds__(WAVES * 64) void spmm_kernel(const int32_t* indptr,
   IO::store(ys + 2 * (int64_t(row) * W + col), acc);
 }
 
+// The same product for SG consecutive slots per workgroup: a row's entries are
+// loaded once and applied to every slot, so the matrix crosses HBM once per SG
+// slots instead of once per slot.  Each (row, slot) sum is spmm_kernel's
+// sequence of FMAs (padding entries included), so the results are bitwise
+// equal.
+template <typename T, int VEC, int LPR, int WAVES, int SG>
+__global__ __launch_bounds__(WAVES * 64) void spmm_slots_kernel(
+    const int32_t* indptr, const int32_t* indices, const void* values, const T* x,
+    int64_t x_stride, T* y, int64_t y_stride, int n_rows, int n_slots) {
+  constexpr int W = LPR * VEC;
+  constexpr int RPW = 64 / LPR;
+  using IO = LaneIO<T, VEC>;
+  using CL = CLane<T, VEC>;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int row = (blockIdx.x * WAVES + wave) * RPW + lane / LPR;
+  const int col = (lane % LPR) * VEC;
+  if (row >= n_rows) return;
+  const int s0 = blockIdx.y * SG;
+  const int ns = n_slots - s0 < SG ? n_slots - s0 : SG;
+  const T* xs = x + 2 * int64_t(s0) * x_stride;
+  T* ys = y + 2 * int64_t(s0) * y_stride;
+  const int k0 = indptr[row], k1 = indptr[row + 1];
+  CL acc[SG];
+#pragma unroll
+  for (int s = 0; s < SG; ++s) zero(acc[s]);
+  for (int k = k0; k < k1; k += 4) {
+    T hr[4], hi[4];
+    int j[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      hr[u] = hi[u] = T(0);
+      j[u] = -1;
+      if (k + u < k1) {
+        j[u] = indices[k + u];
+        load_value<T>(values, k + u, hr[u], hi[u]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < SG; ++s) {
+      if (s < ns) {
+        CL xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (j[u] >= 0)
+            xv[u] = IO::load(xs + 2 * (int64_t(s) * x_stride + int64_t(j[u]) * W + col));
+          else
+            zero(xv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            acc[s].re[v] = fma(hr[u], xv[u].re[v], acc[s].re[v]);
+            acc[s].re[v] = fma(-hi[u], xv[u].im[v], acc[s].re[v]);
+            acc[s].im[v] = fma(hr[u], xv[u].im[v], acc[s].im[v]);
+            acc[s].im[v] = fma(hi[u], xv[u].re[v], acc[s].im[v]);
+          }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < SG; ++s)
+    if (s < ns) IO::store(ys + 2 * (int64_t(s) * y_stride + int64_t(row) * W + col), acc[s]);
+}
+
 // Bond records of a current-like operator (kwant.operator.Current,
 // kwant/operator.pyx:940-1001): for every entry k of the bond matrix B (row
 // r, column s = indices[k], value h = values[k]) and every block column c,
@@ -766,6 +833,15 @@ template <typename T, int VEC, int LPR>
 int launch_spmm_one(const kpm_csr* op, const void* x, int64_t xs, void* y, int64_t ys,
                     int n_slots, hipStream_t stream) {
   constexpr int RPB = kWaves * 64 / LPR;
+  // slots per workgroup of the multi-slot kernel
+  constexpr int SG = 8;
+  if (n_slots >= SG) {
+    dim3 grid((op->n_rows + RPB - 1) / RPB, (n_slots + SG - 1) / SG), block(kWaves * 64);
+    hipLaunchKernelGGL((spmm_slots_kernel<T, VEC, LPR, kWaves, SG>), grid, block, 0, stream,
+                       op->indptr, op->indices, op->values, reinterpret_cast<const T*>(x), xs,
+                       reinterpret_cast<T*>(y), ys, op->n_rows, n_slots);
+    return check_hip(hipGetLastError(), "spmm_slots_kernel launch");
+  }
   dim3 grid((op->n_rows + RPB - 1) / RPB, n_slots), block(kWaves * 64);
   hipLaunchKernelGGL((spmm_kernel<T, VEC, LPR, kWaves>), grid, block, 0, stream, op->indptr,
                      op->indices, op->values, reinterpret_cast<const T*>(x), xs,
