@@ -1039,8 +1039,10 @@ def main():
                                  eng.DOTS_DOUBLING, variant=args.variant)
         mom64 = moments_from_dots(r64.step(full))
         del r64
+        from kwant_amd import autotune
         m128 = eng.DeviceCSR(h, "complex128")
-        r128 = eng.BlockRecursion(m128, (a, b), block, width, eng.DOTS_DOUBLING)
+        v128 = autotune.pick_variant(m128, (a, b), width, eng.DOTS_DOUBLING, force=True)
+        r128 = eng.BlockRecursion(m128, (a, b), block, width, eng.DOTS_DOUBLING, variant=v128)
         torch.cuda.synchronize()
         s0 = time.perf_counter()
         dots128 = r128.step(full)
@@ -1059,9 +1061,9 @@ def main():
         check = {"moments_c64_vs_c128_max_err_over_mu0": err,
                  "dos_l1_c64_vs_c128": l1,
                  "num_moments": 2 * full,
-                 "c128_steps_per_s": full / t128}
-        log("c64 vs c128 moments: max |d mu| / mu0 = %.3e, DOS L1 %.1e; c128 %.0f steps/s"
-            % (err, l1, full / t128))
+                 "c128_steps_per_s": full / t128, "c128_variant": v128}
+        log("c64 vs c128 moments: max |d mu| / mu0 = %.3e, DOS L1 %.1e; c128 %.0f steps/s "
+            "(variant %d)" % (err, l1, full / t128, v128))
         if args.tile_rows > 0:
             c128_two = two_step_bench(h, pos, (a, b), block, width, args.warmup, 256,
                                       args.tile_rows, -1, -1, log, precision="complex128")
