@@ -29,7 +29,8 @@ True to tile a kwant system by its site positions: the device row order groups
 the rows into compact tiles and the plain DOS path runs two Chebyshev steps per
 launch; the moments are bitwise those without tiles.  The default 'auto' tiles
 kwant systems with positions and at least 2^18 orbitals on the plain DOS path
-with the default row order, and each block keeps whichever of the two-step and
+with the default row order and random vectors, when orbitals x vectors x
+num_moments / 2 reaches 2^32, and each block keeps whichever of the two-step and
 one-step launches its timing picks; 'graph' tiles any matrix by hop distances
 in its graph, about 1 s per 10^6 rows of host set-up), ``gemm_precision`` for the Correlator
 ('fp64' | 'fp32' | 'isliced4' | 'isliced5' | 'sliced3' | 'sliced4': the 'isliced'
@@ -96,8 +97,11 @@ def _as_csr(hamiltonian, params=None):
         raise ValueError("'hamiltonian' is neither a matrix nor a Kwant system.")
 
 
-# 'auto' tiles kwant systems with site positions from this many orbitals on
+# 'auto' tiles kwant systems with site positions from this many orbitals on,
+# when the expansion asked for is at least this many orbital x vector x step
+# products (the tiling and its timing cost about a second at 2 10^6 orbitals)
 TILES_AUTO_ROWS = 1 << 18
+TILES_AUTO_WORK = 1 << 32
 
 
 def _has_positions(syst):
@@ -497,9 +501,12 @@ class SpectralDensity:
                             "must be provided.")
         self.eps = eps
         csr = _as_csr(hamiltonian, params)
-        # 'auto' tiles only the plain DOS path with the default row order
+        # 'auto' tiles only the plain DOS path with the default row order, and
+        # only long expansions (the default 100 moments x 10 vectors is not)
+        work = csr.shape[0] * (num_vectors or 0) * ((num_moments or 100) // 2)
         tiling = _tiling_of(hamiltonian, tiles, csr=csr, auto_ok=(
-            operator is None and matrix_format == 'csr'
+            operator is None and matrix_format == 'csr' and vector_factory is None
+            and not energy_resolution and work >= TILES_AUTO_WORK
             and isinstance(reorder, str) and reorder == 'auto'))
         hamiltonian = csr
 

@@ -175,11 +175,12 @@ def test_tiles_true_from_site_positions():
     assert np.array_equal(np.array(tiled._moments_list), np.array(plain._moments_list))
 
 
-def test_tiles_auto_on_large_systems():
+def test_tiles_auto_on_large_systems(monkeypatch):
     """The default tiles='auto' tiles a kwant system with site positions from
-    2^18 orbitals on (plain DOS path, default row order) and nothing else: not
-    a smaller system, not an operator run, not an explicit row order.  The
-    moments equal those of the same row order without tiles, bitwise."""
+    2^18 orbitals on (plain DOS path, default row order, a long enough
+    expansion) and nothing else: not a smaller system, not an operator run, not
+    an explicit row order, not a short expansion.  The moments equal those of
+    the same row order without tiles, bitwise."""
     from kwant_amd import kpm
     from kwant_amd.assembly import LatticeSystem
 
@@ -193,6 +194,9 @@ def test_tiles_auto_on_large_systems():
     kw = dict(num_moments=24, num_vectors=16, bounds=(-4.5, 4.5), rng=3)
     big = square(512)
     assert big.site_ranges[-1][2] == kpm.TILES_AUTO_ROWS
+    # 2^18 x 16 x 12 orbital-vector-steps: below the default work threshold
+    assert kpm.SpectralDensity(big, **kw)._tiling is None
+    monkeypatch.setattr(kpm, "TILES_AUTO_WORK", 1 << 20)
     auto = kpm.SpectralDensity(big, **kw)
     assert auto._tiling is not None
     plain = kpm.SpectralDensity(big, tiles=None, reorder=auto._tiling.order, **kw)
