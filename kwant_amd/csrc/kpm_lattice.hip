@@ -58,6 +58,10 @@ __host__ __device__ inline int64_t cubic4_entries_before(int64_t L, int64_t x, i
 // On-site entry (o, j) of site s: (X + X^H)_oj / 2 with X_ij = sigma (g_re + i g_im).
 __device__ inline void cubic4_onsite(uint64_t seed, double sigma, int64_t s, int o, int j,
                                      double& ar, double& ai) {
+  // no FMA contraction: every product is rounded on its own, so (o, j) and
+  // (j, o) add and subtract the same two numbers and the block is Hermitian
+  // bit for bit (A[j, o] = conj A[o, j], Im A[o, o] = +0), as the oracle's
+#pragma clang fp contract(off)
   const uint64_t kij = uint64_t(16 * s + 4 * o + j) * 2;
   const uint64_t kji = uint64_t(16 * s + 4 * j + o) * 2;
   const double xr = sigma * std_normal(seed, kij), xi = sigma * std_normal(seed, kij + 1);

@@ -32,6 +32,10 @@ def test_gen_cubic4_matches_oracle():
     assert np.array_equal(indptr.cpu().numpy(), big.indptr)
     assert np.array_equal(indices.cpu().numpy(), big.indices)
     assert np.max(np.abs(values.cpu().numpy() - big.data)) < 1e-14
+    # Hermitian bit for bit (no FMA contraction in the on-site formula)
+    gpu = sp.csr_matrix((values.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()),
+                        shape=big.shape)
+    assert (gpu - gpu.conj().T).count_nonzero() == 0
     rad = radius.cpu().numpy().view(np.float32)
     for i, m in enumerate(mats):
         g = lo.gershgorin_radius(m)
