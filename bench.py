@@ -535,8 +535,12 @@ def density_bench(mat, host_h, ab, block, width, variant, dos_ms, steps, log_fn)
     the copy to the host are outside the timed region."""
     import torch
     from kwant_amd import _engine as eng
+    from kwant_amd import autotune
     n, nnz = mat.shape[0], mat.nnz
     w = block.clone()
+    # the launch shape SpectralDensity would tune for this mode (variant < 0)
+    if variant is None or variant < 0:
+        variant = autotune.pick_variant(mat, ab, width, eng.DOTS_LOCAL, force=True)
     rec = eng.BlockRecursion(mat, ab, block, width, eng.DOTS_LOCAL, w=w, variant=variant)
     chunk = 64
     rec.local_steps(chunk)
@@ -1120,7 +1124,7 @@ def main():
             % (ms_step, steps_s, 100 * achieved / HBM_PEAK_GBS))
     site_density = None
     if not args.no_density:
-        site_density = density_bench(m64, h, (a, b), block64, width, args.variant,
+        site_density = density_bench(m64, h, (a, b), block64, width, -1,
                                      one_step["ms_per_step"], 256, log)
     del m64, block64
     torch.cuda.empty_cache()
